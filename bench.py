@@ -1,0 +1,129 @@
+"""Benchmark: FP64 cell-updates/s of the HIP shallow-water step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload: the 4096x4096 Cartesian grid (dx = 10 m, seeded beach + 64 bumps +
+noise, west Flather M2 tide, Manning 0.025, f = 1e-4, CFL 0.45), state resident
+in HBM, self-timed dt (maxDt fused into every step), one step = both SSP-RK2
+stages + friction + CFL reduction, replayed from a HIP graph.  Prints one JSON
+line (rank 0).  --gpus above 1 is refused: this build runs on one GPU.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "water-simulator_amd"))
+
+METRIC = "FP64 cell-updates/s + % HBM peak; batched vs plain; adjoint/forward time"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(seconds=12.0):
+    """The CPU oracle (single-threaded C restatement of the reference step) on a
+    256x256 sample of the same workload recipe, timed on this host."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import swe_amd
+    import workloads
+    st, p = workloads.large_grid(nx=256, ny=256, dx=p_dx())
+    rows = swe_amd.geometry_cartesian(256, 256, p["dx"], coriolis=p["coriolis"])
+    o = oracle.OracleSim(256, 256, rows, manning=p["manning"], coriolis=p["coriolis"], cfl=p["cfl"],
+                         dx=p["dx"], dy=p["dx"])
+    o.load_state(*st)
+    o.set_bc(p["bc"])
+    ext = [1.0, 0, 0, 0, 0, 0, 0, 0]
+    o.step(None, ext)
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        o.run(5, ext)
+        n += 5
+    el = time.perf_counter() - t0
+    return dict(value=256 * 256 * n / el, unit="cell-updates/s", cores=1, kind="port",
+                sample=f"oracle C restatement, 256x256 sample of the 4096x4096 recipe, {n} self-timed steps in {el:.1f} s")
+
+
+def p_dx():
+    return 10.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--chunk", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+    import swe_amd
+    import workloads
+
+    log("[bench] building 4096x4096 workload")
+    st, p = workloads.large_grid()
+    nx, ny = p["nx"], p["ny"]
+    rows = swe_amd.geometry_cartesian(nx, ny, p["dx"], coriolis=p["coriolis"])
+    sim = swe_amd.Solver(nx, ny, rows, manning=p["manning"], coriolis=p["coriolis"], cfl=p["cfl"],
+                         dx=p["dx"], dy=p["dx"])
+    sim.load_state(*st)
+    del st
+    sim.set_bc(p["bc"])
+    sim.set_harmonic(0, amp=[p["tide_amp"]], omega=[p["tide_omega"]], phase=[0.0])
+    if args.chunk:
+        sim.set_chunk_rows(args.chunk)
+    sim.use_graph(True)
+    log(f"[bench] warmup {args.warmup} steps")
+    sim.advance(args.warmup)
+    sim.sync()
+    log(f"[bench] timing {args.steps} steps")
+    t0 = time.perf_counter()
+    n = sim.advance(args.steps)
+    sim.sync()
+    el = time.perf_counter() - t0
+    ev_ms = sim.last_advance_ms()
+    if n != args.steps:
+        raise RuntimeError(f"only {n} of {args.steps} steps ran")
+    value = nx * ny * n / el
+    log(f"[bench] {value:.4e} cell-updates/s ({1e3 * el / n:.3f} ms/step; events {ev_ms / n:.3f} ms/step)")
+    ms = sim.profile_step(args.profile_steps)
+    W, H = nx + 4, ny + 4
+    bytes_s2 = 80.0 * W * H            # reads A(h,hu,hv) + B(h,hu,hv) + b, writes C(h,hu,hv)
+    achieved = bytes_s2 / (ms[1] * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_k_sweep_stage2_4096.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    finite = sim.get_scalar(swe_amd.S_NONFINITE) == 0
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": n,
+        "warmup": args.warmup, "ms_per_step": 1e3 * el / n, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded SplitMix64 bed), state resident in HBM",
+        "config": {"workload": "large_grid_4096x4096_dx10_flather_tide_manning0.025_f1e-4_cfl0.45",
+                   "nx": nx, "ny": ny, "dt": "self-timed (maxDt each step)", "graph": True,
+                   "chunk_rows": args.chunk or "default"},
+        "hbm_pct_algorithmic_56B": 100.0 * 56.0 * value / (HBM_PEAK_GBS * 1e9),
+        "kernel_ms": {"stage1": ms[0], "stage2": ms[1], "finalize": ms[2]},
+        "roofline": {"bound": "hbm", "kernel": "k_sweep<stage2>", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "finite": finite,
+    }
+    if not args.no_cpu:
+        log("[bench] cpu baseline")
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,136 @@
+/*
+ * swe_amd.h -- C ABI of the MI355X (gfx950) shallow-water time step.
+ *
+ * This is the drop-in boundary for ShallowWater of src/swe.mjs in
+ * Syntaxswine/water-simulator.  The reference has no FFI: its "operator API" is
+ * the JS class.  Each entry point below replaces one method or one hot loop of
+ * that class (file:line of the reference in the comment); the N-API addon and
+ * the JS module in water-simulator_amd/js bind them one to one.
+ *
+ * Conventions: every function returns 0 (SWE_OK) or a negative SWE_E* code and
+ * never throws; swe_last_error() gives the text.  Arrays are host pointers to
+ * row-major doubles of W*H = (nx+4)*(ny+4) cells (the reference's layout,
+ * swe.mjs:196-199) or H per-row values.  Calls are synchronous: when one
+ * returns, its results are in the host buffers it was given.  A context is not
+ * thread-safe; one stream per context.
+ */
+#ifndef SWE_AMD_H
+#define SWE_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWE_OK 0
+#define SWE_EINVAL (-1)
+#define SWE_EHIP (-2)
+#define SWE_ENOMEM (-3)
+#define SWE_ESTATE (-4)
+
+enum swe_kind { SWE_CARTESIAN = 0, SWE_SPHERE = 1 };
+enum swe_limiter { SWE_LIM_MINMOD = 0, SWE_LIM_MC = 1, SWE_LIM_CENTRAL = 2 };
+/* boundary kinds: reflect swe.mjs:1026, outflow :1042, periodic :1119, flather :1084;
+ * STORED = ghost cells supplied by the caller (an unrecognised JS boundary). */
+enum swe_bc { SWE_BC_REFLECT = 0, SWE_BC_OUTFLOW = 1, SWE_BC_PERIODIC = 2, SWE_BC_FLATHER = 3,
+              SWE_BC_STORED = 4 };
+enum swe_side { SWE_WEST = 0, SWE_EAST = 1, SWE_SOUTH = 2, SWE_NORTH = 3 };
+/* fields (W*H doubles each) */
+enum swe_field { SWE_F_B = 0, SWE_F_H = 1, SWE_F_HU = 2, SWE_F_HV = 3, SWE_F_MANNING = 4,
+                 SWE_F_RH = 5, SWE_F_RHU = 6, SWE_F_RHV = 7, SWE_F_THETA = 8, SWE_F_SPONGE = 9 };
+/* geometry rows (H doubles each), the arrays of geometry.mjs:301-309 the step reads */
+enum swe_row { SWE_ROW_DX = 0, SWE_ROW_DYN = 1, SWE_ROW_DYS = 2, SWE_ROW_DYCFL = 3,
+               SWE_ROW_BEDPHI = 4, SWE_ROW_TANPHI = 5, SWE_ROW_F = 6, SWE_ROW_GEO = 7,
+               SWE_ROW_AREA = 8, SWE_NROWS = 9 };
+/* scalars settable after creation */
+enum swe_scalar { SWE_S_MANNING = 0, SWE_S_CORIOLIS = 1, SWE_S_CFL = 2, SWE_S_T = 3,
+                  SWE_S_SPONGE_REF = 4, SWE_S_STEPS = 5, SWE_S_MASS_FLOORED = 6,
+                  SWE_S_DT_NEXT = 7, SWE_S_NONFINITE = 8 };
+/* dt policy of swe_advance */
+enum swe_dt_mode { SWE_DT_SELF = 0, SWE_DT_FIXED = 1, SWE_DT_UNTIL = 2 };
+
+typedef struct swe_desc {
+  int32_t nx, ny;           /* interior cells */
+  int32_t kind;             /* swe_kind */
+  int32_t order;            /* 1 or 2 */
+  int32_t limiter;          /* swe_limiter */
+  int32_t reserved;
+  double g;                 /* gravity; the reference's G = 9.80665 (swe.mjs:85) */
+  double min_depth;         /* minDepth (swe.mjs:172) */
+  double eps4;              /* max(minDepth,1e-12)**4 as the host engine computed it (swe.mjs:190) */
+  double cfl;
+  double coriolis;          /* Cartesian f (swe.mjs:172) */
+  double manning;           /* scalar n */
+  double R;                 /* sphere radius (geometry.mjs:191); ignored for Cartesian */
+  const double *rows[SWE_NROWS];   /* host rows, H each; AREA may be NULL */
+} swe_desc;
+
+typedef struct swe_ctx swe_ctx;
+
+/* new ShallowWater(opts) minus the closure sampling (swe.mjs:170-244): allocates
+ * device buffers for a W*H grid and uploads the geometry rows. */
+int swe_create(const swe_desc *desc, swe_ctx **out);
+void swe_destroy(swe_ctx *ctx);
+const char *swe_last_error(const swe_ctx *ctx);
+/* x**y for finite x > 0 exactly as the device computes it (fdlibm e_pow, the
+ * algorithm behind the JS engine's Math.pow to within 1 ulp); the host uses it
+ * for eps4 = max(minDepth,1e-12)**4 (swe.mjs:190). */
+double swe_js_pow(double x, double y);
+/* version string and the gfx target the library was built for */
+const char *swe_build_info(void);
+
+/* Host <-> device copies of whole fields (the JS object's Float64Array mirrors).
+ * SWE_F_MANNING: setManningField (swe.mjs:713-731); NULL returns to the scalar.
+ * SWE_F_SPONGE: per-cell makeSponge strengths (swe.mjs:1174-1186); NULL removes it. */
+int swe_set_field(swe_ctx *ctx, int field, const double *host);
+int swe_get_field(swe_ctx *ctx, int field, double *host);
+/* replace one geometry row (verify-sphere mutates sim.geom between steps) */
+int swe_set_row(swe_ctx *ctx, int row, const double *host);
+int swe_set_bc(swe_ctx *ctx, int side, int kind);
+int swe_set_scalar(swe_ctx *ctx, int which, double value);
+int swe_get_scalar(swe_ctx *ctx, int which, double *value);
+
+/* applyBC() (swe.mjs:289-303). ext[2*side], ext[2*side+1] = etaExt(t), uExt(t) of a
+ * flather side, evaluated by the caller at the current t. */
+int swe_apply_bc(swe_ctx *ctx, const double ext[8]);
+/* residual(dt) (swe.mjs:308-629) on the stored state, ghost cells as stored;
+ * fills SWE_F_RH/RHU/RHV/THETA. */
+int swe_residual(swe_ctx *ctx, double dt);
+/* maxDt() (swe.mjs:632-647); +inf when every interior cell is dry */
+int swe_max_dt(swe_ctx *ctx, double *out);
+/* step(dt) (swe.mjs:754-814) without the host forcing hook: dt NaN = self-timed.
+ * *dt_taken = 0 when the step was refused (dt not > 0 or not finite).
+ * want_theta/want_rh: also fill THETA / RH* with stage 2's values (what the
+ * reference leaves in sim.theta / sim.rh after a step). */
+int swe_step(swe_ctx *ctx, double dt, const double ext[8], int want_theta, int want_rh,
+             double *dt_taken, double *mass_floored);
+/* Device-resident run of nsteps steps (no host round trip): dt policy SELF
+ * (maxDt every step), FIXED (dt_value) or UNTIL (min(maxDt, t_end - t)); the
+ * flather forcing comes from swe_set_harmonic.  Stops early once a step is
+ * refused. */
+int swe_advance(swe_ctx *ctx, int64_t nsteps, int dt_mode, double dt_value, double t_end,
+                int64_t *steps_taken);
+/* Flather forcing for swe_advance: eta(t) = msl + sum amp_k cos(omega_k t - phase_k),
+ * u(t) = u0 + sum uamp_k cos(uomega_k t - uphase_k), k < 4 (Tide.eta, tide.mjs:103-107). */
+int swe_set_harmonic(swe_ctx *ctx, int side, int n_eta, double msl, const double *amp,
+                     const double *omega, const double *phase, int n_u, double u0,
+                     const double *uamp, const double *uomega, const double *uphase);
+/* capture one device-resident step in a HIP graph and replay it (1 = on) */
+int swe_use_graph(swe_ctx *ctx, int on);
+/* rows of grid per wavefront chunk of the fused stage kernel (autotuned; 0 = default) */
+int swe_set_chunk_rows(swe_ctx *ctx, int rows);
+/* Measurement hook (no reference counterpart): run nsteps self-timed device
+ * steps with HIP events around each kernel on the context's stream and return
+ * the mean duration in ms of [stage-1 sweep, stage-2 sweep, finalize]. */
+int swe_profile_step(swe_ctx *ctx, int nsteps, double out_ms[3]);
+/* block until all work on the context's stream is done */
+int swe_sync(swe_ctx *ctx);
+/* device pointer of a field of the current state (for zero-copy callers) */
+int swe_device_ptr(swe_ctx *ctx, int field, double **dptr);
+/* elapsed ms of the last swe_advance measured with HIP events on the context's stream */
+int swe_last_advance_ms(swe_ctx *ctx, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

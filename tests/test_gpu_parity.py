@@ -1,0 +1,185 @@
+"""Parity of the HIP step (through the C ABI) with the reference and the oracle.
+
+Bar: bit-exact.  Every golden scenario is replayed on the GPU and must match the
+reference solver's fingerprints exactly wherever the oracle does, and must match
+the oracle itself bit for bit at every step in all scenarios (friction included:
+both use the same pow).  Seeded random basins of several shapes -- crossing the
+58-column strip and row-chunk boundaries of the fused kernel -- are compared
+with the oracle bit for bit over 25 steps for every boundary kind, limiter,
+order, friction mode and the sponge.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import goldens
+import oracle
+import swe_amd
+from gpu_util import bits, first_diff, pair
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_factory(nx, ny, rows, **kw):
+    kw.pop("use_manning_field", None)
+    return swe_amd.Solver(nx, ny, rows, **kw)
+
+
+@pytest.mark.parametrize("name", goldens.STEP_SCENARIOS)
+def test_gpu_replays_reference(name):
+    say(f"[gpu] replay {name}")
+    fx = goldens.load(name)
+    sim = goldens.make_sim(gpu_factory, fx)
+    fr = goldens.uses_friction(fx)
+    problems = goldens.replay(sim, fx, exact=not fr, rtol=1e-13)
+    assert not problems, problems
+
+
+@pytest.mark.parametrize("name", goldens.STEP_SCENARIOS)
+def test_gpu_matches_oracle_every_step(name):
+    say(f"[gpu] vs oracle {name}")
+    fx = goldens.load(name)
+    g = goldens.make_sim(gpu_factory, fx)
+    o = goldens.make_sim(oracle.OracleSim, fx)
+    sponge = fx.get("sponge")
+    W = fx["params"]["nx"] + 4
+    for n, st in enumerate(fx["steps"], start=1):
+        ext = [goldens.hexf(x) for x in st["ext"]]
+        dt_arg = None if st["dtArg"] is None else goldens.hexf(st["dtArg"])
+        dg = g.step(dt_arg, ext)
+        do = o.step(dt_arg, ext)
+        if sponge and do > 0:
+            o.sponge(do, goldens.hexf(sponge["etaRef"]))
+        if not st["finite"]:
+            assert not g.finite()
+            break
+        assert dg == do, (n, dg, do)
+        _, h, hu, hv = g.state()
+        for nm, a, b in (("h", h, o.h), ("hu", hu, o.hu), ("hv", hv, o.hv)):
+            assert first_diff(a, b, W) is None, (n, nm, first_diff(a, b, W))
+        assert g.mass_floored == o.mass_floored or abs(g.mass_floored - o.mass_floored) <= 1e-15 * abs(o.mass_floored)
+
+
+@pytest.mark.parametrize("name", ["probe_positivity", "probe_sphere"])
+def test_gpu_residual_probes(name):
+    say(f"[gpu] probe {name}")
+    fx = goldens.load(name)
+    for pr in fx["probes"]:
+        sim = goldens.make_sim(gpu_factory, {**fx, "manningField": None, "sponge": None})
+        assert sim.max_dt() == goldens.hexf(fx["maxDt"])
+        sim.apply_bc()
+        b, h, hu, hv = sim.state()
+        for key, a in (("b", b), ("h", h), ("hu", hu), ("hv", hv)):
+            assert np.array_equal(bits(a), bits(goldens.arr(pr["after_bc"][key]))), key
+        sim.residual(goldens.hexf(pr["dt"]))
+        for f, key in ((swe_amd.F_RH, "rh"), (swe_amd.F_RHU, "rhu"), (swe_amd.F_RHV, "rhv"),
+                       (swe_amd.F_THETA, "theta")):
+            got, want = sim.get_field(f), goldens.arr(pr[key])
+            W = fx["params"]["nx"] + 4
+            assert first_diff(got, want, W) is None, (pr["mult"], key, first_diff(got, want, W))
+
+
+def test_gpu_stage2_theta_and_rh_after_step():
+    say("[gpu] theta/rh after step")
+    for name in ("lake_island", "dambreak_dry", "sphere_pole"):
+        fx = goldens.load(name)
+        sim = goldens.make_sim(gpu_factory, fx)
+        st = fx["steps"][0]
+        sim.step(None, [goldens.hexf(x) for x in st["ext"]], want_theta=True, want_rh=True)
+        cp = fx["checkpoints"]["1"]
+        W = fx["params"]["nx"] + 4
+        for f, key in ((swe_amd.F_THETA, "theta"), (swe_amd.F_RH, "rh"), (swe_amd.F_RHU, "rhu"),
+                       (swe_amd.F_RHV, "rhv"), (swe_amd.F_B, "b")):
+            got = sim.get_field(f)
+            assert first_diff(got, goldens.arr(cp[key]), W) is None, (name, key)
+
+
+CASES = [
+    dict(nx=100, ny=70, bc=("reflect",) * 4),
+    dict(nx=131, ny=61, bc=("outflow", "reflect", "periodic", "periodic")),
+    dict(nx=57, ny=150, bc=("periodic", "periodic", "reflect", "outflow"), limiter="minmod"),
+    dict(nx=120, ny=40, bc=("flather", "flather", "reflect", "flather"), manning=0.03),
+    dict(nx=117, ny=33, bc=("reflect",) * 4, order=1, coriolis=2e-3),
+    dict(nx=64, ny=64, bc=("periodic",) * 4, limiter="central", wetdry=False),
+    dict(nx=90, ny=45, bc=("reflect", "outflow", "reflect", "reflect"), manning_field=True, sponge=True),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_gpu_random_basins_bit_exact(case):
+    kw = dict(CASES[case])
+    nx, ny = kw.pop("nx"), kw.pop("ny")
+    say(f"[gpu] random basin {case} {nx}x{ny} {kw}")
+    g, o = pair(nx, ny, 1000 + case, **kw)
+    W = nx + 4
+    ext = [0.3, 0.02, -0.1, 0.0, 0.0, 0.0, 0.05, -0.01]
+    for chunk in (None,):
+        pass
+    for n in range(25):
+        dg = g.step(None, ext)
+        do = o.step(None, ext)
+        if kw.get("sponge"):
+            o.sponge(do, 0.0)
+        assert dg == do, (n, dg, do)
+        _, h, hu, hv = g.state()
+        for nm, a, b in (("h", h, o.h), ("hu", hu, o.hu), ("hv", hv, o.hv)):
+            assert first_diff(a, b, W) is None, (n, nm, first_diff(a, b, W))
+
+
+@pytest.mark.parametrize("rows", [8, 13, 40, 1000])
+def test_gpu_chunk_rows_do_not_change_results(rows):
+    say(f"[gpu] chunk rows {rows}")
+    g1, _ = pair(150, 90, 7)
+    g2, _ = pair(150, 90, 7)
+    g2.set_chunk_rows(rows)
+    for _ in range(10):
+        g1.step(None)
+        g2.step(None)
+    for a, b in zip(g1.state(), g2.state()):
+        assert np.array_equal(bits(a), bits(b))
+
+
+def test_gpu_advance_and_graph_match_strict_steps():
+    say("[gpu] advance / graph")
+    g1, _ = pair(200, 120, 11, bc=("flather", "reflect", "reflect", "reflect"), manning=0.025)
+    g2, _ = pair(200, 120, 11, bc=("flather", "reflect", "reflect", "reflect"), manning=0.025)
+    g3, _ = pair(200, 120, 11, bc=("flather", "reflect", "reflect", "reflect"), manning=0.025)
+    for g in (g2, g3):
+        g.set_harmonic(0, amp=[0.0], omega=[0.0], phase=[0.0], msl=0.0)
+    g3.use_graph(True)
+    for _ in range(9):
+        g1.step(None, [0.0] * 8)
+    assert g2.advance(9) == 9
+    assert g3.advance(9) == 9
+    for a, b, c in zip(g1.state(), g2.state(), g3.state()):
+        assert np.array_equal(bits(a), bits(b))
+        assert np.array_equal(bits(a), bits(c))
+    assert g1.t == g2.t == g3.t
+
+
+def test_gpu_refused_step_keeps_state():
+    say("[gpu] refused step")
+    g, o = pair(40, 30, 3)
+    before = g.state()
+    assert g.step(0.0) == 0.0
+    assert g.step(math.inf) == 0.0
+    o.apply_bc()
+    after = g.state()
+    for a, b in zip(after[1:], (o.h, o.hu, o.hv)):
+        assert np.array_equal(bits(a), bits(b))
+    assert np.array_equal(bits(after[0]), bits(o.b))
+    _ = before
+
+
+def test_gpu_all_dry_max_dt_is_infinite():
+    rows = swe_amd.geometry_cartesian(20, 10, 1.0)
+    g = swe_amd.Solver(20, 10, rows, manning=0.0)
+    assert g.max_dt() == math.inf
+    assert g.step(None) == 0.0

@@ -1,0 +1,632 @@
+// C ABI (include/swe_amd.h) over the HIP kernels of swe_kernels.hip.
+// A context owns the device buffers of one basin and one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/swe_amd.h"
+#include "swe_device.hpp"
+#include "swe_kernels.hpp"
+
+namespace swe {
+hipError_t launch_sweep(int mode, const SweepArgs &a, hipStream_t st);
+int sweep_waves(const Params &P, int rows_per_chunk);
+hipError_t launch_apply_bc(const Params &P, StateW S, double *b, const Ctl *ctl, hipStream_t st);
+hipError_t launch_maxdt(const Params &P, StateP S, Ctl *ctl, hipStream_t st);
+hipError_t launch_finalize(Ctl *ctl, const double *mf_part, int nparts, double cfl, int phase, hipStream_t st);
+}  // namespace swe
+
+using namespace swe;
+
+namespace {
+constexpr int MIN_CHUNK = 8;
+enum { R_DX, R_RDX, R_DYN, R_RDYN, R_DYS, R_RDYS, R_DYCFL, R_RDYCFL, R_BED, R_RBED, R_TAN, R_F, R_GEO,
+       R_AREA, R_COUNT };
+}
+
+struct swe_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Params P{};
+  size_t N = 0;
+  double *rows[R_COUNT] = {};
+  std::vector<double> hrows[SWE_NROWS];
+  double *b = nullptr;
+  double *st[3][3] = {};       // [buffer][h,hu,hv]
+  int A = 0, Bi = 1, C = 2;     // A = current state
+  double *mfield = nullptr, *sponge = nullptr;
+  double *rh = nullptr, *rhu = nullptr, *rhv = nullptr, *theta = nullptr;
+  Ctl *dctl = nullptr;
+  Ctl *hctl = nullptr;          // pinned host shadow
+  double *mf_part = nullptr;
+  int max_parts = 0;
+  int chunk = 0;
+  double manning = 0.0, coriolis = 0.0;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  bool use_graph = false;
+  int graph_chunk = -1;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  std::string err;
+};
+
+static thread_local std::string g_err;
+
+#define HIPCHK(ctx, expr)                                                        \
+  do {                                                                           \
+    hipError_t e_ = (expr);                                                      \
+    if (e_ != hipSuccess) {                                                      \
+      (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);            \
+      return SWE_EHIP;                                                           \
+    }                                                                            \
+  } while (0)
+
+static int fail(swe_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg; else g_err = msg;
+  return code;
+}
+
+static void refresh_params(swe_ctx *c) {
+  Params &P = c->P;
+  P.friction = c->mfield ? 2 : ((c->manning != 0.0 && c->manning == c->manning) ? 1 : 0);
+  P.manning = c->manning;
+  P.coriolis = c->coriolis;
+  P.coriolis_on = (c->coriolis != 0.0 && c->coriolis == c->coriolis) ? 1 : 0;
+  P.sponge = c->sponge ? 1 : 0;
+  Geo &G = P.geo;
+  G.dx = c->rows[R_DX]; G.rdx = c->rows[R_RDX]; G.dyN = c->rows[R_DYN]; G.rdyN = c->rows[R_RDYN];
+  G.dyS = c->rows[R_DYS]; G.rdyS = c->rows[R_RDYS]; G.dyCFL = c->rows[R_DYCFL]; G.rdyCFL = c->rows[R_RDYCFL];
+  G.bedPhi = c->rows[R_BED]; G.rbedPhi = c->rows[R_RBED]; G.tanPhi = c->rows[R_TAN]; G.fRow = c->rows[R_F];
+  G.geo = c->rows[R_GEO];
+}
+
+static void drop_graphs(swe_ctx *c) {
+  for (int i = 0; i < 2; ++i) {
+    if (c->gexec[i]) { (void)hipGraphExecDestroy(c->gexec[i]); c->gexec[i] = nullptr; }
+    if (c->graph[i]) { (void)hipGraphDestroy(c->graph[i]); c->graph[i] = nullptr; }
+  }
+}
+
+static int upload_row(swe_ctx *c, int row, const double *host) {
+  const int H = c->P.H;
+  std::vector<double> v(host, host + H);
+  c->hrows[row] = v;
+  auto put = [&](int slot, const std::vector<double> &x) -> int {
+    HIPCHK(c, hipMemcpyAsync(c->rows[slot], x.data(), sizeof(double) * H, hipMemcpyHostToDevice, c->stream));
+    return SWE_OK;
+  };
+  std::vector<double> r(H);
+  for (int j = 0; j < H; ++j) r[j] = 1.0 / v[j];          // correctly rounded reciprocals (1/inf = 0)
+  int rc = SWE_OK;
+  switch (row) {
+    case SWE_ROW_DX: rc = put(R_DX, v); if (!rc) rc = put(R_RDX, r); break;
+    case SWE_ROW_DYN: rc = put(R_DYN, v); if (!rc) rc = put(R_RDYN, r); break;
+    case SWE_ROW_DYS: rc = put(R_DYS, v); if (!rc) rc = put(R_RDYS, r); break;
+    case SWE_ROW_DYCFL: rc = put(R_DYCFL, v); if (!rc) rc = put(R_RDYCFL, r); break;
+    case SWE_ROW_BEDPHI: rc = put(R_BED, v); if (!rc) rc = put(R_RBED, r); break;
+    case SWE_ROW_TANPHI: rc = put(R_TAN, v); break;
+    case SWE_ROW_F: rc = put(R_F, v); break;
+    case SWE_ROW_GEO: rc = put(R_GEO, v); break;
+    case SWE_ROW_AREA: rc = put(R_AREA, v); break;
+    default: return fail(c, SWE_EINVAL, "bad row id");
+  }
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SWE_OK;
+}
+
+static int default_chunk(const Params &P) {
+  const int strips = (P.W + SWEEP_OUT - 1) / SWEEP_OUT;
+  const int target_waves = 4096;               // about 4 per SIMD on 256 CUs
+  int chunks = (target_waves + strips - 1) / strips;
+  if (chunks < 1) chunks = 1;
+  int rows = (P.H + chunks - 1) / chunks;
+  if (rows < 32) rows = 32;
+  if (rows > P.H) rows = P.H;
+  if (rows < MIN_CHUNK) rows = MIN_CHUNK;
+  return rows;
+}
+
+extern "C" {
+
+double swe_js_pow(double x, double y) {
+  if (y == 0.0) return 1.0;
+  if (x != x || y != y) return x + y;
+  if (y == 1.0) return x;
+  if (y == 2.0) return x * x;
+  if (y == 0.5 && x >= 0.0) return std::sqrt(x);
+  if (!(x > 0.0) || std::isinf(x) || std::isinf(y)) return std::pow(x, y);
+  return swe::js_pow_pos(x, y);
+}
+
+const char *swe_build_info(void) { return "water-simulator_amd 0.1 gfx950 fp64"; }
+
+const char *swe_last_error(const swe_ctx *ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int swe_create(const swe_desc *d, swe_ctx **out) {
+  if (!d || !out) return fail(nullptr, SWE_EINVAL, "null argument");
+  *out = nullptr;
+  if (d->nx < 2 || d->ny < 1) return fail(nullptr, SWE_EINVAL, "nx must be >= 2 and ny >= 1");
+  if (d->order != 1 && d->order != 2) return fail(nullptr, SWE_EINVAL, "order must be 1 or 2");
+  if (d->limiter < 0 || d->limiter > 2) return fail(nullptr, SWE_EINVAL, "unknown limiter");
+  if (d->kind != SWE_CARTESIAN && d->kind != SWE_SPHERE) return fail(nullptr, SWE_EINVAL, "unknown kind");
+  for (int r = 0; r < SWE_NROWS; ++r)
+    if (!d->rows[r] && r != SWE_ROW_AREA) return fail(nullptr, SWE_EINVAL, "missing geometry row");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, SWE_EHIP, "no HIP device");
+  swe_ctx *c = new swe_ctx();
+  Params &P = c->P;
+  P.nx = d->nx; P.ny = d->ny; P.W = d->nx + 4; P.H = d->ny + 4;
+  P.sphere = d->kind == SWE_SPHERE; P.order = d->order; P.limiter = d->limiter;
+  for (int s = 0; s < 4; ++s) P.bc[s] = P.sphere ? (s < 2 ? BC_PERIODIC : BC_REFLECT) : BC_REFLECT;
+  P.g = d->g; P.half_g = 0.5 * d->g; P.neg_half_g = -d->g * 0.5; P.rg = 1.0 / d->g;
+  P.min_depth = d->min_depth; P.eps4 = d->eps4; P.cfl = d->cfl; P.R = d->R;
+  c->manning = d->manning; c->coriolis = d->coriolis;
+  c->N = (size_t)P.W * P.H;
+  const size_t bytes = c->N * sizeof(double);
+  auto bail = [&](const char *what, hipError_t e) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    swe_destroy(c);
+    return fail(nullptr, e == hipErrorOutOfMemory ? SWE_ENOMEM : SWE_EHIP, m);
+  };
+  hipError_t e;
+  if ((e = hipGetDevice(&c->device)) != hipSuccess) return bail("hipGetDevice", e);
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return bail("stream", e);
+  if ((e = hipMalloc(&c->b, bytes)) != hipSuccess) return bail("hipMalloc b", e);
+  for (int i = 0; i < 3; ++i)
+    for (int f = 0; f < 3; ++f)
+      if ((e = hipMalloc(&c->st[i][f], bytes)) != hipSuccess) return bail("hipMalloc state", e);
+  for (int r = 0; r < R_COUNT; ++r)
+    if ((e = hipMalloc(&c->rows[r], sizeof(double) * P.H)) != hipSuccess) return bail("hipMalloc rows", e);
+  if ((e = hipMalloc(&c->dctl, sizeof(Ctl))) != hipSuccess) return bail("hipMalloc ctl", e);
+  if ((e = hipHostMalloc((void **)&c->hctl, sizeof(Ctl), hipHostMallocDefault)) != hipSuccess) return bail("pinned ctl", e);
+  const int strips = (P.W + SWEEP_OUT - 1) / SWEEP_OUT;
+  c->max_parts = strips * ((P.H + MIN_CHUNK - 1) / MIN_CHUNK);
+  if ((e = hipMalloc(&c->mf_part, sizeof(double) * 2 * c->max_parts)) != hipSuccess) return bail("hipMalloc part", e);
+  if ((e = hipEventCreate(&c->ev0)) != hipSuccess) return bail("event", e);
+  if ((e = hipEventCreate(&c->ev1)) != hipSuccess) return bail("event", e);
+  for (int f = 0; f < 3; ++f)
+    for (int i = 0; i < 3; ++i) (void)hipMemsetAsync(c->st[i][f], 0, bytes, c->stream);
+  (void)hipMemsetAsync(c->b, 0, bytes, c->stream);
+  std::memset(c->hctl, 0, sizeof(Ctl));
+  c->hctl->dt_mode = DT_HOST;
+  if ((e = hipMemcpyAsync(c->dctl, c->hctl, sizeof(Ctl), hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+    return bail("ctl upload", e);
+  std::vector<double> zero(P.H, 0.0);
+  for (int r = 0; r < SWE_NROWS; ++r) {
+    const double *src = d->rows[r] ? d->rows[r] : zero.data();
+    int rc = upload_row(c, r, src);
+    if (rc) { std::string m = c->err; swe_destroy(c); return fail(nullptr, rc, m); }
+  }
+  c->chunk = default_chunk(P);
+  refresh_params(c);
+  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail("sync", e);
+  *out = c;
+  return SWE_OK;
+}
+
+void swe_destroy(swe_ctx *c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graphs(c);
+  (void)hipFree(c->b);
+  for (auto &s : c->st) for (auto *p : s) (void)hipFree(p);
+  for (auto *p : c->rows) (void)hipFree(p);
+  (void)hipFree(c->mfield); (void)hipFree(c->sponge);
+  (void)hipFree(c->rh); (void)hipFree(c->rhu); (void)hipFree(c->rhv); (void)hipFree(c->theta);
+  (void)hipFree(c->dctl); (void)hipFree(c->mf_part);
+  if (c->hctl) (void)hipHostFree(c->hctl);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+static int ensure_probe(swe_ctx *c) {
+  const size_t bytes = c->N * sizeof(double);
+  double **arr[4] = {&c->rh, &c->rhu, &c->rhv, &c->theta};
+  for (auto **p : arr) {
+    if (!*p) {
+      HIPCHK(c, hipMalloc(p, bytes));
+      HIPCHK(c, hipMemsetAsync(*p, 0, bytes, c->stream));
+    }
+  }
+  return SWE_OK;
+}
+
+static double *field_ptr(swe_ctx *c, int f) {
+  switch (f) {
+    case SWE_F_B: return c->b;
+    case SWE_F_H: return c->st[c->A][0];
+    case SWE_F_HU: return c->st[c->A][1];
+    case SWE_F_HV: return c->st[c->A][2];
+    case SWE_F_MANNING: return c->mfield;
+    case SWE_F_SPONGE: return c->sponge;
+    case SWE_F_RH: return c->rh;
+    case SWE_F_RHU: return c->rhu;
+    case SWE_F_RHV: return c->rhv;
+    case SWE_F_THETA: return c->theta;
+    default: return nullptr;
+  }
+}
+
+int swe_set_field(swe_ctx *c, int f, const double *host) {
+  if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  const size_t bytes = c->N * sizeof(double);
+  if (f == SWE_F_MANNING || f == SWE_F_SPONGE) {
+    double **p = f == SWE_F_MANNING ? &c->mfield : &c->sponge;
+    if (!host) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      HIPCHK(c, hipFree(*p));
+      *p = nullptr;
+      refresh_params(c);
+      drop_graphs(c);
+      return SWE_OK;
+    }
+    if (f == SWE_F_MANNING)
+      for (size_t k = 0; k < c->N; ++k)
+        if (!(host[k] >= 0) || !std::isfinite(host[k]))
+          return fail(c, SWE_EINVAL, "setManningField: Manning n must be finite and >= 0");
+    if (!*p) { HIPCHK(c, hipMalloc(p, bytes)); drop_graphs(c); }
+    HIPCHK(c, hipMemcpyAsync(*p, host, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    refresh_params(c);
+    return SWE_OK;
+  }
+  if (f < SWE_F_B || f > SWE_F_HV) return fail(c, SWE_EINVAL, "field is not writable");
+  if (!host) return fail(c, SWE_EINVAL, "null host pointer");
+  HIPCHK(c, hipMemcpyAsync(field_ptr(c, f), host, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SWE_OK;
+}
+
+int swe_get_field(swe_ctx *c, int f, double *host) {
+  if (!c || !host) return fail(c, SWE_EINVAL, "null argument");
+  double *p = field_ptr(c, f);
+  if (!p) return fail(c, SWE_ESTATE, "field not available (never computed or not set)");
+  HIPCHK(c, hipMemcpyAsync(host, p, c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SWE_OK;
+}
+
+int swe_set_row(swe_ctx *c, int row, const double *host) {
+  if (!c || !host) return fail(c, SWE_EINVAL, "null argument");
+  if (row < 0 || row >= SWE_NROWS) return fail(c, SWE_EINVAL, "bad row id");
+  return upload_row(c, row, host);
+}
+
+int swe_set_bc(swe_ctx *c, int side, int kind) {
+  if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  if (side < 0 || side > 3 || kind < 0 || kind > SWE_BC_STORED) return fail(c, SWE_EINVAL, "bad side or kind");
+  if (c->P.bc[side] != kind) drop_graphs(c);
+  c->P.bc[side] = kind;
+  return SWE_OK;
+}
+
+int swe_set_scalar(swe_ctx *c, int which, double v) {
+  if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  switch (which) {
+    case SWE_S_MANNING: c->manning = v; break;
+    case SWE_S_CORIOLIS: c->coriolis = v; break;
+    case SWE_S_CFL: c->P.cfl = v; break;
+    case SWE_S_SPONGE_REF: c->P.sponge_ref = v; break;
+    case SWE_S_T: c->hctl->t = v; break;
+    case SWE_S_STEPS: c->hctl->steps = (long long)v; break;
+    default: return fail(c, SWE_EINVAL, "scalar is read-only or unknown");
+  }
+  refresh_params(c);
+  drop_graphs(c);
+  if (which == SWE_S_T || which == SWE_S_STEPS) {
+    HIPCHK(c, hipMemcpyAsync(c->dctl, c->hctl, sizeof(Ctl), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return SWE_OK;
+}
+
+static int pull_ctl(swe_ctx *c) {
+  HIPCHK(c, hipMemcpyAsync(c->hctl, c->dctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SWE_OK;
+}
+
+int swe_get_scalar(swe_ctx *c, int which, double *v) {
+  if (!c || !v) return fail(c, SWE_EINVAL, "null argument");
+  int rc = pull_ctl(c);
+  if (rc) return rc;
+  switch (which) {
+    case SWE_S_MANNING: *v = c->manning; break;
+    case SWE_S_CORIOLIS: *v = c->coriolis; break;
+    case SWE_S_CFL: *v = c->P.cfl; break;
+    case SWE_S_SPONGE_REF: *v = c->P.sponge_ref; break;
+    case SWE_S_T: *v = c->hctl->t; break;
+    case SWE_S_STEPS: *v = (double)c->hctl->steps; break;
+    case SWE_S_MASS_FLOORED: *v = c->hctl->mf; break;
+    case SWE_S_DT_NEXT: *v = c->hctl->dt_next; break;
+    case SWE_S_NONFINITE: *v = c->hctl->nonfinite; break;
+    default: return fail(c, SWE_EINVAL, "unknown scalar");
+  }
+  return SWE_OK;
+}
+
+static int push_ctl(swe_ctx *c) {
+  HIPCHK(c, hipMemcpyAsync(c->dctl, c->hctl, sizeof(Ctl), hipMemcpyHostToDevice, c->stream));
+  return SWE_OK;
+}
+
+int swe_apply_bc(swe_ctx *c, const double ext[8]) {
+  if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  int rc = pull_ctl(c);
+  if (rc) return rc;
+  for (int i = 0; i < 8; ++i) c->hctl->ext[i] = ext ? ext[i] : 0.0;
+  if ((rc = push_ctl(c))) return rc;
+  StateW S = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
+  HIPCHK(c, launch_apply_bc(c->P, S, c->b, c->dctl, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SWE_OK;
+}
+
+static SweepArgs base_args(swe_ctx *c) {
+  SweepArgs a{};
+  a.P = c->P;
+  a.b = c->b;
+  a.mfield = c->mfield;
+  a.sponge = c->sponge;
+  a.ctl = c->dctl;
+  a.mf_part = c->mf_part;
+  a.rows_per_chunk = c->chunk;
+  return a;
+}
+
+int swe_residual(swe_ctx *c, double dt) {
+  if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  int rc = ensure_probe(c);
+  if (rc) return rc;
+  if ((rc = pull_ctl(c))) return rc;
+  c->hctl->dt = dt;
+  if ((rc = push_ctl(c))) return rc;
+  SweepArgs a = base_args(c);
+  for (int s = 0; s < 4; ++s) a.P.bc[s] = BC_STORED;      // ghosts as the caller left them
+  a.in = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
+  a.rh = c->rh; a.rhu = c->rhu; a.rhv = c->rhv; a.theta = c->theta;
+  HIPCHK(c, launch_sweep(MODE_PROBE, a, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SWE_OK;
+}
+
+int swe_max_dt(swe_ctx *c, double *out) {
+  if (!c || !out) return fail(c, SWE_EINVAL, "null argument");
+  int rc = pull_ctl(c);
+  if (rc) return rc;
+  const int saved = c->hctl->dt_mode;
+  c->hctl->dt_mode = DT_HOST;
+  c->hctl->cfl_bits = 0;
+  if ((rc = push_ctl(c))) return rc;
+  StateP S = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
+  HIPCHK(c, launch_maxdt(c->P, S, c->dctl, c->stream));
+  HIPCHK(c, launch_finalize(c->dctl, c->mf_part, 0, c->P.cfl, 0, c->stream));
+  if ((rc = pull_ctl(c))) return rc;
+  *out = c->hctl->dt_next;
+  c->hctl->dt_mode = saved;
+  return push_ctl(c);
+}
+
+// one step on the device, stream-ordered; the caller handles the ctl block
+static int enqueue_step(swe_ctx *c, int want_theta, int want_rh) {
+  const int np = sweep_waves(c->P, c->chunk);
+  if (np > c->max_parts) return fail(c, SWE_EINVAL, "chunk too small");
+  SweepArgs a = base_args(c);
+  a.in = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
+  a.out = {c->st[c->Bi][0], c->st[c->Bi][1], c->st[c->Bi][2]};
+  HIPCHK(c, launch_sweep(MODE_STAGE1, a, c->stream));
+  SweepArgs b2 = base_args(c);
+  b2.in = {c->st[c->Bi][0], c->st[c->Bi][1], c->st[c->Bi][2]};
+  b2.h0 = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
+  b2.out = {c->st[c->C][0], c->st[c->C][1], c->st[c->C][2]};
+  b2.mf_part = c->mf_part + np;
+  if (want_theta || want_rh) {
+    int rc = ensure_probe(c);
+    if (rc) return rc;
+    b2.P.want_theta = want_theta ? 1 : 0;
+    b2.P.want_rh = want_rh ? 1 : 0;
+    b2.theta = c->theta; b2.rh = c->rh; b2.rhu = c->rhu; b2.rhv = c->rhv;
+  }
+  HIPCHK(c, launch_sweep(MODE_STAGE2, b2, c->stream));
+  HIPCHK(c, launch_finalize(c->dctl, c->mf_part, np, c->P.cfl, 1, c->stream));
+  std::swap(c->A, c->C);
+  return SWE_OK;
+}
+
+int swe_step(swe_ctx *c, double dt, const double ext[8], int want_theta, int want_rh,
+             double *dt_taken, double *mass_floored) {
+  if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  for (int s = 0; s < 4; ++s)
+    if (c->P.bc[s] == BC_STORED) return fail(c, SWE_EINVAL, "swe_step needs device boundary kinds on every side");
+  int rc = pull_ctl(c);
+  if (rc) return rc;
+  Ctl *h = c->hctl;
+  for (int i = 0; i < 8; ++i) h->ext[i] = ext ? ext[i] : 0.0;
+  h->harm_on = 0;
+  h->cfl_bits = 0;
+  const bool self = std::isnan(dt);
+  h->dt_mode = self ? DT_SELF : DT_HOST;
+  if (!self) h->dt = dt;
+  if ((rc = push_ctl(c))) return rc;
+  if (self) {
+    StateP S = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
+    HIPCHK(c, launch_maxdt(c->P, S, c->dctl, c->stream));
+    HIPCHK(c, launch_finalize(c->dctl, c->mf_part, 0, c->P.cfl, 0, c->stream));
+  }
+  if ((rc = enqueue_step(c, want_theta, want_rh))) return rc;
+  if ((rc = pull_ctl(c))) return rc;
+  const double taken = h->dt_last;
+  if (!(taken > 0)) {
+    // refused step: the reference has still run applyBC (swe.mjs:782-784)
+    if ((rc = swe_apply_bc(c, ext))) return rc;
+  }
+  if (dt_taken) *dt_taken = taken;
+  if (mass_floored) *mass_floored = h->mf;
+  h->dt_mode = DT_HOST;
+  return push_ctl(c);
+}
+
+int swe_set_harmonic(swe_ctx *c, int side, int n_eta, double msl, const double *amp, const double *omega,
+                     const double *phase, int n_u, double u0, const double *uamp, const double *uomega,
+                     const double *uphase) {
+  if (!c || side < 0 || side > 3 || n_eta < 0 || n_eta > MAX_HARM || n_u < 0 || n_u > MAX_HARM)
+    return fail(c, SWE_EINVAL, "bad harmonic forcing");
+  int rc = pull_ctl(c);
+  if (rc) return rc;
+  Harm &hm = c->hctl->harm[side];
+  std::memset(&hm, 0, sizeof(hm));
+  hm.n_eta = n_eta; hm.msl = msl; hm.n_u = n_u; hm.u0 = u0;
+  for (int k = 0; k < n_eta; ++k) { hm.amp[k] = amp[k]; hm.omega[k] = omega[k]; hm.phase[k] = phase[k]; }
+  for (int k = 0; k < n_u; ++k) { hm.uamp[k] = uamp[k]; hm.uomega[k] = uomega[k]; hm.uphase[k] = uphase[k]; }
+  c->hctl->harm_on = 1;
+  rc = push_ctl(c);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SWE_OK;
+}
+
+int swe_use_graph(swe_ctx *c, int on) {
+  if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  c->use_graph = on != 0;
+  if (!on) drop_graphs(c);
+  return SWE_OK;
+}
+
+int swe_set_chunk_rows(swe_ctx *c, int rows) {
+  if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  if (rows == 0) rows = default_chunk(c->P);
+  if (rows < MIN_CHUNK) return fail(c, SWE_EINVAL, "chunk rows below minimum (8)");
+  if (rows != c->chunk) drop_graphs(c);
+  c->chunk = rows;
+  return SWE_OK;
+}
+
+static int build_graph(swe_ctx *c, int which) {
+  // which = parity of the buffer rotation (A index 0 or 2)
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const int saveA = c->A, saveC = c->C;
+  int rc = enqueue_step(c, 0, 0);
+  c->A = saveA; c->C = saveC;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess) return fail(c, SWE_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+  c->graph[which] = g;
+  HIPCHK(c, hipGraphInstantiate(&c->gexec[which], g, nullptr, nullptr, 0));
+  return SWE_OK;
+}
+
+int swe_advance(swe_ctx *c, int64_t nsteps, int dt_mode, double dt_value, double t_end, int64_t *taken) {
+  if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  if (nsteps < 0 || dt_mode < SWE_DT_SELF || dt_mode > SWE_DT_UNTIL) return fail(c, SWE_EINVAL, "bad advance args");
+  for (int s = 0; s < 4; ++s)
+    if (c->P.bc[s] == BC_STORED) return fail(c, SWE_EINVAL, "swe_advance needs device boundary kinds on every side");
+  int rc = pull_ctl(c);
+  if (rc) return rc;
+  Ctl *h = c->hctl;
+  const long long steps0 = h->steps;
+  h->dt_mode = dt_mode == SWE_DT_SELF ? DT_SELF : dt_mode == SWE_DT_FIXED ? DT_FIXED : DT_UNTIL;
+  h->dt_fixed = dt_value;
+  h->t_end = t_end;
+  h->cfl_bits = 0;
+  if ((rc = push_ctl(c))) return rc;
+  StateP S = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
+  HIPCHK(c, launch_maxdt(c->P, S, c->dctl, c->stream));
+  HIPCHK(c, launch_finalize(c->dctl, c->mf_part, 0, c->P.cfl, 0, c->stream));
+  if (c->use_graph && c->graph_chunk != c->chunk) { drop_graphs(c); c->graph_chunk = c->chunk; }
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  for (int64_t n = 0; n < nsteps; ++n) {
+    if (c->use_graph) {
+      const int which = c->A == 0 ? 0 : 1;
+      if (!c->gexec[which]) { if ((rc = build_graph(c, which))) return rc; }
+      HIPCHK(c, hipGraphLaunch(c->gexec[which], c->stream));
+      std::swap(c->A, c->C);
+    } else {
+      if ((rc = enqueue_step(c, 0, 0))) return rc;
+    }
+  }
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  if ((rc = pull_ctl(c))) return rc;
+  float ms = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  if (taken) *taken = h->steps - steps0;
+  h->dt_mode = DT_HOST;
+  return push_ctl(c);
+}
+
+int swe_profile_step(swe_ctx *c, int nsteps, double out_ms[3]) {
+  if (!c || !out_ms || nsteps < 1) return fail(c, SWE_EINVAL, "bad profile args");
+  for (int s = 0; s < 4; ++s)
+    if (c->P.bc[s] == BC_STORED) return fail(c, SWE_EINVAL, "profiling needs device boundary kinds");
+  int rc = pull_ctl(c);
+  if (rc) return rc;
+  c->hctl->dt_mode = DT_SELF;
+  c->hctl->cfl_bits = 0;
+  if ((rc = push_ctl(c))) return rc;
+  StateP S = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
+  HIPCHK(c, launch_maxdt(c->P, S, c->dctl, c->stream));
+  HIPCHK(c, launch_finalize(c->dctl, c->mf_part, 0, c->P.cfl, 0, c->stream));
+  const int np = sweep_waves(c->P, c->chunk);
+  std::vector<hipEvent_t> ev(4 * nsteps);
+  for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
+  for (int n = 0; n < nsteps; ++n) {
+    SweepArgs a = base_args(c);
+    a.in = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
+    a.out = {c->st[c->Bi][0], c->st[c->Bi][1], c->st[c->Bi][2]};
+    SweepArgs b2 = base_args(c);
+    b2.in = {c->st[c->Bi][0], c->st[c->Bi][1], c->st[c->Bi][2]};
+    b2.h0 = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
+    b2.out = {c->st[c->C][0], c->st[c->C][1], c->st[c->C][2]};
+    b2.mf_part = c->mf_part + np;
+    HIPCHK(c, hipEventRecord(ev[4 * n], c->stream));
+    HIPCHK(c, launch_sweep(MODE_STAGE1, a, c->stream));
+    HIPCHK(c, hipEventRecord(ev[4 * n + 1], c->stream));
+    HIPCHK(c, launch_sweep(MODE_STAGE2, b2, c->stream));
+    HIPCHK(c, hipEventRecord(ev[4 * n + 2], c->stream));
+    HIPCHK(c, launch_finalize(c->dctl, c->mf_part, np, c->P.cfl, 1, c->stream));
+    HIPCHK(c, hipEventRecord(ev[4 * n + 3], c->stream));
+    std::swap(c->A, c->C);
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double acc[3] = {0, 0, 0};
+  for (int n = 0; n < nsteps; ++n)
+    for (int k = 0; k < 3; ++k) {
+      float ms = 0.f;
+      HIPCHK(c, hipEventElapsedTime(&ms, ev[4 * n + k], ev[4 * n + k + 1]));
+      acc[k] += ms;
+    }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  for (int k = 0; k < 3; ++k) out_ms[k] = acc[k] / nsteps;
+  if ((rc = pull_ctl(c))) return rc;
+  c->hctl->dt_mode = DT_HOST;
+  return push_ctl(c);
+}
+
+int swe_sync(swe_ctx *c) {
+  if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SWE_OK;
+}
+
+int swe_device_ptr(swe_ctx *c, int f, double **p) {
+  if (!c || !p) return fail(c, SWE_EINVAL, "null argument");
+  *p = field_ptr(c, f);
+  return *p ? SWE_OK : fail(c, SWE_ESTATE, "field not allocated");
+}
+
+int swe_last_advance_ms(swe_ctx *c, double *ms) {
+  if (!c || !ms) return fail(c, SWE_EINVAL, "null argument");
+  *ms = c->last_ms;
+  return SWE_OK;
+}
+
+}  // extern "C"

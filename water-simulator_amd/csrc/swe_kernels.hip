@@ -1,0 +1,532 @@
+// HIP kernels (gfx950) of the shallow-water time step of src/swe.mjs.
+//
+// k_sweep: one fused stage of SSP-RK2 (swe.mjs:754-814 stage bodies, with
+//   residual() :308-629, dryClean() :734-745, and in stage 2 applyFriction()
+//   :659-704, the sponge forcing :1188-1197 and the next step's maxDt()
+//   :632-647).  One wavefront owns a strip of 64 columns (58 outputs + 3 halo
+//   columns each side) and marches down a chunk of rows.  Along y every value a
+//   cell needs stays in that lane's registers from one row to the next; along x
+//   the lanes trade neighbour values with cross-lane shuffles.  Nothing but the
+//   state is read from or written to HBM: slopes, face states, HLLC fluxes,
+//   positivity factors and residuals never leave registers.
+//
+// Row pipeline of iteration r (row r is loaded):
+//   slopes + face states at s=r-1, y-face (s-1|s), x-faces at s,
+//   theta at q=r-2, x-face contributions at q, y-face (q-1|q) contributions,
+//   and the cell update of row o=r-3.
+#include "swe_device.hpp"
+#include "swe_kernels.hpp"
+
+namespace swe {
+
+struct Cell { double b, h, hu, hv; };
+
+__device__ __forceinline__ Cell ld(const StateP &S, const double *b, int k) {
+  Cell c; c.b = b[k]; c.h = S.h[k]; c.hu = S.hu[k]; c.hv = S.hv[k]; return c;
+}
+
+__device__ __forceinline__ int iclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Source cell (interior coordinates) of ghost (i,j) on `side` for BC `kind`:
+// reflect swe.mjs:1026-1034, outflow :1042-1045, flather :1086-1089, periodic :1119-1124.
+__device__ __forceinline__ void bc_src(int kind, int side, int i, int j, int nx, int ny, int &si, int &sj) {
+  if (kind == BC_REFLECT) {
+    si = i; sj = j;
+    if (side == 0) si = -1 - i; else if (side == 1) si = 2 * nx - 1 - i;
+    else if (side == 2) sj = -1 - j; else sj = 2 * ny - 1 - j;
+    si = iclamp(si, 0, nx - 1); sj = iclamp(sj, 0, ny - 1);
+  } else if (kind == BC_PERIODIC) {
+    // i+nx / i-nx (j likewise); when nx or ny is 1 the reference's second ghost
+    // layer reads the first one it has just written, which is the same as
+    // wrapping the index modulo the interior size
+    si = i; sj = j;
+    if (side < 2) si = ((i % nx) + nx) % nx;
+    else sj = ((j % ny) + ny) % ny;
+  } else {
+    si = iclamp(i, 0, nx - 1); sj = iclamp(j, 0, ny - 1);
+  }
+}
+
+// Ghost value from its source cell (the bodies of reflect/outflow/periodic/flather).
+__device__ __forceinline__ Cell bc_apply(int kind, int side, const Cell &s, const Params &P, const double *ext) {
+  Cell g;
+  if (kind == BC_REFLECT) {
+    const bool flipX = side < 2;
+    g.b = s.b; g.h = s.h;
+    g.hu = flipX ? -s.hu : s.hu;
+    g.hv = flipX ? s.hv : -s.hv;
+  } else if (kind == BC_FLATHER) {                     // swe.mjs:1084-1116
+    g.b = s.b;
+    const double bed = s.b, hInt = s.h;
+    const double e0 = ext[2 * side], u0 = ext[2 * side + 1];
+    if (hInt <= P.min_depth) { g.h = jmax0(e0 - bed); g.hu = 0.0; g.hv = 0.0; return g; }
+    const double etaInt = bed + hInt;
+    const double r = __builtin_sqrt(P.g / hInt);
+    const double sgn = (side == 0 || side == 2) ? 1.0 : -1.0;
+    const double uInt = side < 2 ? vel1(s.hu, hInt, P.eps4) : vel1(s.hv, hInt, P.eps4);
+    const double Rin = u0 + sgn * e0 * r;
+    const double Rout = uInt - sgn * etaInt * r;
+    const double un = 0.5 * (Rin + Rout);
+    const double en = sgn * (Rin - Rout) / (2 * r);
+    g.h = jmax0(en - bed);
+    if (side < 2) { g.hu = g.h * un; g.hv = s.hv; }
+    else { g.hv = g.h * un; g.hu = s.hu; }
+  } else {
+    g = s;
+  }
+  return g;
+}
+
+// Ghost of a W/E column at an interior row, as applyBC's first pass leaves it.
+__device__ __noinline__ Cell ghost_we(const Params &P, const StateP &S, const double *b, const double *ext,
+                                      int gi, int gj) {
+  const int side = gi < 2 ? 0 : 1;
+  const int kind = P.bc[side];
+  if (kind == BC_STORED) return ld(S, b, gj * P.W + gi);
+  int si, sj;
+  bc_src(kind, side, gi - 2, gj - 2, P.nx, P.ny, si, sj);
+  const Cell s = ld(S, b, (sj + 2) * P.W + (si + 2));
+  return bc_apply(kind, side, s, P, ext);
+}
+
+// Any cell after applyBC: interior as stored, W/E ghosts from the first pass,
+// S/N ghost rows (corners included) from the second pass, whose periodic source
+// may itself be a W/E ghost.
+__device__ __noinline__ Cell ghost_cell(const Params &P, const StateP &S, const double *b, const double *ext,
+                                        int gi, int gj) {
+  if (gj >= 2 && gj < P.H - 2) return ghost_we(P, S, b, ext, gi, gj);
+  const int side = gj < 2 ? 2 : 3;
+  const int kind = P.bc[side];
+  if (kind == BC_STORED) return ld(S, b, gj * P.W + gi);
+  int si, sj;
+  bc_src(kind, side, gi - 2, gj - 2, P.nx, P.ny, si, sj);
+  const int sgi = si + 2, sgj = sj + 2;
+  Cell s;
+  if (sgi >= 2 && sgi < P.W - 2) s = ld(S, b, sgj * P.W + sgi);
+  else s = ghost_we(P, S, b, ext, sgi, sgj);
+  return bc_apply(kind, side, s, P, ext);
+}
+
+__device__ __forceinline__ Cell load_cell(const Params &P, const StateP &S, const double *b, const double *ext,
+                                          int gi, int gj) {
+  if (gi >= 2 && gi < P.W - 2 && gj >= 2 && gj < P.H - 2) return ld(S, b, gj * P.W + gi);
+  return ghost_cell(P, S, b, ext, gi, gj);
+}
+
+struct Row { double h, b, eta, U, V; };          // what the slope stencil needs of a cell
+struct Face4 { double eta, b, u, v; };           // reconstructed state on one side of a face
+struct FluxF { double M, NL, NR, T; };           // face flux: mass, normal+pL, normal+pR, transverse
+struct Part3 { double h, hu, hv; };
+
+__device__ __forceinline__ double shdn(double v) { return __shfl_down(v, 1, SWEEP_LANES); }
+__device__ __forceinline__ double shup(double v) { return __shfl_up(v, 1, SWEEP_LANES); }
+
+// Interface flux + hydrostatic pressure corrections (swe.mjs:413-445 body, and
+// :448-475 with normal/tangential swapped by the caller).
+__device__ __forceinline__ FluxF face_flux(const Face4 &L, const Face4 &R, const HllcConst &C, double half_g) {
+  const double hLf = jmax0(L.eta - L.b);
+  const double hRf = jmax0(R.eta - R.b);
+  const double bStar = jmax(L.b, R.b);
+  const double hsL = jmax0(L.eta - bStar);
+  const double hsR = jmax0(R.eta - bStar);
+  const Flux3 f = hllc(hsL, hsL * L.u, hsL * L.v, hsR, hsR * R.u, hsR * R.v, C);
+  const double pL = half_g * (hLf * hLf - hsL * hsL);
+  const double pR = half_g * (hRf * hRf - hsR * hsR);
+  FluxF o;
+  o.M = f.m; o.NL = f.n + pL; o.NR = f.n + pR; o.T = f.t;
+  return o;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(SWEEP_LANES) void k_sweep(SweepArgs A) {
+  const Params &P = A.P;
+  const int W = P.W, H = P.H;
+  const int lane = threadIdx.x;
+  const int gi = blockIdx.x * SWEEP_OUT - SWEEP_HALO + lane;
+  const bool colIn = gi >= 0 && gi < W;
+  const bool outLane = colIn && lane >= SWEEP_HALO && lane < SWEEP_LANES - SWEEP_HALO;
+  const int y0 = blockIdx.y * A.rows_per_chunk;
+  const int y1 = min(H, y0 + A.rows_per_chunk);
+  const Ctl *ctl = A.ctl;
+  const double dt = ctl->dt;
+  const int wave_id = blockIdx.y * gridDim.x + blockIdx.x;
+  if (MODE != MODE_PROBE && (!(dt > 0) || __builtin_isinf(dt))) {
+    // a refused step (swe.mjs:784) leaves the state as it was: stage 2 hands the
+    // step-start state through to its output buffer unchanged
+    if (MODE == MODE_STAGE2 && colIn && outLane) {
+      for (int o = y0; o < y1; ++o) {
+        const int k = o * W + gi;
+        A.out.h[k] = A.h0.h[k]; A.out.hu[k] = A.h0.hu[k]; A.out.hv[k] = A.h0.hv[k];
+      }
+    }
+    if (lane == 0) A.mf_part[wave_id] = 0.0;
+    return;
+  }
+  const bool posOn = dt > 0;
+  const double *ext = ctl->ext;
+  const double *b = A.b;
+  const HllcConst HC = {P.g, P.half_g, P.rg, P.min_depth};
+  const double md = P.min_depth;
+  const bool o2 = P.order >= 2;
+  const bool bedOn = o2 || P.sphere;
+  const Geo &G = P.geo;
+
+  auto load_row = [&](int r) -> Row {
+    Row c;
+    if (r >= 0 && r < H && colIn) {
+      const Cell x = load_cell(P, A.in, b, ext, gi, r);
+      c.h = x.h; c.b = x.b; c.eta = x.b + x.h;
+      const Vel2 uv = vel2(x.hu, x.hv, x.h, P.eps4);
+      c.U = uv.u; c.V = uv.v;
+    } else {
+      c.h = 0.0; c.b = 0.0; c.eta = 0.0; c.U = 0.0; c.V = 0.0;
+    }
+    return c;
+  };
+
+  Row cM = load_row(y0 - 3);
+  Row c0 = load_row(y0 - 2);
+  Face4 nPrev = {0.0, 0.0, 0.0, 0.0};
+  FluxF fxPrev = {0.0, 0.0, 0.0, 0.0}, fyPrev = {0.0, 0.0, 0.0, 0.0};
+  double thPrev = 1.0;
+  Part3 xpPrev = {0.0, 0.0, 0.0}, ysPrev = {0.0, 0.0, 0.0};
+  Part3 bedPrev = {0.0, 0.0, 0.0}, bedPrev2 = {0.0, 0.0, 0.0};
+  double Uo = 0.0;
+  double mf = 0.0;
+  unsigned long long cflb = 0ull;
+  bool bad = false;
+
+  const bool xin = gi >= 2 && gi <= W - 3;             // interior column
+  const bool slopeCol = gi >= 1 && gi <= W - 2;
+  const bool fxCol = gi >= 1 && gi <= W - 3;           // owns x-face (gi|gi+1)
+  const bool fxColW = gi >= 2 && gi <= W - 2;          // x-face (gi-1|gi) exists
+  const bool fyCol = gi >= 2 && gi <= W - 3;
+
+  for (int r = y0 - 1; r <= y1 + 2; ++r) {
+    const Row cP = load_row(r);
+    const int s = r - 1;
+    // ---- slopes at row s (swe.mjs:364-411) ---------------------------------
+    const double hW = shup(c0.h), hE = shdn(c0.h);
+    const double eW = shup(c0.eta), eE = shdn(c0.eta);
+    const double bW = shup(c0.b), bE = shdn(c0.b);
+    const double UW = shup(c0.U), UE = shdn(c0.U);
+    const double VW = shup(c0.V), VE = shdn(c0.V);
+    double sEx = 0.0, sBx = 0.0, sUx = 0.0, sVx = 0.0, sEy = 0.0, sBy = 0.0, sUy = 0.0, sVy = 0.0;
+    if (o2 && slopeCol && s >= 1 && s <= H - 2) {
+      if (!(c0.h <= md || hE <= md || hW <= md)) {
+        sEx = limit(P.limiter, c0.eta - eW, eE - c0.eta);
+        sBx = limit(P.limiter, c0.b - bW, bE - c0.b);
+        sUx = limit(P.limiter, c0.U - UW, UE - c0.U);
+        sVx = limit(P.limiter, c0.V - VW, VE - c0.V);
+      }
+      if (!(c0.h <= md || cP.h <= md || cM.h <= md)) {
+        sEy = limit(P.limiter, c0.eta - cM.eta, cP.eta - c0.eta);
+        sBy = limit(P.limiter, c0.b - cM.b, cP.b - c0.b);
+        sUy = limit(P.limiter, c0.U - cM.U, cP.U - c0.U);
+        sVy = limit(P.limiter, c0.V - cM.V, cP.V - c0.V);
+      }
+    }
+    // reconstructed face states of cell (gi,s) (swe.mjs:418-427)
+    const Face4 fE = {c0.eta + 0.5 * sEx, c0.b + 0.5 * sBx, c0.U + 0.5 * sUx, c0.V + 0.5 * sVx};
+    const Face4 fWst = {c0.eta - 0.5 * sEx, c0.b - 0.5 * sBx, c0.U - 0.5 * sUx, c0.V - 0.5 * sVx};
+    const Face4 fN = {c0.eta + 0.5 * sEy, c0.b + 0.5 * sBy, c0.U + 0.5 * sUy, c0.V + 0.5 * sVy};
+    const Face4 fS = {c0.eta - 0.5 * sEy, c0.b - 0.5 * sBy, c0.U - 0.5 * sUy, c0.V - 0.5 * sVy};
+    // centred bed term and spherical geometric source of cell (gi,s) (swe.mjs:570-598)
+    Part3 bedCur = {0.0, 0.0, 0.0};
+    if (bedOn && xin && s >= 2 && s <= H - 3) {
+      const double hPx = jmax0(fE.eta - fE.b), hMx = jmax0(fWst.eta - fWst.b);
+      const double hPy = jmax0(fN.eta - fN.b), hMy = jmax0(fS.eta - fS.b);
+      bedCur.hu = div_r(P.neg_half_g * (hPx + hMx) * sBx, G.dx[s], G.rdx[s]);
+      bedCur.h = div_r(P.neg_half_g * (hPy + hMy) * sBy, G.bedPhi[s], G.rbedPhi[s]);   // rhv bed term
+      bedCur.hv = P.sphere ? G.geo[s] * P.g * (hPy * hPy + hMy * hMy) : 0.0;           // rhv geometric term
+    }
+    // ---- y-face (s-1|s): v normal, u tangential (swe.mjs:448-475) ----------
+    FluxF fyCur = {0.0, 0.0, 0.0, 0.0};
+    if (fyCol && s - 1 >= 1 && s - 1 <= H - 3) {
+      const Face4 Ly = {nPrev.eta, nPrev.b, nPrev.v, nPrev.u};
+      const Face4 Ry = {fS.eta, fS.b, fS.v, fS.u};
+      fyCur = face_flux(Ly, Ry, HC, P.half_g);
+    }
+    // ---- x-face (gi|gi+1) at row s -----------------------------------------
+    const Face4 fWR = {shdn(fWst.eta), shdn(fWst.b), shdn(fWst.u), shdn(fWst.v)};
+    FluxF fxCur = {0.0, 0.0, 0.0, 0.0};
+    if (fxCol && s >= 2 && s <= H - 3) fxCur = face_flux(fE, fWR, HC, P.half_g);
+
+    // ---- positivity factor at row q = s-1 (swe.mjs:507-529) -----------------
+    const int q = s - 1;
+    const double hq = cM.h;
+    double th = 1.0;
+    {
+      const double eM = fxPrev.M, wM = shup(fxPrev.M);
+      if (posOn && slopeCol && q >= 1 && q <= H - 2) {
+        double out = 0.0;
+        if (eM > 0) out += div_r(eM, G.dx[q], G.rdx[q]);
+        if (wM < 0) out -= div_r(wM, G.dx[q], G.rdx[q]);
+        if (fyCur.M > 0) out += div_r(fyCur.M, G.dyN[q], G.rdyN[q]);
+        if (fyPrev.M < 0) out -= div_r(fyPrev.M, G.dyS[q], G.rdyS[q]);
+        if (out * dt > hq) th = hq > 0 ? hq / (out * dt) : 0.0;
+      }
+    }
+    // ---- x-face contributions at row q (swe.mjs:530-540) ---------------------
+    Part3 xpCur;
+    {
+      const double thE = shdn(th);
+      const double the = fxPrev.M >= 0 ? th : thE;
+      double inv = 0.0;
+      if (q >= 0 && q < H) inv = (the == 1.0) ? G.rdx[q] : the / G.dx[q];
+      const double eMc = fxPrev.M * inv, eNL = fxPrev.NL * inv, eNR = fxPrev.NR * inv, eT = fxPrev.T * inv;
+      const double wMc = shup(eMc), wNR = shup(eNR), wT = shup(eT);
+      const bool rowX = q >= 2 && q <= H - 3;
+      double ah = 0.0, ahu = 0.0, ahv = 0.0;
+      if (rowX && fxColW) { ah += wMc; ahu += wNR; ahv += wT; }
+      if (rowX && fxCol) { ah -= eMc; ahu -= eNL; ahv -= eT; }
+      xpCur.h = ah; xpCur.hu = ahu; xpCur.hv = ahv;
+    }
+    // ---- y-face (q-1|q) contributions (swe.mjs:541-555) ----------------------
+    Part3 ysCur = {0.0, 0.0, 0.0}, yn = {0.0, 0.0, 0.0};
+    const bool fyPrevOn = fyCol && q - 1 >= 1 && q - 1 <= H - 3;
+    if (fyPrevOn) {
+      const double thy = fyPrev.M >= 0 ? thPrev : th;
+      const double invL = (thy == 1.0) ? G.rdyN[q - 1] : thy / G.dyN[q - 1];
+      const double invR = (thy == 1.0) ? G.rdyS[q] : thy / G.dyS[q];
+      yn.h = fyPrev.M * invL; yn.hu = fyPrev.T * invL; yn.hv = fyPrev.NL * invL;
+      ysCur.h = fyPrev.M * invR; ysCur.hu = fyPrev.T * invR; ysCur.hv = fyPrev.NR * invR;
+    }
+
+    // ---- cell update of row o = q-1 ------------------------------------------
+    const int o = q - 1;
+    if (outLane && o >= y0 && o < y1) {
+      const int k = o * W + gi;
+      const bool fySouth = fyCol && o - 1 >= 1 && o - 1 <= H - 3;
+      double rh = xpPrev.h, rhu = xpPrev.hu, rhv = xpPrev.hv;
+      if (fySouth) { rh += ysPrev.h; rhu += ysPrev.hu; rhv += ysPrev.hv; }
+      if (fyPrevOn) { rh -= yn.h; rhu -= yn.hu; rhv -= yn.hv; }
+      const bool interior = xin && o >= 2 && o <= H - 3;
+      const Cell cur = load_cell(P, A.in, b, ext, gi, o);
+      if (interior) {
+        if (bedOn) {
+          rhu += bedPrev2.hu;
+          rhv += bedPrev2.h;
+          if (P.sphere) rhv += bedPrev2.hv;
+        }
+        if (P.sphere) {                                   // swe.mjs:600-619
+          const double fe = G.fRow[o] + Uo * (G.tanPhi[o] / P.R);
+          rhu += fe * cur.hv;
+          rhv -= fe * cur.hu;
+        } else if (P.coriolis_on) {                       // swe.mjs:620-628
+          rhu += P.coriolis * cur.hv;
+          rhv -= P.coriolis * cur.hu;
+        }
+      }
+      if (MODE == MODE_PROBE) {
+        A.rh[k] = rh; A.rhu[k] = rhu; A.rhv[k] = rhv; A.theta[k] = thPrev;
+      } else if (MODE == MODE_STAGE1) {
+        if (interior) {                                   // swe.mjs:790-796
+          double nh = cur.h + dt * rh, nhu = cur.hu + dt * rhu, nhv = cur.hv + dt * rhv;
+          if (nh < 0) { mf -= nh; nh = 0.0; }
+          if (nh <= md) { nhu = 0.0; nhv = 0.0; }
+          A.out.h[k] = nh; A.out.hu[k] = nhu; A.out.hv[k] = nhv;
+        }
+      } else {                                            // stage 2, swe.mjs:799-812
+        const Cell c0a = load_cell(P, A.h0, b, ext, gi, o);
+        double nh = 0.5 * (c0a.h + cur.h + dt * rh);
+        double nhu = 0.5 * (c0a.hu + cur.hu + dt * rhu);
+        double nhv = 0.5 * (c0a.hv + cur.hv + dt * rhv);
+        if (interior) {
+          if (nh < 0) { mf -= nh; nh = 0.0; }
+          if (nh <= md) { nhu = 0.0; nhv = 0.0; }
+          // applyFriction (swe.mjs:659-704)
+          if (P.friction) {
+            if (nh <= md) { nhu = 0.0; nhv = 0.0; }
+            else {
+              const double n = P.friction == 2 ? A.mfield[k] : P.manning;
+              if (n != 0.0 && n == n) {
+                const Vel2 uv = vel2(nhu, nhv, nh, P.eps4);
+                const double speed = js_hypot(uv.u, uv.v);
+                if (!(speed < 1e-14)) {
+                  const double denom = 1 + dt * P.g * n * n * speed / js_pow_pos(nh, 4.0 / 3.0);
+                  nhu = nhu / denom;
+                  nhv = nhv / denom;
+                }
+              }
+            }
+          }
+          // makeSponge forcing body (swe.mjs:1188-1197)
+          if (P.sponge) {
+            const double str = A.sponge[k];
+            if (str > 0) {
+              const double f = jmin(1.0, str * dt * 60);
+              const double target = jmax0(P.sponge_ref - b[k]);
+              nh += (target - nh) * f;
+              nhu -= nhu * f;
+              nhv -= nhv * f;
+            }
+          }
+          // next step's maxDt (swe.mjs:632-647)
+          if (nh > md) {
+            const double c = __builtin_sqrt(P.g * nh);
+            const Vel2 uv = vel2(nhu, nhv, nh, P.eps4);
+            const double sp = div_r(__builtin_fabs(uv.u) + c, G.dx[o], G.rdx[o]) +
+                              div_r(__builtin_fabs(uv.v) + c, G.dyCFL[o], G.rdyCFL[o]);
+            if (sp > 0) {
+              const unsigned long long bits = (unsigned long long)__double_as_longlong(sp);
+              if (bits > cflb) cflb = bits;
+            }
+          }
+        } else {
+          A.b[k] = cur.b;                                 // ghost bed, as applyBC writes it
+        }
+        if (!(__builtin_isfinite(nh) && __builtin_isfinite(nhu) && __builtin_isfinite(nhv))) bad = true;
+        A.out.h[k] = nh; A.out.hu[k] = nhu; A.out.hv[k] = nhv;
+        if (A.P.want_theta) A.theta[k] = thPrev;
+        if (A.P.want_rh) { A.rh[k] = rh; A.rhu[k] = rhu; A.rhv[k] = rhv; }
+      }
+    }
+
+    // ---- rotate the pipeline ---------------------------------------------------
+    Uo = cM.U;
+    cM = c0; c0 = cP;
+    nPrev = fN;
+    fxPrev = fxCur; fyPrev = fyCur;
+    thPrev = th;
+    xpPrev = xpCur; ysPrev = ysCur;
+    bedPrev2 = bedPrev; bedPrev = bedCur;
+  }
+
+  if (MODE == MODE_PROBE) return;
+  // wave reductions: massFloored partial (fixed order), max wave speed, non-finite flag
+  for (int off = 32; off >= 1; off >>= 1) mf += __shfl_down(mf, off, SWEEP_LANES);
+  if (lane == 0) A.mf_part[wave_id] = mf;
+  if (MODE == MODE_STAGE2) {
+    for (int off = 32; off >= 1; off >>= 1) {
+      const unsigned long long o2b = __shfl_down(cflb, off, SWEEP_LANES);
+      if (o2b > cflb) cflb = o2b;
+    }
+    if (lane == 0 && cflb) atomicMax(&A.ctl->cfl_bits, cflb);
+    if (__any(bad) && lane == 0) atomicOr(&A.ctl->nonfinite, 1);
+  }
+}
+
+// applyBC on stored arrays (swe.mjs:289-303): every ghost cell rewritten from the
+// interior.  Used by the explicit applyBC()/residual() probe path.
+__global__ void k_apply_bc(Params P, StateW S, double *b, const Ctl *ctl) {
+  const int nG = 2 * P.W * 2 + 2 * (P.H - 4) * 2;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nG) return;
+  int gi, gj;
+  if (t < 4 * P.W) { const int rr = t / P.W; gi = t % P.W; gj = rr < 2 ? rr : P.H - 4 + rr; }
+  else { const int u = t - 4 * P.W; const int cc = u / (P.H - 4); gj = 2 + u % (P.H - 4); gi = cc < 2 ? cc : P.W - 4 + cc; }
+  const StateP R = {S.h, S.hu, S.hv};
+  // every ghost value depends only on interior cells (a periodic S/N source in a
+  // ghost column is recomputed, not read), so the writes below race with no read
+  const Cell c = ghost_cell(P, R, b, ctl->ext, gi, gj);
+  const int k = gj * P.W + gi;
+  b[k] = c.b; S.h[k] = c.h; S.hu[k] = c.hu; S.hv[k] = c.hv;
+}
+
+// maxDt() over the stored state (swe.mjs:632-647); result into ctl->cfl_bits.
+__global__ void k_maxdt(Params P, StateP S, Ctl *ctl) {
+  const int n = P.nx * P.ny;
+  unsigned long long best = 0ull;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    const int j = 2 + t / P.nx, i = 2 + t % P.nx;
+    const int k = j * P.W + i;
+    const double hk = S.h[k];
+    if (hk <= P.min_depth) continue;
+    const double c = __builtin_sqrt(P.g * hk);
+    const Vel2 uv = vel2(S.hu[k], S.hv[k], hk, P.eps4);
+    const double sp = div_r(__builtin_fabs(uv.u) + c, P.geo.dx[j], P.geo.rdx[j]) +
+                      div_r(__builtin_fabs(uv.v) + c, P.geo.dyCFL[j], P.geo.rdyCFL[j]);
+    if (sp > 0) {
+      const unsigned long long bits = (unsigned long long)__double_as_longlong(sp);
+      if (bits > best) best = bits;
+    }
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    const unsigned long long o = __shfl_down(best, off, 64);
+    if (o > best) best = o;
+  }
+  if ((threadIdx.x & 63) == 0 && best) atomicMax(&ctl->cfl_bits, best);
+}
+
+__device__ __forceinline__ double harm_eval(int n, double c0, const double *amp, const double *om,
+                                            const double *ph, double t) {
+  double e = c0;
+  for (int k = 0; k < n; ++k) e += amp[k] * cos(om[k] * t - ph[k]);
+  return e;
+}
+
+// Turn the max-reduction into a step size and set up the next step.
+//   phase 0: after k_maxdt -- dt for the step about to run
+//   phase 1: after a step -- massFloored, t, steps, dt_next, next dt and forcing
+__global__ void k_finalize(Ctl *ctl, const double *mf_part, int nparts, double cfl, int phase) {
+  const int lane = threadIdx.x;
+  double s1 = 0.0, s2 = 0.0;
+  if (phase == 1) {
+    for (int i = lane; i < nparts; i += 64) { s1 += mf_part[i]; s2 += mf_part[nparts + i]; }
+    for (int off = 32; off >= 1; off >>= 1) { s1 += __shfl_down(s1, off, 64); s2 += __shfl_down(s2, off, 64); }
+  }
+  if (lane != 0) return;
+  const unsigned long long bits = ctl->cfl_bits;
+  const double inv = __longlong_as_double((long long)bits);
+  const double dtn = bits ? cfl / inv : __builtin_inf();
+  ctl->cfl_bits = 0ull;
+  ctl->dt_next = dtn;
+  if (phase == 1) {
+    const double dt = ctl->dt;
+    if ((dt > 0) && !__builtin_isinf(dt)) {
+      ctl->mf = s1 + s2;
+      ctl->t = ctl->t + dt;
+      ctl->steps = ctl->steps + 1;
+      ctl->dt_last = dt;
+    } else {
+      ctl->dt_last = 0.0;
+    }
+  }
+  // dt of the next step
+  if (ctl->dt_mode == DT_SELF) ctl->dt = dtn;
+  else if (ctl->dt_mode == DT_FIXED) ctl->dt = ctl->dt_fixed;
+  else if (ctl->dt_mode == DT_UNTIL) ctl->dt = fmin(dtn, ctl->t_end - ctl->t);
+  if (ctl->harm_on) {
+    for (int side = 0; side < 4; ++side) {
+      const Harm &h = ctl->harm[side];
+      ctl->ext[2 * side] = harm_eval(h.n_eta, h.msl, h.amp, h.omega, h.phase, ctl->t);
+      ctl->ext[2 * side + 1] = harm_eval(h.n_u, h.u0, h.uamp, h.uomega, h.uphase, ctl->t);
+    }
+  }
+}
+
+// ---- launchers (host side, C++ linkage inside the library) ------------------
+hipError_t launch_sweep(int mode, const SweepArgs &a, hipStream_t st) {
+  dim3 grid((a.P.W + SWEEP_OUT - 1) / SWEEP_OUT, (a.P.H + a.rows_per_chunk - 1) / a.rows_per_chunk);
+  dim3 block(SWEEP_LANES);
+  if (mode == MODE_STAGE1) hipLaunchKernelGGL(k_sweep<MODE_STAGE1>, grid, block, 0, st, a);
+  else if (mode == MODE_STAGE2) hipLaunchKernelGGL(k_sweep<MODE_STAGE2>, grid, block, 0, st, a);
+  else hipLaunchKernelGGL(k_sweep<MODE_PROBE>, grid, block, 0, st, a);
+  return hipGetLastError();
+}
+
+int sweep_waves(const Params &P, int rows_per_chunk) {
+  return ((P.W + SWEEP_OUT - 1) / SWEEP_OUT) * ((P.H + rows_per_chunk - 1) / rows_per_chunk);
+}
+
+hipError_t launch_apply_bc(const Params &P, StateW S, double *b, const Ctl *ctl, hipStream_t st) {
+  const int nG = 4 * P.W + 4 * (P.H - 4);
+  hipLaunchKernelGGL(k_apply_bc, dim3((nG + 255) / 256), dim3(256), 0, st, P, S, b, ctl);
+  return hipGetLastError();
+}
+
+hipError_t launch_maxdt(const Params &P, StateP S, Ctl *ctl, hipStream_t st) {
+  const int n = P.nx * P.ny;
+  int blocks = (n + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(k_maxdt, dim3(blocks), dim3(256), 0, st, P, S, ctl);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize(Ctl *ctl, const double *mf_part, int nparts, double cfl, int phase, hipStream_t st) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, st, ctl, mf_part, nparts, cfl, phase);
+  return hipGetLastError();
+}
+
+}  // namespace swe

@@ -1,0 +1,82 @@
+// Kernel-side data layout and launch interface for the shallow-water step.
+//
+// Layout in HBM (one basin): every field is a row-major array of W*H doubles,
+// W = nx+4, H = ny+4 (two ghost cells per side), cell k = j*W + i, exactly the
+// reference's indexing (swe.mjs:196-199).  The state lives in three rotating
+// buffers {h,hu,hv}[3]: stage 1 reads A and writes B, stage 2 reads B (stencil)
+// and A (pointwise) and writes C, then A and C swap.  b is shared.  Geometry is
+// 13 per-row arrays of length H (divisors and their correctly rounded
+// reciprocals).  Ghost cells are never read from memory in a step: every load of
+// a ghost cell recomputes the boundary condition from the interior (the
+// reference's applyBC, swe.mjs:289-303, folded into the loads).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace swe {
+
+enum BcKind { BC_REFLECT = 0, BC_OUTFLOW = 1, BC_PERIODIC = 2, BC_FLATHER = 3, BC_STORED = 4 };
+enum Mode { MODE_STAGE1 = 1, MODE_STAGE2 = 2, MODE_PROBE = 3 };
+enum DtMode { DT_SELF = 0, DT_FIXED = 1, DT_UNTIL = 2, DT_HOST = 3 };
+enum { MAX_HARM = 4 };
+
+struct Geo {
+  const double *dx, *rdx, *dyN, *rdyN, *dyS, *rdyS, *dyCFL, *rdyCFL, *bedPhi, *rbedPhi,
+      *tanPhi, *fRow, *geo;
+};
+
+// Harmonic boundary forcing evaluated on the device for device-resident runs:
+// eta(t) = msl + sum_k amp_k cos(omega_k t - phase_k); u(t) likewise.
+struct Harm {
+  int n_eta, n_u, pad0, pad1;
+  double msl, u0;
+  double amp[MAX_HARM], omega[MAX_HARM], phase[MAX_HARM];
+  double uamp[MAX_HARM], uomega[MAX_HARM], uphase[MAX_HARM];
+};
+
+// Device-resident control block of one basin.
+struct Ctl {
+  double dt;           // dt of the step about to run (or just run)
+  double t;            // step-start time
+  double ext[8];       // Flather (eta_ext, u_ext) per side at t
+  double dt_next;      // cfl / max wave speed of the state after the last step (inf if dry)
+  double mf;           // massFloored of the last step
+  double dt_fixed, t_end;
+  unsigned long long cfl_bits;   // max-reduction accumulator (bit pattern of a non-negative double)
+  long long steps;
+  int dt_mode, nonfinite, harm_on, pad;
+  double dt_last;      // dt the last step ran with (0 if it was refused)
+  Harm harm[4];
+};
+
+struct Params {
+  int nx, ny, W, H;
+  int sphere, order, limiter, coriolis_on;
+  int bc[4];
+  int friction;        // 0 none, 1 scalar n, 2 per-cell field
+  int sponge;          // per-cell sponge strengths present
+  int want_theta, want_rh;
+  double g, half_g, neg_half_g, rg, min_depth, eps4, cfl, coriolis, manning, R, sponge_ref;
+  Geo geo;
+};
+
+struct StateP { const double *h, *hu, *hv; };
+struct StateW { double *h, *hu, *hv; };
+
+struct SweepArgs {
+  Params P;
+  StateP in;          // stencil input (stage 1: A, stage 2: B, probe: A with stored ghosts)
+  StateP h0;          // stage 2: A (pointwise)
+  StateW out;         // stage 1: B, stage 2: C
+  double *b;          // bed (ghost cells rewritten by stage 2)
+  double *rh, *rhu, *rhv, *theta;   // probe outputs (or optional stage-2 outputs)
+  const double *mfield, *sponge;
+  Ctl *ctl;
+  double *mf_part;    // one partial per wave
+  int rows_per_chunk;
+};
+
+constexpr int SWEEP_LANES = 64;
+constexpr int SWEEP_HALO = 3;
+constexpr int SWEEP_OUT = SWEEP_LANES - 2 * SWEEP_HALO;   // 58 output columns per wave
+
+}  // namespace swe

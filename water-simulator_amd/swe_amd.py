@@ -1,0 +1,389 @@
+"""Python host mirror of the HIP shallow-water step (ctypes over include/swe_amd.h).
+
+`Solver` mirrors ShallowWater of src/swe.mjs (the reference solver): the same
+state arrays (b, h, hu, hv over W*H cells with two ghost cells per side), the
+same boundary kinds, step(dt) with dt=None meaning self-timed, maxDt(),
+applyBC(), residual(dt), setManningField() and the sponge forcing.  All
+arithmetic runs in libswe_amd.so on the GPU; there is no CPU fallback: if the
+library or a GPU is missing, construction raises.
+
+`geometry_cartesian` / `geometry_spherical` restate src/geometry.mjs (the
+metric rows the step divides by).
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libswe_amd.so")
+
+NROWS = 9
+ROWS = ["dxRow", "dyRowN", "dyRowS", "dyCFL", "bedPhi", "tanPhi", "fRow", "geoCoef", "area"]
+F_B, F_H, F_HU, F_HV, F_MANNING, F_RH, F_RHU, F_RHV, F_THETA, F_SPONGE = range(10)
+BC = {"reflect": 0, "outflow": 1, "periodic": 2, "flather": 3, "stored": 4}
+LIMITER = {"minmod": 0, "mc": 1, "central": 2}
+S_MANNING, S_CORIOLIS, S_CFL, S_T, S_SPONGE_REF, S_STEPS, S_MASS_FLOORED, S_DT_NEXT, S_NONFINITE = range(9)
+DT_SELF, DT_FIXED, DT_UNTIL = 0, 1, 2
+
+
+class SweError(RuntimeError):
+    pass
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("kind", ctypes.c_int32),
+                ("order", ctypes.c_int32), ("limiter", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("g", ctypes.c_double), ("min_depth", ctypes.c_double), ("eps4", ctypes.c_double),
+                ("cfl", ctypes.c_double), ("coriolis", ctypes.c_double), ("manning", ctypes.c_double),
+                ("R", ctypes.c_double), ("rows", ctypes.POINTER(ctypes.c_double) * NROWS)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libswe_amd.so (built by __graft_entry__.build()); raises if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SweError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i, d, dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_double)
+        i64 = ctypes.c_int64
+        sig = {
+            "swe_create": (i, [ctypes.POINTER(Desc), ctypes.POINTER(vp)]),
+            "swe_destroy": (None, [vp]),
+            "swe_last_error": (ctypes.c_char_p, [vp]),
+            "swe_build_info": (ctypes.c_char_p, []),
+            "swe_js_pow": (d, [d, d]),
+            "swe_set_field": (i, [vp, i, dp]),
+            "swe_get_field": (i, [vp, i, dp]),
+            "swe_set_row": (i, [vp, i, dp]),
+            "swe_set_bc": (i, [vp, i, i]),
+            "swe_set_scalar": (i, [vp, i, d]),
+            "swe_get_scalar": (i, [vp, i, dp]),
+            "swe_apply_bc": (i, [vp, dp]),
+            "swe_residual": (i, [vp, d]),
+            "swe_max_dt": (i, [vp, dp]),
+            "swe_step": (i, [vp, d, dp, i, i, dp, dp]),
+            "swe_advance": (i, [vp, i64, i, d, d, ctypes.POINTER(i64)]),
+            "swe_set_harmonic": (i, [vp, i, i, d, dp, dp, dp, i, d, dp, dp, dp]),
+            "swe_use_graph": (i, [vp, i]),
+            "swe_set_chunk_rows": (i, [vp, i]),
+            "swe_sync": (i, [vp]),
+            "swe_device_ptr": (i, [vp, i, ctypes.POINTER(dp)]),
+            "swe_last_advance_ms": (i, [vp, dp]),
+            "swe_profile_step": (i, [vp, i, dp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ext(ext):
+    e = np.zeros(8)
+    if ext is not None:
+        e[:] = ext
+    return e
+
+
+def js_pow4(x):
+    """max(minDepth,1e-12)**4 as the JS engine evaluates it (the library's fdlibm pow)."""
+    return lib().swe_js_pow(x, 4.0)
+
+
+class Solver:
+    """One basin on the GPU.  Mirrors the reference ShallowWater's state and methods."""
+
+    def __init__(self, nx, ny, rows, *, sphere=False, order=2, limiter="mc", g=9.80665, min_depth=1e-3,
+                 eps4=None, cfl=0.45, coriolis=0.0, manning=0.025, R=math.inf, dx=None, dy=None,
+                 use_manning_field=False):
+        L = lib()
+        if eps4 is None:
+            eps4 = js_pow4(max(min_depth, 1e-12))
+        self.nx, self.ny, self.W, self.H = nx, ny, nx + 4, ny + 4
+        self.N = self.W * self.H
+        self.sphere = sphere
+        self.min_depth = min_depth
+        self.dx, self.dy = dx, dy
+        self._rows = {r: np.ascontiguousarray(rows[r], dtype=np.float64) for r in ROWS}
+        d = Desc()
+        d.nx, d.ny, d.kind, d.order = nx, ny, int(sphere), order
+        d.limiter = LIMITER[limiter] if isinstance(limiter, str) else limiter
+        d.g, d.min_depth, d.eps4, d.cfl = g, min_depth, eps4, cfl
+        d.coriolis, d.manning, d.R = coriolis, manning, R
+        for k, r in enumerate(ROWS):
+            d.rows[k] = _dp(self._rows[r])
+        h = ctypes.c_void_p()
+        rc = L.swe_create(ctypes.byref(d), ctypes.byref(h))
+        if rc != 0:
+            raise SweError(f"swe_create: {L.swe_last_error(None).decode()}")
+        self._h = h
+        self._mf = 0.0
+        self._probe_rh = False
+
+    def _ck(self, rc, what):
+        if rc != 0:
+            raise SweError(f"{what}: {lib().swe_last_error(self._h).decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().swe_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state --------------------------------------------------------------
+    def set_field(self, f, a):
+        if a is None:
+            self._ck(lib().swe_set_field(self._h, f, None), "swe_set_field")
+            return
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.size == self.N
+        self._ck(lib().swe_set_field(self._h, f, _dp(a)), "swe_set_field")
+
+    def get_field(self, f):
+        out = np.empty(self.N)
+        self._ck(lib().swe_get_field(self._h, f, _dp(out)), "swe_get_field")
+        return out
+
+    field = get_field
+
+    def load_state(self, b, h, hu, hv):
+        for f, a in ((F_B, b), (F_H, h), (F_HU, hu), (F_HV, hv)):
+            self.set_field(f, a)
+
+    def state(self):
+        return tuple(self.get_field(f) for f in (F_B, F_H, F_HU, F_HV))
+
+    @property
+    def h(self):
+        return self.get_field(F_H)
+
+    @property
+    def hu(self):
+        return self.get_field(F_HU)
+
+    @property
+    def hv(self):
+        return self.get_field(F_HV)
+
+    @property
+    def b(self):
+        return self.get_field(F_B)
+
+    def set_manning_field(self, n):
+        self.set_field(F_MANNING, n)
+
+    def set_sponge(self, strength, eta_ref=0.0):
+        self.set_field(F_SPONGE, strength)
+        self.set_scalar(S_SPONGE_REF, eta_ref)
+
+    def set_row(self, which, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self._ck(lib().swe_set_row(self._h, ROWS.index(which) if isinstance(which, str) else which, _dp(a)),
+                 "swe_set_row")
+
+    def set_bc(self, kinds):
+        for side, k in enumerate(kinds):
+            self._ck(lib().swe_set_bc(self._h, side, BC[k] if isinstance(k, str) else k), "swe_set_bc")
+
+    def set_scalar(self, which, v):
+        self._ck(lib().swe_set_scalar(self._h, which, float(v)), "swe_set_scalar")
+
+    def get_scalar(self, which):
+        v = ctypes.c_double()
+        self._ck(lib().swe_get_scalar(self._h, which, ctypes.byref(v)), "swe_get_scalar")
+        return v.value
+
+    @property
+    def t(self):
+        return self.get_scalar(S_T)
+
+    def set_time(self, t, steps=0):
+        self.set_scalar(S_T, t)
+        self.set_scalar(S_STEPS, steps)
+
+    # ---- the step ---------------------------------------------------------------
+    def apply_bc(self, ext=None):
+        e = _ext(ext)
+        self._ck(lib().swe_apply_bc(self._h, _dp(e)), "swe_apply_bc")
+
+    def residual(self, dt):
+        self._ck(lib().swe_residual(self._h, float(dt)), "swe_residual")
+
+    def max_dt(self):
+        v = ctypes.c_double()
+        self._ck(lib().swe_max_dt(self._h, ctypes.byref(v)), "swe_max_dt")
+        return v.value
+
+    def step(self, dt=None, ext=None, want_theta=False, want_rh=False):
+        e = _ext(ext)
+        taken, mf = ctypes.c_double(), ctypes.c_double()
+        self._ck(lib().swe_step(self._h, math.nan if dt is None else float(dt), _dp(e), int(want_theta),
+                                int(want_rh), ctypes.byref(taken), ctypes.byref(mf)), "swe_step")
+        self._mf = mf.value
+        return taken.value
+
+    def sponge(self, dt, ref):
+        """The sponge forcing runs inside step() on the device (set_sponge)."""
+
+    def advance(self, nsteps, dt_mode=DT_SELF, dt_value=0.0, t_end=math.inf):
+        n = ctypes.c_int64()
+        self._ck(lib().swe_advance(self._h, int(nsteps), dt_mode, float(dt_value), float(t_end), ctypes.byref(n)),
+                 "swe_advance")
+        return n.value
+
+    def last_advance_ms(self):
+        v = ctypes.c_double()
+        self._ck(lib().swe_last_advance_ms(self._h, ctypes.byref(v)), "swe_last_advance_ms")
+        return v.value
+
+    def set_harmonic(self, side, amp=(), omega=(), phase=(), msl=0.0, uamp=(), uomega=(), uphase=(), u0=0.0):
+        def arr(x):
+            a = np.zeros(4)
+            a[:len(x)] = x
+            return a
+        A, O, P, UA, UO, UP = (arr(x) for x in (amp, omega, phase, uamp, uomega, uphase))
+        self._keep = (A, O, P, UA, UO, UP)
+        self._ck(lib().swe_set_harmonic(self._h, side, len(amp), msl, _dp(A), _dp(O), _dp(P), len(uamp), u0,
+                                        _dp(UA), _dp(UO), _dp(UP)), "swe_set_harmonic")
+
+    def profile_step(self, nsteps):
+        """Mean kernel durations (ms) [stage 1, stage 2, finalize] over nsteps device steps."""
+        out = np.zeros(3)
+        self._ck(lib().swe_profile_step(self._h, int(nsteps), _dp(out)), "swe_profile_step")
+        return out
+
+    def use_graph(self, on=True):
+        self._ck(lib().swe_use_graph(self._h, int(on)), "swe_use_graph")
+
+    def set_chunk_rows(self, rows):
+        self._ck(lib().swe_set_chunk_rows(self._h, int(rows)), "swe_set_chunk_rows")
+
+    def sync(self):
+        self._ck(lib().swe_sync(self._h), "swe_sync")
+
+    # ---- diagnostics over the host mirror (swe.mjs:819-917) ---------------------
+    @property
+    def mass_floored(self):
+        return self._mf
+
+    def finite(self):
+        _, h, hu, hv = self.state()
+        return bool(np.all(np.isfinite(h)) and np.all(np.isfinite(hu)) and np.all(np.isfinite(hv)))
+
+    def fingerprint(self):
+        """FNV-1a 64 over the raw bytes of h, hu, hv (full arrays, ghosts included)."""
+        _, h, hu, hv = self.state()
+        return fnv1a64(h, hu, hv)
+
+    def interior(self, a):
+        return a.reshape(self.H, self.W)[2:-2, 2:-2]
+
+
+_FNV_P = np.uint64(0x100000001b3)
+
+
+def fnv1a64(*arrays):
+    h = 0xcbf29ce484222325
+    data = b"".join(np.ascontiguousarray(a, dtype="<f8").tobytes() for a in arrays)
+    buf = np.frombuffer(data, dtype=np.uint8)
+    # vectorised over bytes is inherently sequential; process in Python ints (small grids only)
+    P = 0x100000001b3
+    M = 0xFFFFFFFFFFFFFFFF
+    for byte in buf.tolist():
+        h ^= byte
+        h = (h * P) & M
+    return h
+
+
+# ---- geometry (src/geometry.mjs) -----------------------------------------------
+R_EARTH = 6371e3
+OMEGA_EARTH = 7.292115e-5
+
+
+def geometry_cartesian(nx, ny, dx, dy=None, coriolis=0.0, ng=2):
+    """cartesianGeometry (geometry.mjs:163-176): every row ASSIGNED the scalars."""
+    dy = dx if dy is None else dy
+    H = ny + 2 * ng
+    rows = {r: np.zeros(H) for r in ROWS}
+    rows["dxRow"][:] = dx
+    rows["dyRowN"][:] = dy
+    rows["dyRowS"][:] = dy
+    rows["dyCFL"][:] = dy
+    rows["bedPhi"][:] = dy
+    rows["area"][:] = dx * dy
+    rows["fRow"][:] = coriolis
+    return rows
+
+
+def _cos_lat(p):
+    co = math.pi / 2 - abs(p)
+    return 0.0 if co <= 0 else math.sin(co)
+
+
+def geometry_spherical(nx, ny, R=R_EARTH, lat0=-90.0, lat1=90.0, omega=0.0, ng=2):
+    """sphericalGeometry (geometry.mjs:190-299): product-form area, divisors as divisors,
+    Infinity at a pole face, reflected latitudes for rows beyond a pole.  Transcendentals
+    come from the C library, so rows can differ from the JS engine's in the last ulp."""
+    if not lat1 > lat0:
+        raise ValueError(f"lat1 ({lat1}) must exceed lat0 ({lat0})")
+    if lat0 < -90 or lat1 > 90:
+        raise ValueError(f"latitudes must lie in [-90, 90]; got {lat0}..{lat1}")
+    touches = lat0 <= -90 + 1e-12 or lat1 >= 90 - 1e-12
+    if touches and nx % 2 != 0:
+        raise ValueError(f"nx must be even when the domain reaches a pole (got {nx})")
+    H = ny + 2 * ng
+    rows = {r: np.zeros(H) for r in ROWS}
+    dlam = 2 * math.pi / nx
+    p0, p1 = lat0 * math.pi / 180, lat1 * math.pi / 180
+    dphi = (p1 - p0) / ny
+    half_pi = math.pi / 2
+
+    def face_lat(k):
+        return p0 if k == 0 else p1 if k == ny else p0 + k * dphi
+
+    def clamp(p):
+        return half_pi if p > half_pi else -half_pi if p < -half_pi else p
+
+    def reflect(p):
+        return math.pi - p if p > half_pi else -math.pi - p if p < -half_pi else p
+
+    dy = R * dphi
+    for j in range(H):
+        jj = j - ng
+        rawS, rawN = face_lat(jj), face_lat(jj + 1)
+        pS, pN = clamp(rawS), clamp(rawN)
+        pC = 0.5 * (pS + pN)
+        half = 0.5 * (pN - pS)
+        lyS = R * dlam * _cos_lat(pS)
+        lyN = R * dlam * _cos_lat(pN)
+        lx = R * dphi
+        A = 2 * R * R * dlam * _cos_lat(pC) * math.sin(half)
+        if not A > 0:
+            rS, rN = reflect(rawS), reflect(rawN)
+            A = abs(2 * R * R * dlam * _cos_lat(0.5 * (rS + rN)) * math.sin(0.5 * (rN - rS)))
+        rows["area"][j] = A
+        rows["dxRow"][j] = A / lx
+        rows["dyRowN"][j] = A / lyN if lyN > 0 else math.inf
+        rows["dyRowS"][j] = A / lyS if lyS > 0 else math.inf
+        rows["dyCFL"][j] = min(rows["dyRowN"][j], rows["dyRowS"][j], dy)
+        rows["bedPhi"][j] = 2 * A / (lyN + lyS) if (lyN + lyS) > 0 else math.inf
+        rows["tanPhi"][j] = math.tan(pC)
+        rows["fRow"][j] = 2 * omega * math.sin(pC)
+        rows["geoCoef"][j] = (lyN - lyS) / (4 * A)
+    return rows

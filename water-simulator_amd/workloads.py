@@ -1,0 +1,79 @@
+"""Seeded synthetic workloads of the benchmark configurations.
+
+Every random number comes from SplitMix64 (Steele, Lea & Flood, 2014) seeded per
+configuration, mapped to doubles in [0,1) by the top 53 bits; normals by
+Box-Muller.  Inputs are built on the host with numpy and uploaded once.
+"""
+import math
+
+import numpy as np
+
+G_REF = 9.80665
+M2_PERIOD_S = 12.42 * 3600.0
+
+
+class SplitMix64:
+    def __init__(self, seed):
+        self.x = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next(self):
+        self.x = (self.x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = self.x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    def uniform(self, lo=0.0, hi=1.0):
+        return lo + (hi - lo) * ((self.next() >> 11) * (1.0 / 9007199254740992.0))
+
+    def normals(self, n):
+        """n standard normals (vectorised Box-Muller over SplitMix64 draws)."""
+        m = (n + 1) // 2
+        raw = np.empty(2 * m, dtype=np.uint64)
+        # vectorised SplitMix64: states x + k*gamma for k = 1..2m
+        gamma = np.uint64(0x9E3779B97F4A7C15)
+        ks = np.arange(1, 2 * m + 1, dtype=np.uint64)
+        with np.errstate(over="ignore"):
+            z = np.uint64(self.x) + ks * gamma
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        raw[:] = z
+        self.x = (self.x + 2 * m * 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        u = (raw >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+        u1 = np.maximum(u[0::2], 1e-300)
+        u2 = u[1::2]
+        r = np.sqrt(-2.0 * np.log(u1))
+        out = np.empty(2 * m)
+        out[0::2] = r * np.cos(2 * np.pi * u2)
+        out[1::2] = r * np.sin(2 * np.pi * u2)
+        return out[:n]
+
+
+def large_grid(nx=4096, ny=4096, dx=10.0, seed=2):
+    """4096x4096 Cartesian, dx = 10 m: a plane beach from -30 m to +3 m along x, 64 Gaussian
+    bumps (height U(-5,5) m, sigma U(50,400) m), N(0,0.2) m noise; eta0 = 0, u = v = 0.
+    West Flather tide 1 m, period 12.42 h; Manning 0.025; f = 1e-4; CFL 0.45.
+    Returns (b, h, hu, hv) over (ny+4)*(nx+4) cells and the run parameters."""
+    rng = SplitMix64(seed)
+    W, H = nx + 4, ny + 4
+    x = (np.arange(W) - 2 + 0.5) * dx
+    y = (np.arange(H) - 2 + 0.5) * dx
+    Lx, Ly = nx * dx, ny * dx
+    b = np.empty((H, W))
+    b[:] = -30.0 + 33.0 * (x / Lx)
+    for _ in range(64):
+        cx, cy = rng.uniform(0, Lx), rng.uniform(0, Ly)
+        amp, sig = rng.uniform(-5, 5), rng.uniform(50, 400)
+        i0, i1 = np.searchsorted(x, cx - 5 * sig), np.searchsorted(x, cx + 5 * sig)
+        j0, j1 = np.searchsorted(y, cy - 5 * sig), np.searchsorted(y, cy + 5 * sig)
+        gx = np.exp(-((x[i0:i1] - cx) / sig) ** 2)
+        gy = np.exp(-((y[j0:j1] - cy) / sig) ** 2)
+        b[j0:j1, i0:i1] += amp * np.outer(gy, gx)
+    b += 0.2 * rng.normals(H * W).reshape(H, W)
+    h = np.maximum(0.0, 0.0 - b)
+    z = np.zeros(H * W)
+    params = dict(nx=nx, ny=ny, dx=dx, manning=0.025, coriolis=1e-4, cfl=0.45,
+                  bc=("flather", "reflect", "reflect", "reflect"),
+                  tide_amp=1.0, tide_omega=2 * math.pi / M2_PERIOD_S)
+    return (b.ravel(), h.ravel(), z, z.copy()), params

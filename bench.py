@@ -51,6 +51,20 @@ def cpu_baseline(seconds=12.0):
                 sample=f"oracle C restatement, 256x256 sample of the 4096x4096 recipe, {n} self-timed steps in {el:.1f} s")
 
 
+def committed_traffic(kernel):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary (FETCH_SIZE and
+    WRITE_SIZE, each scaled by a plain-copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "k_sweep_large_grid_4096.md")
+    if not os.path.exists(path):
+        return None
+    key = f"traffic_bytes_per_launch {kernel}:"
+    with open(path) as f:
+        for line in f:
+            if line.startswith(key):
+                return float(line.split(":", 1)[1])
+    return None
+
+
 def p_dx():
     return 10.0
 
@@ -100,11 +114,7 @@ def main():
     W, H = nx + 4, ny + 4
     bytes_s2 = 80.0 * W * H            # reads A(h,hu,hv) + B(h,hu,hv) + b, writes C(h,hu,hv)
     achieved = bytes_s2 / (ms[1] * 1e-3) / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_k_sweep_stage2_4096.json")
-    if os.path.exists(tf):
-        with open(tf) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+    traffic = committed_traffic("k_sweep<2>")
     finite = sim.get_scalar(swe_amd.S_NONFINITE) == 0
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": n,

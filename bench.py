@@ -76,7 +76,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
-    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=-1, help="rows per wavefront chunk; -1 autotune, 0 default")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -94,8 +94,10 @@ def main():
     del st
     sim.set_bc(p["bc"])
     sim.set_harmonic(0, amp=[p["tide_amp"]], omega=[p["tide_omega"]], phase=[0.0])
-    if args.chunk:
+    if args.chunk > 0:
         sim.set_chunk_rows(args.chunk)
+    elif args.chunk < 0:
+        log(f"[bench] autotuned chunk rows: {sim.autotune()}")
     sim.use_graph(True)
     log(f"[bench] warmup {args.warmup} steps")
     sim.advance(args.warmup)
@@ -122,7 +124,7 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded SplitMix64 bed), state resident in HBM",
         "config": {"workload": "large_grid_4096x4096_dx10_flather_tide_manning0.025_f1e-4_cfl0.45",
                    "nx": nx, "ny": ny, "dt": "self-timed (maxDt each step)", "graph": True,
-                   "chunk_rows": args.chunk or "default"},
+                   "chunk_rows": sim.chunk_rows()},
         "hbm_pct_algorithmic_56B": 100.0 * 56.0 * value / (HBM_PEAK_GBS * 1e9),
         "kernel_ms": {"stage1": ms[0], "stage2": ms[1], "finalize": ms[2]},
         "roofline": {"bound": "hbm", "kernel": "k_sweep<stage2>", "achieved": achieved, "peak": HBM_PEAK_GBS,

@@ -45,7 +45,7 @@ enum swe_row { SWE_ROW_DX = 0, SWE_ROW_DYN = 1, SWE_ROW_DYS = 2, SWE_ROW_DYCFL =
 /* scalars settable after creation */
 enum swe_scalar { SWE_S_MANNING = 0, SWE_S_CORIOLIS = 1, SWE_S_CFL = 2, SWE_S_T = 3,
                   SWE_S_SPONGE_REF = 4, SWE_S_STEPS = 5, SWE_S_MASS_FLOORED = 6,
-                  SWE_S_DT_NEXT = 7, SWE_S_NONFINITE = 8 };
+                  SWE_S_DT_NEXT = 7, SWE_S_NONFINITE = 8, SWE_S_CHUNK_ROWS = 9 };
 /* dt policy of swe_advance */
 enum swe_dt_mode { SWE_DT_SELF = 0, SWE_DT_FIXED = 1, SWE_DT_UNTIL = 2 };
 
@@ -76,6 +76,10 @@ const char *swe_last_error(const swe_ctx *ctx);
  * algorithm behind the JS engine's Math.pow to within 1 ulp); the host uses it
  * for eps4 = max(minDepth,1e-12)**4 (swe.mjs:190). */
 double swe_js_pow(double x, double y);
+/* hllc(out, hL, huL, hvL, hR, huR, hvR, dry) (swe.mjs:974-1011): the interface flux
+ * the kernels use, compiled for the host; the JS module's `hllc` export calls it. */
+void swe_hllc(double out[3], double hL, double huL, double hvL, double hR, double huR, double hvR,
+              double dry, double g);
 /* version string and the gfx target the library was built for */
 const char *swe_build_info(void);
 
@@ -96,6 +100,12 @@ int swe_apply_bc(swe_ctx *ctx, const double ext[8]);
 /* residual(dt) (swe.mjs:308-629) on the stored state, ghost cells as stored;
  * fills SWE_F_RH/RHU/RHV/THETA. */
 int swe_residual(swe_ctx *ctx, double dt);
+/* applyFriction(dt) (swe.mjs:659-704) on the stored state: semi-implicit Manning,
+ * scalar n or the per-cell field. */
+int swe_apply_friction(swe_ctx *ctx, double dt);
+/* dryClean() (swe.mjs:734-745) on the stored state; *added = mass invented by
+ * flooring h (summed per block, then in block order). */
+int swe_dry_clean(swe_ctx *ctx, double *added);
 /* maxDt() (swe.mjs:632-647); +inf when every interior cell is dry */
 int swe_max_dt(swe_ctx *ctx, double *out);
 /* step(dt) (swe.mjs:754-814) without the host forcing hook: dt NaN = self-timed.
@@ -123,6 +133,11 @@ int swe_set_chunk_rows(swe_ctx *ctx, int rows);
  * steps with HIP events around each kernel on the context's stream and return
  * the mean duration in ms of [stage-1 sweep, stage-2 sweep, finalize]. */
 int swe_profile_step(swe_ctx *ctx, int nsteps, double out_ms[3]);
+/* Autotuner (no reference counterpart): time the two stage kernels on the
+ * current state for a set of row-chunk heights (fixed ones and ones that fill
+ * whole rounds of the device's resident wavefront slots), keep the fastest and
+ * return it.  Leaves the state, t and the step count untouched. */
+int swe_autotune(swe_ctx *ctx, int reps, int *best_rows);
 /* block until all work on the context's stream is done */
 int swe_sync(swe_ctx *ctx);
 /* device pointer of a field of the current state (for zero-copy callers) */

@@ -1,0 +1,39 @@
+"""The Node drop-in module (water-simulator_amd/js/swe.mjs over the N-API addon).
+
+CPU: its host pieces (hllc export, metric rows, sponge strengths) against the
+reference's recorded values.  GPU: every golden scenario replayed through the
+drop-in end to end, and the restated verification suites run against it.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+NODE = shutil.which("node")
+needs_node = pytest.mark.skipif(NODE is None, reason="node is not installed")
+ADDON = os.path.join(ROOT, "water-simulator_amd", "napi", "swe_napi.node")
+
+
+def run_node(script, *args, timeout=600):
+    print(f"[node] {script} {' '.join(args)}", flush=True)
+    r = subprocess.run([NODE, os.path.join(ROOT, script), *args], cwd=ROOT, capture_output=True, text=True,
+                       timeout=timeout)
+    sys.stdout.write(r.stdout[-6000:])
+    sys.stderr.write(r.stderr[-3000:])
+    return r.returncode
+
+
+@needs_node
+def test_js_host_pieces():
+    if not os.path.exists(ADDON):
+        pytest.skip("addon not built")
+    assert run_node("tests/js/cpu_checks.mjs") == 0
+
+
+@needs_node
+@pytest.mark.gpu
+def test_js_golden_replay_on_gpu():
+    assert run_node("tests/js/golden_replay.mjs") == 0

@@ -15,9 +15,12 @@
 namespace swe {
 hipError_t launch_sweep(int mode, const SweepArgs &a, hipStream_t st);
 int sweep_waves(const Params &P, int rows_per_chunk);
+const void *k_sweep_stage1_ptr();
 hipError_t launch_apply_bc(const Params &P, StateW S, double *b, const Ctl *ctl, hipStream_t st);
 hipError_t launch_maxdt(const Params &P, StateP S, Ctl *ctl, hipStream_t st);
 hipError_t launch_finalize(Ctl *ctl, const double *mf_part, int nparts, double cfl, int phase, hipStream_t st);
+hipError_t launch_friction(const Params &P, StateW S, const double *mfield, double dt, hipStream_t st);
+hipError_t launch_dry_clean(const Params &P, StateW S, double *part, int blocks, hipStream_t st);
 }  // namespace swe
 
 using namespace swe;
@@ -122,11 +125,13 @@ static int upload_row(swe_ctx *c, int row, const double *host) {
 
 static int default_chunk(const Params &P) {
   const int strips = (P.W + SWEEP_OUT - 1) / SWEEP_OUT;
-  const int target_waves = 4096;               // about 4 per SIMD on 256 CUs
+  // about two rounds of the 256 CU x 4 SIMD x 3 resident wavefronts the
+  // stage kernels' registers allow; swe_autotune refines this on the device
+  const int target_waves = 6144;
   int chunks = (target_waves + strips - 1) / strips;
   if (chunks < 1) chunks = 1;
   int rows = (P.H + chunks - 1) / chunks;
-  if (rows < 32) rows = 32;
+  if (rows < 24) rows = 24;
   if (rows > P.H) rows = P.H;
   if (rows < MIN_CHUNK) rows = MIN_CHUNK;
   return rows;
@@ -142,6 +147,13 @@ double swe_js_pow(double x, double y) {
   if (y == 0.5 && x >= 0.0) return std::sqrt(x);
   if (!(x > 0.0) || std::isinf(x) || std::isinf(y)) return std::pow(x, y);
   return swe::js_pow_pos(x, y);
+}
+
+void swe_hllc(double out[3], double hL, double huL, double hvL, double hR, double huR, double hvR,
+              double dry, double g) {
+  const HllcConst C = {g, 0.5 * g, 1.0 / g, dry};
+  const Flux3 f = swe::hllc(hL, huL, hvL, hR, huR, hvR, C);
+  out[0] = f.m; out[1] = f.n; out[2] = f.t;
 }
 
 const char *swe_build_info(void) { return "water-simulator_amd 0.1 gfx950 fp64"; }
@@ -348,6 +360,7 @@ int swe_get_scalar(swe_ctx *c, int which, double *v) {
     case SWE_S_MASS_FLOORED: *v = c->hctl->mf; break;
     case SWE_S_DT_NEXT: *v = c->hctl->dt_next; break;
     case SWE_S_NONFINITE: *v = c->hctl->nonfinite; break;
+    case SWE_S_CHUNK_ROWS: *v = c->chunk; break;
     default: return fail(c, SWE_EINVAL, "unknown scalar");
   }
   return SWE_OK;
@@ -398,6 +411,31 @@ int swe_residual(swe_ctx *c, double dt) {
   return SWE_OK;
 }
 
+int swe_apply_friction(swe_ctx *c, double dt) {
+  if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  if (c->P.friction == 0) return SWE_OK;                  // `if (!n) return;` (swe.mjs:689)
+  StateW S = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
+  HIPCHK(c, launch_friction(c->P, S, c->mfield, dt, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SWE_OK;
+}
+
+int swe_dry_clean(swe_ctx *c, double *added) {
+  if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  int blocks = (c->P.nx * c->P.ny + 255) / 256;
+  if (blocks > 2 * c->max_parts) blocks = 2 * c->max_parts;
+  if (blocks < 1) blocks = 1;
+  StateW S = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
+  HIPCHK(c, launch_dry_clean(c->P, S, c->mf_part, blocks, c->stream));
+  std::vector<double> part(blocks);
+  HIPCHK(c, hipMemcpyAsync(part.data(), c->mf_part, sizeof(double) * blocks, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double s = 0.0;
+  for (double p : part) s += p;
+  if (added) *added = s;
+  return SWE_OK;
+}
+
 int swe_max_dt(swe_ctx *c, double *out) {
   if (!c || !out) return fail(c, SWE_EINVAL, "null argument");
   int rc = pull_ctl(c);
@@ -415,17 +453,23 @@ int swe_max_dt(swe_ctx *c, double *out) {
   return push_ctl(c);
 }
 
-// one step on the device, stream-ordered; the caller handles the ctl block
+// one step on the device, stream-ordered; the caller handles the ctl block.
+// applyBC before each stage (swe.mjs:784,798) writes the ghost ring of that
+// stage's input; a refused step still gets its applyBC, as in the reference.
 static int enqueue_step(swe_ctx *c, int want_theta, int want_rh) {
   const int np = sweep_waves(c->P, c->chunk);
   if (np > c->max_parts) return fail(c, SWE_EINVAL, "chunk too small");
+  StateW SA = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
+  StateW SB = {c->st[c->Bi][0], c->st[c->Bi][1], c->st[c->Bi][2]};
+  HIPCHK(c, launch_apply_bc(c->P, SA, c->b, c->dctl, c->stream));
   SweepArgs a = base_args(c);
-  a.in = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
-  a.out = {c->st[c->Bi][0], c->st[c->Bi][1], c->st[c->Bi][2]};
+  a.in = {SA.h, SA.hu, SA.hv};
+  a.out = SB;
   HIPCHK(c, launch_sweep(MODE_STAGE1, a, c->stream));
+  HIPCHK(c, launch_apply_bc(c->P, SB, c->b, c->dctl, c->stream));
   SweepArgs b2 = base_args(c);
-  b2.in = {c->st[c->Bi][0], c->st[c->Bi][1], c->st[c->Bi][2]};
-  b2.h0 = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
+  b2.in = {SB.h, SB.hu, SB.hv};
+  b2.h0 = {SA.h, SA.hu, SA.hv};
   b2.out = {c->st[c->C][0], c->st[c->C][1], c->st[c->C][2]};
   b2.mf_part = c->mf_part + np;
   if (want_theta || want_rh) {
@@ -464,10 +508,6 @@ int swe_step(swe_ctx *c, double dt, const double ext[8], int want_theta, int wan
   if ((rc = enqueue_step(c, want_theta, want_rh))) return rc;
   if ((rc = pull_ctl(c))) return rc;
   const double taken = h->dt_last;
-  if (!(taken > 0)) {
-    // refused step: the reference has still run applyBC (swe.mjs:782-784)
-    if ((rc = swe_apply_bc(c, ext))) return rc;
-  }
   if (dt_taken) *dt_taken = taken;
   if (mass_floored) *mass_floored = h->mf;
   h->dt_mode = DT_HOST;
@@ -506,6 +546,83 @@ int swe_set_chunk_rows(swe_ctx *c, int rows) {
   if (rows < MIN_CHUNK) return fail(c, SWE_EINVAL, "chunk rows below minimum (8)");
   if (rows != c->chunk) drop_graphs(c);
   c->chunk = rows;
+  return SWE_OK;
+}
+
+// Times stage 1 + stage 2 of the current state for one chunk height without
+// touching the state: A is only read, B and C are scratch, t and the step count
+// are not advanced (no finalize).
+static int time_chunk(swe_ctx *c, int rows, int reps, float *ms) {
+  c->chunk = rows;
+  const int np = sweep_waves(c->P, rows);
+  if (np > c->max_parts) return fail(c, SWE_EINVAL, "chunk too small");
+  StateW SA = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
+  StateW SB = {c->st[c->Bi][0], c->st[c->Bi][1], c->st[c->Bi][2]};
+  SweepArgs a = base_args(c);
+  a.in = {SA.h, SA.hu, SA.hv};
+  a.out = SB;
+  SweepArgs b2 = base_args(c);
+  b2.in = {SB.h, SB.hu, SB.hv};
+  b2.h0 = {SA.h, SA.hu, SA.hv};
+  b2.out = {c->st[c->C][0], c->st[c->C][1], c->st[c->C][2]};
+  b2.mf_part = c->mf_part + np;
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  for (int r = 0; r < reps; ++r) {
+    HIPCHK(c, launch_sweep(MODE_STAGE1, a, c->stream));
+    HIPCHK(c, launch_sweep(MODE_STAGE2, b2, c->stream));
+  }
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  HIPCHK(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
+  return SWE_OK;
+}
+
+int swe_autotune(swe_ctx *c, int reps, int *best_rows) {
+  if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  if (reps < 1) reps = 1;
+  int rc = pull_ctl(c);
+  if (rc) return rc;
+  // a valid dt for the trial sweeps: the CFL step of the current state
+  const int saved_mode = c->hctl->dt_mode;
+  const double saved_dt = c->hctl->dt;
+  c->hctl->dt_mode = DT_SELF;
+  c->hctl->cfl_bits = 0;
+  if ((rc = push_ctl(c))) return rc;
+  StateP S = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
+  HIPCHK(c, launch_maxdt(c->P, S, c->dctl, c->stream));
+  HIPCHK(c, launch_finalize(c->dctl, c->mf_part, 0, c->P.cfl, 0, c->stream));
+  StateW SA = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
+  HIPCHK(c, launch_apply_bc(c->P, SA, c->b, c->dctl, c->stream));
+  // resident wavefront slots: CUs x 4 SIMDs x waves per SIMD the kernel's registers allow
+  int cus = 0, per_cu = 0;
+  HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_stage1_ptr(), SWEEP_LANES, 0));
+  const int slots = cus * (per_cu > 0 ? per_cu : 4);
+  const int strips = (c->P.W + SWEEP_OUT - 1) / SWEEP_OUT;
+  std::vector<int> cand = {16, 24, 32, 40, 48, 64, 96};
+  for (int rounds = 1; rounds <= 4; ++rounds) {
+    const int chunks = (rounds * slots + strips - 1) / strips;
+    if (chunks >= 1) cand.push_back((c->P.H + chunks - 1) / chunks);
+  }
+  const int old = c->chunk;
+  int best = old;
+  float best_ms = 1e30f;
+  for (int rows : cand) {
+    if (rows < MIN_CHUNK || rows > c->P.H || sweep_waves(c->P, rows) > c->max_parts) continue;
+    float ms = 0.f;
+    if ((rc = time_chunk(c, rows, 1, &ms))) { c->chunk = old; return rc; }   // warm
+    if ((rc = time_chunk(c, rows, reps, &ms))) { c->chunk = old; return rc; }
+    if (ms < best_ms) { best_ms = ms; best = rows; }
+  }
+  c->chunk = best;
+  if (best != old) drop_graphs(c);
+  if ((rc = pull_ctl(c))) return rc;
+  c->hctl->dt_mode = saved_mode;
+  c->hctl->dt = saved_dt;
+  c->hctl->cfl_bits = 0;
+  if ((rc = push_ctl(c))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (best_rows) *best_rows = best;
   return SWE_OK;
 }
 
@@ -587,9 +704,11 @@ int swe_profile_step(swe_ctx *c, int nsteps, double out_ms[3]) {
     b2.h0 = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
     b2.out = {c->st[c->C][0], c->st[c->C][1], c->st[c->C][2]};
     b2.mf_part = c->mf_part + np;
+    HIPCHK(c, launch_apply_bc(c->P, {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]}, c->b, c->dctl, c->stream));
     HIPCHK(c, hipEventRecord(ev[4 * n], c->stream));
     HIPCHK(c, launch_sweep(MODE_STAGE1, a, c->stream));
     HIPCHK(c, hipEventRecord(ev[4 * n + 1], c->stream));
+    HIPCHK(c, launch_apply_bc(c->P, {c->st[c->Bi][0], c->st[c->Bi][1], c->st[c->Bi][2]}, c->b, c->dctl, c->stream));
     HIPCHK(c, launch_sweep(MODE_STAGE2, b2, c->stream));
     HIPCHK(c, hipEventRecord(ev[4 * n + 2], c->stream));
     HIPCHK(c, launch_finalize(c->dctl, c->mf_part, np, c->P.cfl, 1, c->stream));

@@ -16,15 +16,15 @@ constexpr double SQRT2 = 1.4142135623730951;   // Math.SQRT2
 
 // ---- JS Math semantics ----------------------------------------------------
 // Math.max(0, x): NaN propagates, -0 becomes +0
-__device__ __forceinline__ double jmax0(double x) { return (x > 0.0 || x != x) ? x : 0.0; }
-__device__ __forceinline__ double jmax(double a, double b) {
+__host__ __device__ __forceinline__ double jmax0(double x) { return (x > 0.0 || x != x) ? x : 0.0; }
+__host__ __device__ __forceinline__ double jmax(double a, double b) {
   if (a != a || b != b) return __builtin_nan("");
   if (a > b) return a;
   if (b > a) return b;
   if (a == 0.0) return (__builtin_signbit(a) && __builtin_signbit(b)) ? -0.0 : 0.0;
   return a;
 }
-__device__ __forceinline__ double jmin(double a, double b) {
+__host__ __device__ __forceinline__ double jmin(double a, double b) {
   if (a != a || b != b) return __builtin_nan("");
   if (a < b) return a;
   if (b < a) return b;
@@ -36,7 +36,7 @@ __device__ __forceinline__ double jmin(double a, double b) {
 // x / d given r = RN(1/d): q = RN(x r); e = x - q d (exact by FMA); RN(q + e r)
 // is RN(x/d) for d, x, x/d normal (Markstein).  A zero quotient keeps the sign
 // of x*r (what x/d gives); an infinite d (a pole face) takes the IEEE divide.
-__device__ __forceinline__ double div_r(double x, double d, double r) {
+__host__ __device__ __forceinline__ double div_r(double x, double d, double r) {
   if (r == 0.0) return x / d;
   const double q = x * r;
   const double e = __builtin_fma(-q, d, x);
@@ -46,7 +46,7 @@ __device__ __forceinline__ double div_r(double x, double d, double r) {
 
 // ---- vel(): swe.mjs:282-285.  Both components share one denominator. --------
 struct Vel2 { double u, v; };
-__device__ __forceinline__ double vel_den(double h, double eps4) {
+__host__ __device__ __forceinline__ double vel_den(double h, double eps4) {
   const double h4 = h * h * h * h;
   return __builtin_sqrt(h4 + jmax(h4, eps4));
 }
@@ -59,7 +59,7 @@ __device__ __forceinline__ Vel2 vel2(double hu, double hv, double h, double eps4
   o.v = div_r(s * hv, den, r);
   return o;
 }
-__device__ __forceinline__ double vel1(double hq, double h, double eps4) {
+__host__ __device__ __forceinline__ double vel1(double hq, double h, double eps4) {
   return (SQRT2 * h * hq) / vel_den(h, eps4);
 }
 
@@ -84,7 +84,7 @@ __device__ __forceinline__ double limit(int lim, double a, double b) {
 struct Flux3 { double m, n, t; };
 struct HllcConst { double g, half_g, rg, dry; };   // rg = RN(1/g)
 
-__device__ __forceinline__ Flux3 hllc(double hL, double huL, double hvL,
+__host__ __device__ __forceinline__ Flux3 hllc(double hL, double huL, double hvL,
                                       double hR, double huR, double hvR, const HllcConst &C) {
   Flux3 out;
   const bool wetL = hL > C.dry, wetR = hR > C.dry;

@@ -1,5 +1,9 @@
 // HIP kernels (gfx950) of the shallow-water time step of src/swe.mjs.
 //
+// k_apply_bc runs before each stage and leaves the ghost ring of the stage's
+// input exactly as the reference's applyBC() does; the sweeps then read ghost
+// cells like any other cell.
+//
 // k_sweep: one fused stage of SSP-RK2 (swe.mjs:754-814 stage bodies, with
 //   residual() :308-629, dryClean() :734-745, and in stage 2 applyFriction()
 //   :659-704, the sponge forcing :1188-1197 and the next step's maxDt()
@@ -78,7 +82,7 @@ __device__ __forceinline__ Cell bc_apply(int kind, int side, const Cell &s, cons
 }
 
 // Ghost of a W/E column at an interior row, as applyBC's first pass leaves it.
-__device__ __noinline__ Cell ghost_we(const Params &P, const StateP &S, const double *b, const double *ext,
+__device__ __forceinline__ Cell ghost_we(const Params &P, const StateP &S, const double *b, const double *ext,
                                       int gi, int gj) {
   const int side = gi < 2 ? 0 : 1;
   const int kind = P.bc[side];
@@ -92,7 +96,7 @@ __device__ __noinline__ Cell ghost_we(const Params &P, const StateP &S, const do
 // Any cell after applyBC: interior as stored, W/E ghosts from the first pass,
 // S/N ghost rows (corners included) from the second pass, whose periodic source
 // may itself be a W/E ghost.
-__device__ __noinline__ Cell ghost_cell(const Params &P, const StateP &S, const double *b, const double *ext,
+__device__ __forceinline__ Cell ghost_cell(const Params &P, const StateP &S, const double *b, const double *ext,
                                         int gi, int gj) {
   if (gj >= 2 && gj < P.H - 2) return ghost_we(P, S, b, ext, gi, gj);
   const int side = gj < 2 ? 2 : 3;
@@ -107,19 +111,32 @@ __device__ __noinline__ Cell ghost_cell(const Params &P, const StateP &S, const 
   return bc_apply(kind, side, s, P, ext);
 }
 
-__device__ __forceinline__ Cell load_cell(const Params &P, const StateP &S, const double *b, const double *ext,
-                                          int gi, int gj) {
-  if (gi >= 2 && gi < P.W - 2 && gj >= 2 && gj < P.H - 2) return ld(S, b, gj * P.W + gi);
-  return ghost_cell(P, S, b, ext, gi, gj);
-}
-
 struct Row { double h, b, eta, U, V; };          // what the slope stencil needs of a cell
 struct Face4 { double eta, b, u, v; };           // reconstructed state on one side of a face
 struct FluxF { double M, NL, NR, T; };           // face flux: mass, normal+pL, normal+pR, transverse
 struct Part3 { double h, hu, hv; };
 
+#ifndef SWE_USE_DPP
+#define SWE_USE_DPP 0
+#endif
+#if SWE_USE_DPP
+// Whole-wavefront shift by one lane through DPP (wave_shl:1 = 0x130 brings lane
+// i+1 to lane i, wave_shr:1 = 0x138 brings lane i-1): two v_mov_b32_dpp per
+// double, no LDS round trip.  The end lane keeps its own value.
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const int lo = (int)(unsigned)u, hi = (int)(unsigned)(u >> 32);
+  const int rlo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
+  const int rhi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)rhi << 32) | (unsigned)rlo);
+}
+__device__ __forceinline__ double shdn(double v) { return dpp_shift<0x130>(v); }
+__device__ __forceinline__ double shup(double v) { return dpp_shift<0x138>(v); }
+#else
 __device__ __forceinline__ double shdn(double v) { return __shfl_down(v, 1, SWEEP_LANES); }
 __device__ __forceinline__ double shup(double v) { return __shfl_up(v, 1, SWEEP_LANES); }
+#endif
 
 // Interface flux + hydrostatic pressure corrections (swe.mjs:413-445 body, and
 // :448-475 with normal/tangential swapped by the caller).
@@ -163,7 +180,6 @@ __global__ __launch_bounds__(SWEEP_LANES) void k_sweep(SweepArgs A) {
     return;
   }
   const bool posOn = dt > 0;
-  const double *ext = ctl->ext;
   const double *b = A.b;
   const HllcConst HC = {P.g, P.half_g, P.rg, P.min_depth};
   const double md = P.min_depth;
@@ -174,7 +190,7 @@ __global__ __launch_bounds__(SWEEP_LANES) void k_sweep(SweepArgs A) {
   auto load_row = [&](int r) -> Row {
     Row c;
     if (r >= 0 && r < H && colIn) {
-      const Cell x = load_cell(P, A.in, b, ext, gi, r);
+      const Cell x = ld(A.in, b, r * W + gi);
       c.h = x.h; c.b = x.b; c.eta = x.b + x.h;
       const Vel2 uv = vel2(x.hu, x.hv, x.h, P.eps4);
       c.U = uv.u; c.V = uv.v;
@@ -302,7 +318,7 @@ __global__ __launch_bounds__(SWEEP_LANES) void k_sweep(SweepArgs A) {
       if (fySouth) { rh += ysPrev.h; rhu += ysPrev.hu; rhv += ysPrev.hv; }
       if (fyPrevOn) { rh -= yn.h; rhu -= yn.hu; rhv -= yn.hv; }
       const bool interior = xin && o >= 2 && o <= H - 3;
-      const Cell cur = load_cell(P, A.in, b, ext, gi, o);
+      const Cell cur = ld(A.in, b, k);
       if (interior) {
         if (bedOn) {
           rhu += bedPrev2.hu;
@@ -328,10 +344,10 @@ __global__ __launch_bounds__(SWEEP_LANES) void k_sweep(SweepArgs A) {
           A.out.h[k] = nh; A.out.hu[k] = nhu; A.out.hv[k] = nhv;
         }
       } else {                                            // stage 2, swe.mjs:799-812
-        const Cell c0a = load_cell(P, A.h0, b, ext, gi, o);
-        double nh = 0.5 * (c0a.h + cur.h + dt * rh);
-        double nhu = 0.5 * (c0a.hu + cur.hu + dt * rhu);
-        double nhv = 0.5 * (c0a.hv + cur.hv + dt * rhv);
+        const double h0 = A.h0.h[k], hu0 = A.h0.hu[k], hv0 = A.h0.hv[k];
+        double nh = 0.5 * (h0 + cur.h + dt * rh);
+        double nhu = 0.5 * (hu0 + cur.hu + dt * rhu);
+        double nhv = 0.5 * (hv0 + cur.hv + dt * rhv);
         if (interior) {
           if (nh < 0) { mf -= nh; nh = 0.0; }
           if (nh <= md) { nhu = 0.0; nhv = 0.0; }
@@ -373,8 +389,6 @@ __global__ __launch_bounds__(SWEEP_LANES) void k_sweep(SweepArgs A) {
               if (bits > cflb) cflb = bits;
             }
           }
-        } else {
-          A.b[k] = cur.b;                                 // ghost bed, as applyBC writes it
         }
         if (!(__builtin_isfinite(nh) && __builtin_isfinite(nhu) && __builtin_isfinite(nhv))) bad = true;
         A.out.h[k] = nh; A.out.hu[k] = nhu; A.out.hv[k] = nhv;
@@ -449,6 +463,59 @@ __global__ void k_maxdt(Params P, StateP S, Ctl *ctl) {
   if ((threadIdx.x & 63) == 0 && best) atomicMax(&ctl->cfl_bits, best);
 }
 
+// applyFriction(dt) on the stored state (swe.mjs:659-704), interior cells.
+__global__ void k_friction(Params P, StateW S, const double *mfield, double dt) {
+  const int n = P.nx * P.ny;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    const int k = (2 + t / P.nx) * P.W + 2 + t % P.nx;
+    const double hk = S.h[k];
+    if (hk <= P.min_depth) { S.hu[k] = 0.0; S.hv[k] = 0.0; continue; }
+    const double nk = P.friction == 2 ? mfield[k] : P.manning;
+    if (nk == 0.0 || nk != nk) continue;
+    double hu = S.hu[k], hv = S.hv[k];
+    const Vel2 uv = vel2(hu, hv, hk, P.eps4);
+    const double speed = js_hypot(uv.u, uv.v);
+    if (speed < 1e-14) continue;
+    const double denom = 1 + dt * P.g * nk * nk * speed / js_pow_pos(hk, 4.0 / 3.0);
+    S.hu[k] = hu / denom;
+    S.hv[k] = hv / denom;
+  }
+}
+
+// dryClean() on the stored state (swe.mjs:734-745); the floored mass of each
+// block goes to part[block] (summed in block order by the host).
+__global__ void k_dry_clean(Params P, StateW S, double *part) {
+  const int n = P.nx * P.ny;
+  double added = 0.0;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    const int k = (2 + t / P.nx) * P.W + 2 + t % P.nx;
+    double h = S.h[k];
+    if (h < 0) { added -= h; h = 0.0; S.h[k] = 0.0; }
+    if (h <= P.min_depth) { S.hu[k] = 0.0; S.hv[k] = 0.0; }
+  }
+  for (int off = 32; off >= 1; off >>= 1) added += __shfl_down(added, off, 64);
+  __shared__ double w[16];
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = added;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += w[i];
+    part[blockIdx.x] = s;
+  }
+}
+
+hipError_t launch_friction(const Params &P, StateW S, const double *mfield, double dt, hipStream_t st) {
+  int blocks = (P.nx * P.ny + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(k_friction, dim3(blocks), dim3(256), 0, st, P, S, mfield, dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_dry_clean(const Params &P, StateW S, double *part, int blocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_dry_clean, dim3(blocks), dim3(256), 0, st, P, S, part);
+  return hipGetLastError();
+}
+
 __device__ __forceinline__ double harm_eval(int n, double c0, const double *amp, const double *om,
                                             const double *ph, double t) {
   double e = c0;
@@ -505,6 +572,8 @@ hipError_t launch_sweep(int mode, const SweepArgs &a, hipStream_t st) {
   else hipLaunchKernelGGL(k_sweep<MODE_PROBE>, grid, block, 0, st, a);
   return hipGetLastError();
 }
+
+const void *k_sweep_stage1_ptr() { return reinterpret_cast<const void *>(&k_sweep<MODE_STAGE1>); }
 
 int sweep_waves(const Params &P, int rows_per_chunk) {
   return ((P.W + SWEEP_OUT - 1) / SWEEP_OUT) * ((P.H + rows_per_chunk - 1) / rows_per_chunk);

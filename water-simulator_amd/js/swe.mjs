@@ -1,0 +1,419 @@
+// Drop-in for src/swe.mjs of Syntaxswine/water-simulator whose time step runs on
+// an MI355X through the N-API addon (../napi/swe_napi.node) over the C ABI of
+// include/swe_amd.h.  Same exports, same constructor options, same public fields
+// and methods, same numbers: the GPU kernels reproduce the reference's step bit
+// for bit (Manning friction to the last ulp of Math.pow).
+//
+// Coherence: callers read and write sim.h, sim.hu, sim.hv and sim.b between
+// calls, so every device call uploads the host arrays it needs first and
+// downloads what it changed afterwards (strict mode).  advance(n) runs n steps
+// device-resident for throughput work and downloads once at the end.
+//
+// Boundaries are recognised by identity (reflect, outflow, periodic exported
+// here) or by the tag a flather() closure carries; the flather closure's
+// etaExt(t)/uExt(t) are evaluated here on the host once per step at sim.t and
+// handed to the device, exactly the values the reference's per-ghost-cell calls
+// produce (t does not move inside a step).  A makeSponge() forcing is recognised
+// the same way and runs inside the step on the device; any other forcing
+// callback runs on the host after the step, where the reference calls it.
+
+import { createRequire } from 'module';
+import { cartesianGeometry, sphericalGeometry } from './geometry.mjs';
+
+const require = createRequire(import.meta.url);
+const addon = require('../napi/swe_napi.node');
+
+export const G = 9.80665;          // standard gravity [m/s^2]
+
+const F = { B: 0, H: 1, HU: 2, HV: 3, MANNING: 4, RH: 5, RHU: 6, RHV: 7, THETA: 8, SPONGE: 9 };
+const S = { MANNING: 0, CORIOLIS: 1, CFL: 2, T: 3, SPONGE_REF: 4, STEPS: 5 };
+const ROWS = ['dxRow', 'dyRowN', 'dyRowS', 'dyCFL', 'bedPhi', 'tanPhi', 'fRow', 'geoCoef', 'area'];
+const LIMITERS = { minmod: 0, mc: 1, central: 2 };
+const BC = { REFLECT: 0, OUTFLOW: 1, PERIODIC: 2, FLATHER: 3 };
+const SIDES = ['west', 'east', 'south', 'north'];
+const FLATHER_TAG = Symbol('swe.flather');
+const SPONGE_TAG = Symbol('swe.sponge');
+
+export class ShallowWater {
+  constructor({
+    nx, ny, dx, dy = dx, bed, eta0 = 0, manning = 0.025,
+    minDepth = 1e-3, cfl = 0.45, coriolis = 0, order = 2, limiter = 'mc',
+    sphere = null,
+  }) {
+    this.nx = nx; this.ny = ny; this.dx = dx; this.dy = dy;
+    this.geom = sphere
+      ? sphericalGeometry({ nx, ny, ng: 2, ...sphere })
+      : cartesianGeometry({ nx, ny, dx, dy, ng: 2, coriolis });
+    if (sphere) { this.dx = undefined; this.dy = this.geom.dy; }
+    this.manning = manning; this.minDepth = minDepth; this.cfl = cfl;
+    this.eps4 = Math.max(minDepth, 1e-12) ** 4;
+    this.coriolis = coriolis; this.order = order;
+    this.manningField = null;
+    this.limiterName = limiter;
+    if (!(limiter in LIMITERS)) throw new Error(`unknown limiter ${limiter}; have ${Object.keys(LIMITERS)}`);
+    this.ng = 2;
+    this.W = nx + 2 * this.ng;
+    this.H = ny + 2 * this.ng;
+    const N = this.W * this.H;
+    this.b = new Float64Array(N);
+    this.h = new Float64Array(N);
+    this.hu = new Float64Array(N);
+    this.hv = new Float64Array(N);
+    this.rh = new Float64Array(N); this.rhu = new Float64Array(N); this.rhv = new Float64Array(N);
+    this.theta = new Float64Array(N).fill(1);
+    for (let j = 0; j < this.H; j++) {
+      for (let i = 0; i < this.W; i++) {
+        const k = j * this.W + i;
+        const [x, y] = this.geom.kind === 'sphere'
+          ? this.cellLonLat(i - this.ng, j - this.ng)
+          : this.cellCentre(i - this.ng, j - this.ng);
+        this.b[k] = bed(x, y);
+        const e = typeof eta0 === 'function' ? eta0(x, y) : eta0;
+        this.h[k] = Math.max(0, e - this.b[k]);
+      }
+    }
+    this.t = 0;
+    this.steps = 0;
+    this.massFloored = 0;
+    this.boundaries = this.geom.kind === 'sphere'
+      ? { west: periodic, east: periodic, south: reflect, north: reflect }
+      : { west: reflect, east: reflect, south: reflect, north: reflect };
+    this.forcing = null;
+    this.stats = {};
+
+    this._dev = addon.create({
+      nx, ny, kind: sphere ? 1 : 0, order, limiter: LIMITERS[limiter], g: G, minDepth,
+      eps4: this.eps4, cfl, coriolis: sphere ? 0 : coriolis, manning,
+      R: this.geom.R, rows: ROWS.map((r) => this.geom[r]),
+    });
+    this._rowsSent = ROWS.map((r) => Float64Array.from(this.geom[r]));
+    this._spongeSent = null;
+    this._mfieldSent = null;
+  }
+
+  cellCentre(i, j) {
+    if (this.geom.kind === 'sphere') {
+      throw new Error('cellCentre is Cartesian-only; a spherical cell has no single '
+        + 'x in metres. Use cellLonLat(i, j).');
+    }
+    return [(i + 0.5) * this.dx, (j + 0.5) * this.dy];
+  }
+
+  cellLonLat(i, j) {
+    const g = this.geom;
+    if (g.kind !== 'sphere') throw new Error('cellLonLat is spherical-only');
+    const lon = (i + 0.5) * g.dlam * 180 / Math.PI;
+    return [((lon % 360) + 360) % 360 - 180, g.phiC[j + this.ng] * 180 / Math.PI];
+  }
+
+  idx(i, j) { return (j + this.ng) * this.W + (i + this.ng); }
+  eta(i, j) { const k = this.idx(i, j); return this.b[k] + this.h[k]; }
+  depth(i, j) { return this.h[this.idx(i, j)]; }
+
+  /** Desingularised velocity, the same expression the device uses. */
+  vel(hq, h) {
+    const h4 = h * h * h * h;
+    return (Math.SQRT2 * h * hq) / Math.sqrt(h4 + Math.max(h4, this.eps4));
+  }
+
+  // ---- host <-> device coherence ------------------------------------------
+  _syncParams() {
+    const d = this._dev;
+    addon.setScalar(d, S.MANNING, this.manning);
+    addon.setScalar(d, S.CORIOLIS, this.geom.kind === 'sphere' ? 0 : this.coriolis);
+    addon.setScalar(d, S.CFL, this.cfl);
+    for (let r = 0; r < ROWS.length; r++) {
+      const now = this.geom[ROWS[r]], sent = this._rowsSent[r];
+      let same = now.length === sent.length;
+      for (let j = 0; same && j < now.length; j++) same = Object.is(now[j], sent[j]);
+      if (!same) { addon.setRow(d, r, now); this._rowsSent[r] = Float64Array.from(now); }
+    }
+    if (this.manningField !== this._mfieldSent) {
+      addon.setField(d, F.MANNING, this.manningField);
+      this._mfieldSent = this.manningField;
+    } else if (this.manningField) {
+      addon.setField(d, F.MANNING, this.manningField);
+    }
+  }
+
+  _upload(fields) { for (const f of fields) addon.setField(this._dev, F[f], this[f.toLowerCase()]); }
+  _download(fields) {
+    const map = { B: 'b', H: 'h', HU: 'hu', HV: 'hv', RH: 'rh', RHU: 'rhu', RHV: 'rhv', THETA: 'theta' };
+    for (const f of fields) addon.getField(this._dev, F[f], this[map[f]]);
+  }
+
+  // Boundary kinds and the Flather values at the current t; throws for a
+  // boundary function the device does not implement.
+  _boundaries() {
+    const ext = new Float64Array(8);
+    for (let s = 0; s < 4; s++) {
+      const fn = this.boundaries[SIDES[s]];
+      let kind;
+      if (fn === reflect) kind = BC.REFLECT;
+      else if (fn === outflow) kind = BC.OUTFLOW;
+      else if (fn === periodic) kind = BC.PERIODIC;
+      else if (fn && fn[FLATHER_TAG]) {
+        kind = BC.FLATHER;
+        ext[2 * s] = fn[FLATHER_TAG].etaExt(this.t);
+        ext[2 * s + 1] = fn[FLATHER_TAG].uExt(this.t);
+      } else {
+        throw new Error(`boundary on ${SIDES[s]} is not one of reflect/outflow/periodic/flather; `
+          + 'the GPU backend runs only those');
+      }
+      addon.setBc(this._dev, s, kind);
+    }
+    return ext;
+  }
+
+  applyBC() {
+    const ext = this._boundaries();
+    this._upload(['B', 'H', 'HU', 'HV']);
+    addon.applyBc(this._dev, ext);
+    this._download(['B', 'H', 'HU', 'HV']);
+  }
+
+  residual(dt) {
+    this._syncParams();
+    this._upload(['B', 'H', 'HU', 'HV']);
+    addon.residual(this._dev, dt);
+    this._download(['RH', 'RHU', 'RHV', 'THETA']);
+  }
+
+  maxDt() {
+    this._syncParams();
+    this._upload(['H', 'HU', 'HV']);
+    return addon.maxDt(this._dev);
+  }
+
+  applyFriction(dt) {
+    this._syncParams();
+    this._upload(['H', 'HU', 'HV']);
+    addon.applyFriction(this._dev, dt);
+    this._download(['HU', 'HV']);
+  }
+
+  setManningField(fn) {
+    if (!fn) { this.manningField = null; return this; }
+    const N = this.W * this.H, out = new Float64Array(N);
+    const sph = this.geom.kind === 'sphere';
+    for (let j = 0; j < this.H; j++) {
+      for (let i = 0; i < this.W; i++) {
+        const [x, y] = sph
+          ? this.cellLonLat(i - this.ng, j - this.ng)
+          : this.cellCentre(i - this.ng, j - this.ng);
+        const v = fn(x, y);
+        if (!(v >= 0) || !isFinite(v)) {
+          throw new Error(`setManningField: n = ${v} at (${x}, ${y}); Manning n must be finite and >= 0`);
+        }
+        out[j * this.W + i] = v;
+      }
+    }
+    this.manningField = out;
+    return this;
+  }
+
+  dryClean() {
+    this._upload(['H', 'HU', 'HV']);
+    const added = addon.dryClean(this._dev);
+    this._download(['H', 'HU', 'HV']);
+    return added;
+  }
+
+  /** Advance by dt (or by the CFL step if dt is omitted). SSP-RK2 on the GPU. */
+  step(dt = null) {
+    const ext = this._boundaries();
+    this._syncParams();
+    const sp = this.forcing && this.forcing[SPONGE_TAG];
+    if (sp) {
+      if (this._spongeSent !== sp.strength) { addon.setField(this._dev, F.SPONGE, sp.strength); this._spongeSent = sp.strength; }
+      addon.setScalar(this._dev, S.SPONGE_REF, typeof sp.etaRef === 'function' ? sp.etaRef(this.t) : sp.etaRef);
+    } else if (this._spongeSent) {
+      addon.setField(this._dev, F.SPONGE, null);
+      this._spongeSent = null;
+    }
+    addon.setScalar(this._dev, S.T, this.t);
+    this._upload(['B', 'H', 'HU', 'HV']);
+    const [taken, mf] = addon.step(this._dev, dt == null ? NaN : dt, ext, true, false);
+    this._download(['B', 'H', 'HU', 'HV', 'THETA']);
+    if (!(taken > 0)) return 0;
+    this.massFloored = mf;
+    if (this.forcing && !sp) this.forcing(this, taken);
+    this.t += taken;
+    this.steps++;
+    return taken;
+  }
+
+  /** n self-timed steps device-resident (no per-step host round trip), then one download. */
+  advance(n) {
+    this._boundaries();
+    this._syncParams();
+    if (this.forcing) throw new Error('advance(n) runs without a host forcing hook');
+    addon.setScalar(this._dev, S.T, this.t);
+    this._upload(['B', 'H', 'HU', 'HV']);
+    const taken = addon.advance(this._dev, n, 0, 0, Infinity);
+    this._download(['B', 'H', 'HU', 'HV']);
+    this.t = addon.getScalar(this._dev, S.T);
+    this.steps += taken;
+    return taken;
+  }
+
+  // ---- diagnostics (host, over the coherent mirror) ---------------------------
+  volume() {
+    const { W, ng, nx, ny } = this;
+    if (this.geom.kind !== 'sphere') {
+      let v = 0;
+      for (let j = 0; j < ny; j++) for (let i = 0; i < nx; i++) v += this.h[(j + ng) * W + (i + ng)];
+      return v * this.dx * this.dy;
+    }
+    const area = this.geom.area;
+    let v = 0;
+    for (let j = 0; j < ny; j++) {
+      let row = 0;
+      for (let i = 0; i < nx; i++) row += this.h[(j + ng) * W + (i + ng)];
+      v += row * area[j + ng];
+    }
+    return v;
+  }
+
+  energy() {
+    const { W, ng, nx, ny } = this;
+    const sph = this.geom.kind === 'sphere', area = this.geom.area;
+    let e = 0;
+    for (let j = 0; j < ny; j++) {
+      let row = 0;
+      for (let i = 0; i < nx; i++) {
+        const k = (j + ng) * W + (i + ng), hk = this.h[k];
+        if (hk <= 0) continue;
+        const u = this.vel(this.hu[k], hk), v = this.vel(this.hv[k], hk);
+        row += 0.5 * hk * (u * u + v * v) + 0.5 * G * hk * (hk + 2 * this.b[k]);
+      }
+      if (sph) e += row * area[j + ng]; else e += row;
+    }
+    return sph ? e : e * this.dx * this.dy;
+  }
+
+  maxSurfaceDeviation(ref) {
+    const { W, ng, nx, ny } = this;
+    let m = 0;
+    for (let j = 0; j < ny; j++) {
+      for (let i = 0; i < nx; i++) {
+        const k = (j + ng) * W + (i + ng);
+        if (this.h[k] <= this.minDepth) continue;
+        const d = Math.abs(this.b[k] + this.h[k] - ref);
+        if (d > m) m = d;
+      }
+    }
+    return m;
+  }
+
+  maxSpeed() {
+    const { W, ng, nx, ny } = this;
+    let m = 0;
+    for (let j = 0; j < ny; j++) {
+      for (let i = 0; i < nx; i++) {
+        const k = (j + ng) * W + (i + ng), hk = this.h[k];
+        if (hk <= this.minDepth) continue;
+        const s = Math.hypot(this.vel(this.hu[k], hk), this.vel(this.hv[k], hk));
+        if (s > m) m = s;
+      }
+    }
+    return m;
+  }
+
+  finite() {
+    for (let k = 0; k < this.h.length; k++) {
+      if (!Number.isFinite(this.h[k]) || !Number.isFinite(this.hu[k]) || !Number.isFinite(this.hv[k])) return false;
+    }
+    return true;
+  }
+}
+
+/** HLLC interface flux with Einfeldt speeds and dry states: the kernels' own code. */
+export function hllc(out, hL, huL, hvL, hR, huR, hvR, dry = 1e-4) {
+  const f = addon.hllc(hL, huL, hvL, hR, huR, hvR, dry, G);
+  out[0] = f[0]; out[1] = f[1]; out[2] = f[2];
+  return out;
+}
+
+// ---- boundary conditions: host bodies for direct calls, device kinds by identity
+function clampIdx(v, hi) { return v < 0 ? 0 : v > hi ? hi : v; }
+
+export function reflect(sim, i, j, side) {
+  let si = i, sj = j;
+  if (side === 0) si = -1 - i; else if (side === 1) si = 2 * sim.nx - 1 - i;
+  else if (side === 2) sj = -1 - j; else sj = 2 * sim.ny - 1 - j;
+  si = clampIdx(si, sim.nx - 1); sj = clampIdx(sj, sim.ny - 1);
+  const g = sim.idx(i, j), s = sim.idx(si, sj);
+  sim.b[g] = sim.b[s]; sim.h[g] = sim.h[s];
+  const flipX = side === 0 || side === 1;
+  sim.hu[g] = flipX ? -sim.hu[s] : sim.hu[s];
+  sim.hv[g] = flipX ? sim.hv[s] : -sim.hv[s];
+}
+
+export function outflow(sim, i, j) {
+  const g = sim.idx(i, j), s = sim.idx(clampIdx(i, sim.nx - 1), clampIdx(j, sim.ny - 1));
+  sim.b[g] = sim.b[s]; sim.h[g] = sim.h[s]; sim.hu[g] = sim.hu[s]; sim.hv[g] = sim.hv[s];
+}
+
+export function periodic(sim, i, j, side) {
+  let si = i, sj = j;
+  if (side === 0) si = i + sim.nx; else if (side === 1) si = i - sim.nx;
+  else if (side === 2) sj = j + sim.ny; else sj = j - sim.ny;
+  const g = sim.idx(i, j), s = sim.idx(si, sj);
+  sim.b[g] = sim.b[s]; sim.h[g] = sim.h[s]; sim.hu[g] = sim.hu[s]; sim.hv[g] = sim.hv[s];
+}
+
+/** Flather radiation boundary; the device evaluates it from the tagged etaExt/uExt. */
+export function flather(etaExt, uExt = () => 0) {
+  const fn = (sim, i, j, side) => {
+    const g = sim.idx(i, j), s = sim.idx(clampIdx(i, sim.nx - 1), clampIdx(j, sim.ny - 1));
+    sim.b[g] = sim.b[s];
+    const bed = sim.b[g], hInt = sim.h[s];
+    const e0 = etaExt(sim.t), u0 = uExt(sim.t);
+    if (hInt <= sim.minDepth) { sim.h[g] = Math.max(0, e0 - bed); sim.hu[g] = 0; sim.hv[g] = 0; return; }
+    const etaInt = bed + hInt;
+    const r = Math.sqrt(G / hInt);
+    const sgn = (side === 0 || side === 2) ? 1 : -1;
+    const uInt = (side === 0 || side === 1) ? sim.vel(sim.hu[s], hInt) : sim.vel(sim.hv[s], hInt);
+    const Rin = u0 + sgn * e0 * r;
+    const Rout = uInt - sgn * etaInt * r;
+    const un = 0.5 * (Rin + Rout);
+    const en = sgn * (Rin - Rout) / (2 * r);
+    sim.h[g] = Math.max(0, en - bed);
+    if (side === 0 || side === 1) { sim.hu[g] = sim.h[g] * un; sim.hv[g] = sim.hv[s]; }
+    else { sim.hv[g] = sim.h[g] * un; sim.hu[g] = sim.hu[s]; }
+  };
+  fn[FLATHER_TAG] = { etaExt, uExt };
+  return fn;
+}
+
+/** Sponge layer forcing: relax toward max(0, etaRef - b) over a cos^2-tapered band. */
+export function makeSponge(sim, { side = 'east', width = 20, strength = 0.06, etaRef = 0 }) {
+  const { nx, ny, W, ng } = sim;
+  const str = new Float64Array(W * sim.H);
+  const cells = [];
+  for (let j = 0; j < ny; j++) {
+    for (let i = 0; i < nx; i++) {
+      let d;
+      if (side === 'east') d = nx - 1 - i; else if (side === 'west') d = i;
+      else if (side === 'north') d = ny - 1 - j; else d = j;
+      if (d >= width) continue;
+      const s = strength * Math.cos(0.5 * Math.PI * d / width) ** 2;
+      const k = (j + ng) * W + (i + ng);
+      str[k] = s;
+      cells.push([k, s]);
+    }
+  }
+  const fn = (s2, dt) => {
+    const ref = typeof etaRef === 'function' ? etaRef(s2.t) : etaRef;
+    for (const [k, s] of cells) {
+      const f = Math.min(1, s * dt * 60);
+      const target = Math.max(0, ref - s2.b[k]);
+      s2.h[k] += (target - s2.h[k]) * f;
+      s2.hu[k] -= s2.hu[k] * f;
+      s2.hv[k] -= s2.hv[k] * f;
+    }
+  };
+  fn[SPONGE_TAG] = { strength: str, etaRef };
+  return fn;
+}

@@ -17,14 +17,15 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libswe_amd.so")
+LIB_PATH = os.environ.get("SWE_AMD_LIB") or os.path.join(HERE, "libswe_amd.so")
 
 NROWS = 9
 ROWS = ["dxRow", "dyRowN", "dyRowS", "dyCFL", "bedPhi", "tanPhi", "fRow", "geoCoef", "area"]
 F_B, F_H, F_HU, F_HV, F_MANNING, F_RH, F_RHU, F_RHV, F_THETA, F_SPONGE = range(10)
 BC = {"reflect": 0, "outflow": 1, "periodic": 2, "flather": 3, "stored": 4}
 LIMITER = {"minmod": 0, "mc": 1, "central": 2}
-S_MANNING, S_CORIOLIS, S_CFL, S_T, S_SPONGE_REF, S_STEPS, S_MASS_FLOORED, S_DT_NEXT, S_NONFINITE = range(9)
+(S_MANNING, S_CORIOLIS, S_CFL, S_T, S_SPONGE_REF, S_STEPS, S_MASS_FLOORED, S_DT_NEXT, S_NONFINITE,
+ S_CHUNK_ROWS) = range(10)
 DT_SELF, DT_FIXED, DT_UNTIL = 0, 1, 2
 
 
@@ -76,6 +77,7 @@ def lib():
             "swe_device_ptr": (i, [vp, i, ctypes.POINTER(dp)]),
             "swe_last_advance_ms": (i, [vp, dp]),
             "swe_profile_step": (i, [vp, i, dp]),
+            "swe_autotune": (i, [vp, i, ctypes.POINTER(i)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -267,6 +269,15 @@ class Solver:
         out = np.zeros(3)
         self._ck(lib().swe_profile_step(self._h, int(nsteps), _dp(out)), "swe_profile_step")
         return out
+
+    def chunk_rows(self):
+        return int(self.get_scalar(S_CHUNK_ROWS))
+
+    def autotune(self, reps=3):
+        """Pick the fastest row-chunk height on the device; returns it."""
+        best = ctypes.c_int()
+        self._ck(lib().swe_autotune(self._h, int(reps), ctypes.byref(best)), "swe_autotune")
+        return best.value
 
     def use_graph(self, on=True):
         self._ck(lib().swe_use_graph(self._h, int(on)), "swe_use_graph")

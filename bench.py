@@ -69,6 +69,49 @@ def p_dx():
     return 10.0
 
 
+def batched_measure(steps, warmup, seq_steps):
+    """Config 3: 512 uneven basins in one batch vs the same basins as sequential
+    plain runs on the same GPU (device time of the stepping only)."""
+    import swe_amd
+    import workloads
+    specs = workloads.ensemble(nb=512)
+    cells = sum(s["nx"] * s["ny"] for s in specs)
+    B = swe_amd.Batch([dict(nx=s["nx"], ny=s["ny"], manning=s["manning"], cfl=s["cfl"],
+                            rows=swe_amd.geometry_cartesian(s["nx"], s["ny"], s["dx"])) for s in specs])
+    for i, s in enumerate(specs):
+        B.load_state(i, *s["state"])
+        B.set_bc(i, s["bc"])
+        B.set_harmonic(i, 0, amp=[s["amp"]], omega=[s["omega"]], phase=[0.0],
+                       uamp=[s["uamp"]], uomega=[s["omega"]], uphase=[0.0])
+    rows, waves = B.autotune(steps=2)
+    log(f"[bench] batch autotune: {rows} rows per task, {waves} persistent wavefronts")
+    B.advance(warmup)
+    t0 = time.perf_counter()
+    n = B.advance(steps)
+    el = time.perf_counter() - t0
+    batched = cells * steps / el
+    log(f"[bench] batched: {batched:.4e} cell-updates/s over {len(specs)} basins ({cells / 1e6:.1f} M cells)")
+    B.close()
+    seq_ms = 0.0
+    for s in specs:
+        g = swe_amd.Solver(s["nx"], s["ny"], swe_amd.geometry_cartesian(s["nx"], s["ny"], s["dx"]),
+                           manning=s["manning"], cfl=s["cfl"])
+        g.load_state(*s["state"])
+        g.set_bc(s["bc"])
+        g.set_harmonic(0, amp=[s["amp"]], omega=[s["omega"]], phase=[0.0],
+                       uamp=[s["uamp"]], uomega=[s["omega"]], uphase=[0.0])
+        g.use_graph(True)
+        g.advance(2)
+        g.advance(seq_steps)
+        seq_ms += g.last_advance_ms()
+        g.close()
+    seq = cells * seq_steps / (seq_ms * 1e-3)
+    log(f"[bench] sequential plain: {seq:.4e} cell-updates/s")
+    return {"value": batched, "unit": "cell-updates/s", "basins": len(specs), "cells": cells, "steps": steps,
+            "task_rows": rows, "persistent_wavefronts": waves, "sequential_plain": seq,
+            "batched_vs_plain": batched / seq}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +119,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--batched-steps", type=int, default=100, help="0 skips the batched-ensemble measurement")
     ap.add_argument("--chunk", type=int, default=-1, help="rows per wavefront chunk; -1 autotune, 0 default")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -131,6 +175,9 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "finite": finite,
     }
+    sim.close()
+    if args.batched_steps > 0:
+        out["batched"] = batched_measure(args.batched_steps, 5, 20)
     if not args.no_cpu:
         log("[bench] cpu baseline")
         out["cpu_baseline"] = cpu_baseline()

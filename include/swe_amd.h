@@ -145,6 +145,31 @@ int swe_device_ptr(swe_ctx *ctx, int field, double **dptr);
 /* elapsed ms of the last swe_advance measured with HIP events on the context's stream */
 int swe_last_advance_ms(swe_ctx *ctx, double *ms);
 
+/* ---- batched ensembles (no reference counterpart: the reference runs one
+ * ShallowWater per JS object; these advance many uneven basins in one launch
+ * per kernel phase, each bit-identical to its own plain run) ------------------ */
+typedef struct swe_batch swe_batch;
+int swe_batch_create(const swe_desc *descs, int nbasins, swe_batch **out);
+void swe_batch_destroy(swe_batch *b);
+const char *swe_batch_last_error(const swe_batch *b);
+int swe_batch_set_field(swe_batch *b, int basin, int field, const double *host);
+int swe_batch_get_field(swe_batch *b, int basin, int field, double *host);
+int swe_batch_set_bc(swe_batch *b, int basin, int side, int kind);
+int swe_batch_set_harmonic(swe_batch *b, int basin, int side, int n_eta, double msl, const double *amp,
+                           const double *omega, const double *phase, int n_u, double u0,
+                           const double *uamp, const double *uomega, const double *uphase);
+int swe_batch_get_scalar(swe_batch *b, int basin, int which, double *value);
+/* rows per task and persistent-grid wavefronts (nwaves <= 0 keeps the current) */
+int swe_batch_set_chunk_rows(swe_batch *b, int rows, int nwaves);
+int swe_batch_use_graph(swe_batch *b, int on);
+/* nsteps self-timed steps of every basin (each its own dt); *min_steps = fewest
+ * steps any basin has taken in total */
+int swe_batch_advance(swe_batch *b, int64_t nsteps, int64_t *min_steps);
+int swe_batch_last_advance_ms(swe_batch *b, double *ms);
+/* autotuner: times `steps` steps for each (rows, wavefronts) candidate on a copy
+ * of the current state, restores the state, keeps the fastest */
+int swe_batch_autotune(swe_batch *b, int steps, int *best_rows, int *best_waves);
+
 #ifdef __cplusplus
 }
 #endif

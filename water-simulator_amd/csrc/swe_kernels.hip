@@ -117,7 +117,7 @@ struct FluxF { double M, NL, NR, T; };           // face flux: mass, normal+pL, 
 struct Part3 { double h, hu, hv; };
 
 #ifndef SWE_USE_DPP
-#define SWE_USE_DPP 0
+#define SWE_USE_DPP 1
 #endif
 #if SWE_USE_DPP
 // Whole-wavefront shift by one lane through DPP (wave_shl:1 = 0x130 brings lane
@@ -154,19 +154,21 @@ __device__ __forceinline__ FluxF face_flux(const Face4 &L, const Face4 &R, const
   return o;
 }
 
+// One wavefront's share of a stage: strip `strip` (58 output columns) and rows
+// [chunk*rows_per_chunk, +rows_per_chunk) of the basin described by A.  wave_id
+// indexes the massFloored partial this wavefront owns.
 template <int MODE>
-__global__ __launch_bounds__(SWEEP_LANES) void k_sweep(SweepArgs A) {
+__device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int chunk, int wave_id) {
   const Params &P = A.P;
   const int W = P.W, H = P.H;
-  const int lane = threadIdx.x;
-  const int gi = blockIdx.x * SWEEP_OUT - SWEEP_HALO + lane;
+  const int lane = threadIdx.x & (SWEEP_LANES - 1);
+  const int gi = strip * SWEEP_OUT - SWEEP_HALO + lane;
   const bool colIn = gi >= 0 && gi < W;
   const bool outLane = colIn && lane >= SWEEP_HALO && lane < SWEEP_LANES - SWEEP_HALO;
-  const int y0 = blockIdx.y * A.rows_per_chunk;
+  const int y0 = chunk * A.rows_per_chunk;
   const int y1 = min(H, y0 + A.rows_per_chunk);
   const Ctl *ctl = A.ctl;
   const double dt = ctl->dt;
-  const int wave_id = blockIdx.y * gridDim.x + blockIdx.x;
   if (MODE != MODE_PROBE && (!(dt > 0) || __builtin_isinf(dt))) {
     // a refused step (swe.mjs:784) leaves the state as it was: stage 2 hands the
     // step-start state through to its output buffer unchanged
@@ -421,28 +423,58 @@ __global__ __launch_bounds__(SWEEP_LANES) void k_sweep(SweepArgs A) {
   }
 }
 
-// applyBC on stored arrays (swe.mjs:289-303): every ghost cell rewritten from the
-// interior.  Used by the explicit applyBC()/residual() probe path.
-__global__ void k_apply_bc(Params P, StateW S, double *b, const Ctl *ctl) {
-  const int nG = 2 * P.W * 2 + 2 * (P.H - 4) * 2;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nG) return;
+template <int MODE>
+__global__ __launch_bounds__(SWEEP_LANES) void k_sweep(SweepArgs A) {
+  sweep_body<MODE>(A, blockIdx.x, blockIdx.y, blockIdx.y * gridDim.x + blockIdx.x);
+}
+
+// Batched ensembles: a persistent grid of wavefronts pulls (basin, strip, chunk)
+// tasks from a queue until it is empty, so basins of very different sizes share
+// one launch and the device stays full to the last task.
+template <int MODE>
+__global__ __launch_bounds__(SWEEP_LANES, 3) void k_sweep_batch(const BasinDev *__restrict__ basins,
+                                                             const Task *__restrict__ tasks, int ntasks,
+                                                             unsigned *qctr, int parity) {
+  for (;;) {
+    unsigned t = 0;
+    if (threadIdx.x == 0) t = atomicAdd(qctr, 1u);
+    // wave-uniform from here on: task and basin fields load through the scalar unit
+    t = (unsigned)__builtin_amdgcn_readfirstlane((int)__shfl(t, 0, SWEEP_LANES));
+    if (t >= (unsigned)ntasks) break;
+    const int basin = __builtin_amdgcn_readfirstlane(tasks[t].basin);
+    const int strip = __builtin_amdgcn_readfirstlane(tasks[t].strip);
+    const int chunk = __builtin_amdgcn_readfirstlane(tasks[t].chunk);
+    const int wave = __builtin_amdgcn_readfirstlane(tasks[t].wave);
+    sweep_body<MODE>(basins[basin].sw[parity][MODE - 1], strip, chunk, wave);
+  }
+}
+
+// applyBC on stored arrays (swe.mjs:289-303): ghost cell t of the ring
+// rewritten from the interior.  Every ghost value depends only on interior cells
+// (a periodic S/N source in a ghost column is recomputed, not read), so the
+// threads' writes race with no read.
+__device__ __forceinline__ void apply_bc_cell(const Params &P, StateW S, double *b, const double *ext, int t) {
   int gi, gj;
   if (t < 4 * P.W) { const int rr = t / P.W; gi = t % P.W; gj = rr < 2 ? rr : P.H - 4 + rr; }
   else { const int u = t - 4 * P.W; const int cc = u / (P.H - 4); gj = 2 + u % (P.H - 4); gi = cc < 2 ? cc : P.W - 4 + cc; }
   const StateP R = {S.h, S.hu, S.hv};
-  // every ghost value depends only on interior cells (a periodic S/N source in a
-  // ghost column is recomputed, not read), so the writes below race with no read
-  const Cell c = ghost_cell(P, R, b, ctl->ext, gi, gj);
+  const Cell c = ghost_cell(P, R, b, ext, gi, gj);
   const int k = gj * P.W + gi;
   b[k] = c.b; S.h[k] = c.h; S.hu[k] = c.hu; S.hv[k] = c.hv;
 }
 
-// maxDt() over the stored state (swe.mjs:632-647); result into ctl->cfl_bits.
-__global__ void k_maxdt(Params P, StateP S, Ctl *ctl) {
+__global__ void k_apply_bc(Params P, StateW S, double *b, const Ctl *ctl) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4 * P.W + 4 * (P.H - 4)) return;
+  apply_bc_cell(P, S, b, ctl->ext, t);
+}
+
+// maxDt() over the stored state (swe.mjs:632-647): the largest wave-speed sum of
+// this thread's cells as a bit pattern (non-negative doubles order like integers).
+__device__ __forceinline__ unsigned long long maxdt_cells(const Params &P, StateP S, int t0, int stride) {
   const int n = P.nx * P.ny;
   unsigned long long best = 0ull;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+  for (int t = t0; t < n; t += stride) {
     const int j = 2 + t / P.nx, i = 2 + t % P.nx;
     const int k = j * P.W + i;
     const double hk = S.h[k];
@@ -460,6 +492,11 @@ __global__ void k_maxdt(Params P, StateP S, Ctl *ctl) {
     const unsigned long long o = __shfl_down(best, off, 64);
     if (o > best) best = o;
   }
+  return best;
+}
+
+__global__ void k_maxdt(Params P, StateP S, Ctl *ctl) {
+  const unsigned long long best = maxdt_cells(P, S, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
   if ((threadIdx.x & 63) == 0 && best) atomicMax(&ctl->cfl_bits, best);
 }
 
@@ -523,11 +560,12 @@ __device__ __forceinline__ double harm_eval(int n, double c0, const double *amp,
   return e;
 }
 
-// Turn the max-reduction into a step size and set up the next step.
+// Turn the max-reduction into a step size and set up the next step (one
+// wavefront per basin).
 //   phase 0: after k_maxdt -- dt for the step about to run
 //   phase 1: after a step -- massFloored, t, steps, dt_next, next dt and forcing
-__global__ void k_finalize(Ctl *ctl, const double *mf_part, int nparts, double cfl, int phase) {
-  const int lane = threadIdx.x;
+__device__ __forceinline__ void finalize_body(Ctl *ctl, const double *mf_part, int nparts, double cfl, int phase) {
+  const int lane = threadIdx.x & 63;
   double s1 = 0.0, s2 = 0.0;
   if (phase == 1) {
     for (int i = lane; i < nparts; i += 64) { s1 += mf_part[i]; s2 += mf_part[nparts + i]; }
@@ -550,7 +588,6 @@ __global__ void k_finalize(Ctl *ctl, const double *mf_part, int nparts, double c
       ctl->dt_last = 0.0;
     }
   }
-  // dt of the next step
   if (ctl->dt_mode == DT_SELF) ctl->dt = dtn;
   else if (ctl->dt_mode == DT_FIXED) ctl->dt = ctl->dt_fixed;
   else if (ctl->dt_mode == DT_UNTIL) ctl->dt = fmin(dtn, ctl->t_end - ctl->t);
@@ -561,6 +598,33 @@ __global__ void k_finalize(Ctl *ctl, const double *mf_part, int nparts, double c
       ctl->ext[2 * side + 1] = harm_eval(h.n_u, h.u0, h.uamp, h.uomega, h.uphase, ctl->t);
     }
   }
+}
+
+__global__ void k_finalize(Ctl *ctl, const double *mf_part, int nparts, double cfl, int phase) {
+  finalize_body(ctl, mf_part, nparts, cfl, phase);
+}
+
+// ---- batched ensembles: one launch per phase for all basins -----------------
+__global__ void k_apply_bc_batch(const BasinDev *__restrict__ basins, int parity, int stage) {
+  const BasinDev &B = basins[blockIdx.y];
+  const Params &P = B.sw[parity][0].P;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4 * P.W + 4 * (P.H - 4)) return;
+  apply_bc_cell(P, B.bc_in[parity][stage], B.sw[parity][0].b, B.ctl->ext, t);
+}
+
+__global__ void k_maxdt_batch(const BasinDev *__restrict__ basins, int parity) {
+  const BasinDev &B = basins[blockIdx.y];
+  const SweepArgs &A = B.sw[parity][0];
+  const unsigned long long best = maxdt_cells(A.P, A.in, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+  if ((threadIdx.x & 63) == 0 && best) atomicMax(&B.ctl->cfl_bits, best);
+}
+
+__global__ void k_finalize_batch(const BasinDev *__restrict__ basins, int nb, unsigned *qctr, int phase) {
+  const BasinDev &B = basins[blockIdx.x];
+  finalize_body(B.ctl, B.sw[0][0].mf_part, B.nparts, B.sw[0][0].P.cfl, phase);
+  // the next step's stage launches start their task queues from zero
+  if (blockIdx.x == 0 && threadIdx.x == 0) { qctr[0] = 0u; qctr[1] = 0u; }
 }
 
 // ---- launchers (host side, C++ linkage inside the library) ------------------
@@ -577,6 +641,34 @@ const void *k_sweep_stage1_ptr() { return reinterpret_cast<const void *>(&k_swee
 
 int sweep_waves(const Params &P, int rows_per_chunk) {
   return ((P.W + SWEEP_OUT - 1) / SWEEP_OUT) * ((P.H + rows_per_chunk - 1) / rows_per_chunk);
+}
+
+hipError_t launch_sweep_batch(int mode, const BasinDev *basins, const Task *tasks, int ntasks, unsigned *qctr,
+                              int parity, int nwaves, hipStream_t st) {
+  if (mode == MODE_STAGE1)
+    hipLaunchKernelGGL(k_sweep_batch<MODE_STAGE1>, dim3(nwaves), dim3(SWEEP_LANES), 0, st, basins, tasks, ntasks, qctr, parity);
+  else
+    hipLaunchKernelGGL(k_sweep_batch<MODE_STAGE2>, dim3(nwaves), dim3(SWEEP_LANES), 0, st, basins, tasks, ntasks, qctr + 1, parity);
+  return hipGetLastError();
+}
+
+const void *k_sweep_batch_ptr() { return reinterpret_cast<const void *>(&k_sweep_batch<MODE_STAGE1>); }
+
+hipError_t launch_apply_bc_batch(const BasinDev *basins, int nb, int max_ghost, int parity, int stage, hipStream_t st) {
+  hipLaunchKernelGGL(k_apply_bc_batch, dim3((max_ghost + 255) / 256, nb), dim3(256), 0, st, basins, parity, stage);
+  return hipGetLastError();
+}
+
+hipError_t launch_maxdt_batch(const BasinDev *basins, int nb, int max_cells, int parity, hipStream_t st) {
+  int bx = (max_cells + 255) / 256;
+  if (bx > 64) bx = 64;
+  hipLaunchKernelGGL(k_maxdt_batch, dim3(bx, nb), dim3(256), 0, st, basins, parity);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize_batch(const BasinDev *basins, int nb, unsigned *qctr, int phase, hipStream_t st) {
+  hipLaunchKernelGGL(k_finalize_batch, dim3(nb), dim3(64), 0, st, basins, nb, qctr, phase);
+  return hipGetLastError();
 }
 
 hipError_t launch_apply_bc(const Params &P, StateW S, double *b, const Ctl *ctl, hipStream_t st) {

@@ -75,6 +75,22 @@ struct SweepArgs {
   int rows_per_chunk;
 };
 
+// One basin of a batched ensemble as the batch kernels see it.  The three state
+// buffers rotate in lockstep for every basin, so each basin carries the stage
+// arguments for both rotation parities: sw[parity][stage-1].  bc_in[parity][s] is
+// the state whose ghost ring is filled before stage s+1.
+struct BasinDev {
+  SweepArgs sw[2][2];
+  StateW bc_in[2][2];
+  Ctl *ctl;
+  int nparts;        // wavefronts (tasks) of one stage of this basin
+  int pad;
+};
+
+// A unit of batched work: one wavefront's (strip, chunk) of one basin; `wave`
+// is its massFloored slot in that basin.
+struct Task { int basin, strip, chunk, wave; };
+
 constexpr int SWEEP_LANES = 64;
 constexpr int SWEEP_HALO = 3;
 constexpr int SWEEP_OUT = SWEEP_LANES - 2 * SWEEP_HALO;   // 58 output columns per wave

@@ -398,3 +398,147 @@ def geometry_spherical(nx, ny, R=R_EARTH, lat0=-90.0, lat1=90.0, omega=0.0, ng=2
         rows["fRow"][j] = 2 * omega * math.sin(pC)
         rows["geoCoef"][j] = (lyN - lyS) / (4 * A)
     return rows
+
+
+class Batch:
+    """Many independent basins advanced together (swe_batch_* of the C ABI).
+
+    basins: list of dicts with keys nx, ny, rows (geometry rows) and the Solver
+    keyword arguments (order, limiter, g, min_depth, eps4, cfl, coriolis, manning,
+    sphere, R).  Each basin keeps its own dt, t, boundaries and forcing."""
+
+    def __init__(self, basins):
+        L = lib()
+        self._bind(L)
+        n = len(basins)
+        arr = (Desc * n)()
+        self._keep = []
+        self.shapes = []
+        for i, bd in enumerate(basins):
+            d = arr[i]
+            d.nx, d.ny = bd["nx"], bd["ny"]
+            d.kind = int(bd.get("sphere", False))
+            d.order = bd.get("order", 2)
+            lim = bd.get("limiter", "mc")
+            d.limiter = LIMITER[lim] if isinstance(lim, str) else lim
+            d.g = bd.get("g", 9.80665)
+            d.min_depth = bd.get("min_depth", 1e-3)
+            d.eps4 = bd.get("eps4") or js_pow4(max(d.min_depth, 1e-12))
+            d.cfl = bd.get("cfl", 0.45)
+            d.coriolis = bd.get("coriolis", 0.0)
+            d.manning = bd.get("manning", 0.025)
+            d.R = bd.get("R", math.inf)
+            rows = {r: np.ascontiguousarray(bd["rows"][r], dtype=np.float64) for r in ROWS}
+            self._keep.append(rows)
+            for k, r in enumerate(ROWS):
+                d.rows[k] = _dp(rows[r])
+            self.shapes.append((bd["nx"] + 4, bd["ny"] + 4))
+        h = ctypes.c_void_p()
+        rc = L.swe_batch_create(arr, n, ctypes.byref(h))
+        if rc != 0:
+            raise SweError(f"swe_batch_create: {L.swe_batch_last_error(None).decode()}")
+        self._h = h
+        self.n = n
+
+    @staticmethod
+    def _bind(L):
+        if getattr(L, "_batch_bound", False):
+            return
+        vp, i, d, dp, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_int64
+        sig = {
+            "swe_batch_create": (i, [ctypes.POINTER(Desc), i, ctypes.POINTER(vp)]),
+            "swe_batch_destroy": (None, [vp]),
+            "swe_batch_last_error": (ctypes.c_char_p, [vp]),
+            "swe_batch_set_field": (i, [vp, i, i, dp]),
+            "swe_batch_get_field": (i, [vp, i, i, dp]),
+            "swe_batch_set_bc": (i, [vp, i, i, i]),
+            "swe_batch_set_harmonic": (i, [vp, i, i, i, d, dp, dp, dp, i, d, dp, dp, dp]),
+            "swe_batch_get_scalar": (i, [vp, i, i, dp]),
+            "swe_batch_set_chunk_rows": (i, [vp, i, i]),
+            "swe_batch_use_graph": (i, [vp, i]),
+            "swe_batch_advance": (i, [vp, i64, ctypes.POINTER(i64)]),
+            "swe_batch_last_advance_ms": (i, [vp, dp]),
+            "swe_batch_autotune": (i, [vp, i, ctypes.POINTER(i), ctypes.POINTER(i)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        L._batch_bound = True
+
+    def _ck(self, rc, what):
+        if rc != 0:
+            raise SweError(f"{what}: {lib().swe_batch_last_error(self._h).decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().swe_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_field(self, basin, f, a):
+        W, H = self.shapes[basin]
+        if a is None:
+            self._ck(lib().swe_batch_set_field(self._h, basin, f, None), "swe_batch_set_field")
+            return
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.size == W * H
+        self._ck(lib().swe_batch_set_field(self._h, basin, f, _dp(a)), "swe_batch_set_field")
+
+    def get_field(self, basin, f):
+        W, H = self.shapes[basin]
+        out = np.empty(W * H)
+        self._ck(lib().swe_batch_get_field(self._h, basin, f, _dp(out)), "swe_batch_get_field")
+        return out
+
+    def load_state(self, basin, b, h, hu, hv):
+        for f, a in ((F_B, b), (F_H, h), (F_HU, hu), (F_HV, hv)):
+            self.set_field(basin, f, a)
+
+    def state(self, basin):
+        return tuple(self.get_field(basin, f) for f in (F_B, F_H, F_HU, F_HV))
+
+    def set_bc(self, basin, kinds):
+        for side, k in enumerate(kinds):
+            self._ck(lib().swe_batch_set_bc(self._h, basin, side, BC[k] if isinstance(k, str) else k),
+                     "swe_batch_set_bc")
+
+    def set_harmonic(self, basin, side, amp=(), omega=(), phase=(), msl=0.0, uamp=(), uomega=(), uphase=(), u0=0.0):
+        def arr(x):
+            a = np.zeros(4)
+            a[:len(x)] = x
+            return a
+        A, O, P, UA, UO, UP = (arr(x) for x in (amp, omega, phase, uamp, uomega, uphase))
+        self._ck(lib().swe_batch_set_harmonic(self._h, basin, side, len(amp), msl, _dp(A), _dp(O), _dp(P),
+                                              len(uamp), u0, _dp(UA), _dp(UO), _dp(UP)), "swe_batch_set_harmonic")
+
+    def get_scalar(self, basin, which):
+        v = ctypes.c_double()
+        self._ck(lib().swe_batch_get_scalar(self._h, basin, which, ctypes.byref(v)), "swe_batch_get_scalar")
+        return v.value
+
+    def set_chunk_rows(self, rows, nwaves=0):
+        self._ck(lib().swe_batch_set_chunk_rows(self._h, int(rows), int(nwaves)), "swe_batch_set_chunk_rows")
+
+    def use_graph(self, on=True):
+        self._ck(lib().swe_batch_use_graph(self._h, int(on)), "swe_batch_use_graph")
+
+    def advance(self, nsteps):
+        m = ctypes.c_int64()
+        self._ck(lib().swe_batch_advance(self._h, int(nsteps), ctypes.byref(m)), "swe_batch_advance")
+        return m.value
+
+    def last_advance_ms(self):
+        v = ctypes.c_double()
+        self._ck(lib().swe_batch_last_advance_ms(self._h, ctypes.byref(v)), "swe_batch_last_advance_ms")
+        return v.value
+
+    def autotune(self, steps=2):
+        r, w = ctypes.c_int(), ctypes.c_int()
+        self._ck(lib().swe_batch_autotune(self._h, int(steps), ctypes.byref(r), ctypes.byref(w)), "swe_batch_autotune")
+        return r.value, w.value

@@ -77,3 +77,31 @@ def large_grid(nx=4096, ny=4096, dx=10.0, seed=2):
                   bc=("flather", "reflect", "reflect", "reflect"),
                   tide_amp=1.0, tide_omega=2 * math.pi / M2_PERIOD_S)
     return (b.ravel(), h.ravel(), z, z.copy()), params
+
+
+def ensemble(nb=512, lo=64, hi=1024, dx=5.0, seed=3, with_state=True):
+    """Batched ensemble of quarter-wave resonators: nb basins, nx and ny drawn
+    log-uniform in [lo, hi]; each a closed rectangular channel of depth U(2,20) m
+    with N(0,0.1) m roughness, Flather forcing at the west end (amplitude 0.1 m,
+    period U(0.5,2) x 4L/sqrt(g h), incoming velocity sqrt(g/h) x eta), walls
+    elsewhere, Manning 0.02, CFL 0.45.  Yields one dict per basin."""
+    rng = SplitMix64(seed)
+    out = []
+    for i in range(nb):
+        nx = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        ny = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        depth = rng.uniform(2.0, 20.0)
+        L = nx * dx
+        c = math.sqrt(G_REF * depth)
+        period = rng.uniform(0.5, 2.0) * 4.0 * L / c
+        spec = dict(nx=nx, ny=ny, dx=dx, depth=depth, period=period, manning=0.02, cfl=0.45,
+                    bc=("flather", "reflect", "reflect", "reflect"), amp=0.1,
+                    omega=2 * math.pi / period, uamp=0.1 * math.sqrt(G_REF / depth))
+        if with_state:
+            W, H = nx + 4, ny + 4
+            b = -depth + 0.1 * rng.normals(W * H)
+            h = np.maximum(0.0, -b)
+            z = np.zeros(W * H)
+            spec["state"] = (b, h, z, z.copy())
+        out.append(spec)
+    return out

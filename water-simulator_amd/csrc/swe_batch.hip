@@ -468,9 +468,9 @@ int swe_batch_autotune(swe_batch *b, int steps, int *best_rows, int *best_waves)
   const int par0 = b->parity;
   std::vector<std::vector<double>> save(nb);
   for (int i = 0; i < nb; ++i) {
-    save[i].resize(3 * b->basins[i].N);
-    for (int f = 0; f < 3; ++f)
-      if ((rc = swe_batch_get_field(b, i, SWE_F_H + f, save[i].data() + f * b->basins[i].N))) return rc;
+    save[i].resize(4 * b->basins[i].N);
+    for (int f = 0; f < 4; ++f)
+      if ((rc = swe_batch_get_field(b, i, SWE_F_B + f, save[i].data() + f * b->basins[i].N))) return rc;
   }
   int cus = 0;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
@@ -486,8 +486,8 @@ int swe_batch_autotune(swe_batch *b, int steps, int *best_rows, int *best_waves)
       std::memcpy(b->hctl, ctl0.data(), sizeof(Ctl) * nb);
       if ((rc = push(b))) return rc;
       for (int i = 0; i < nb; ++i)
-        for (int f = 0; f < 3; ++f)
-          if ((rc = swe_batch_set_field(b, i, SWE_F_H + f, save[i].data() + f * b->basins[i].N))) return rc;
+        for (int f = 0; f < 4; ++f)
+          if ((rc = swe_batch_set_field(b, i, SWE_F_B + f, save[i].data() + f * b->basins[i].N))) return rc;
       int64_t done = 0;
       if ((rc = swe_batch_advance(b, steps, &done))) return rc;
       if (b->last_ms < best) { best = b->last_ms; best_r = rows; best_w = base_waves * wmul; }
@@ -497,8 +497,8 @@ int swe_batch_autotune(swe_batch *b, int steps, int *best_rows, int *best_waves)
   std::memcpy(b->hctl, ctl0.data(), sizeof(Ctl) * nb);
   if ((rc = push(b))) return rc;
   for (int i = 0; i < nb; ++i)
-    for (int f = 0; f < 3; ++f)
-      if ((rc = swe_batch_set_field(b, i, SWE_F_H + f, save[i].data() + f * b->basins[i].N))) return rc;
+    for (int f = 0; f < 4; ++f)
+      if ((rc = swe_batch_set_field(b, i, SWE_F_B + f, save[i].data() + f * b->basins[i].N))) return rc;
   b->nwaves = base_waves;
   if ((rc = swe_batch_set_chunk_rows(b, best_r, best_w))) return rc;
   if (best_rows) *best_rows = best_r;

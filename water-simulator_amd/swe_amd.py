@@ -297,29 +297,8 @@ class Solver:
         _, h, hu, hv = self.state()
         return bool(np.all(np.isfinite(h)) and np.all(np.isfinite(hu)) and np.all(np.isfinite(hv)))
 
-    def fingerprint(self):
-        """FNV-1a 64 over the raw bytes of h, hu, hv (full arrays, ghosts included)."""
-        _, h, hu, hv = self.state()
-        return fnv1a64(h, hu, hv)
-
     def interior(self, a):
         return a.reshape(self.H, self.W)[2:-2, 2:-2]
-
-
-_FNV_P = np.uint64(0x100000001b3)
-
-
-def fnv1a64(*arrays):
-    h = 0xcbf29ce484222325
-    data = b"".join(np.ascontiguousarray(a, dtype="<f8").tobytes() for a in arrays)
-    buf = np.frombuffer(data, dtype=np.uint8)
-    # vectorised over bytes is inherently sequential; process in Python ints (small grids only)
-    P = 0x100000001b3
-    M = 0xFFFFFFFFFFFFFFFF
-    for byte in buf.tolist():
-        h ^= byte
-        h = (h * P) & M
-    return h
 
 
 # ---- geometry (src/geometry.mjs) -----------------------------------------------

@@ -105,3 +105,32 @@ def ensemble(nb=512, lo=64, hi=1024, dx=5.0, seed=3, with_state=True):
             spec["state"] = (b, h, z, z.copy())
         out.append(spec)
     return out
+
+
+def thacker_bowl(n=256, g=G_REF, A=0.5, h0=1.0, a=1000.0):
+    """Thacker's (1981) radially symmetric oscillation in a paraboloid bowl:
+    bed z = h0 (r/a)^2 on an n x n grid of the bowl's diameter / 200 (dx = 2a/200),
+    centred; u = v = 0 and the free surface of the closed form at t = 0.
+    Returns (state, params, exact) where exact(t) gives the closed-form depth."""
+    dx = 2 * a / 200
+    L = n * dx
+    c = L / 2
+    W = n + 4
+    x = (np.arange(W) - 2 + 0.5) * dx
+    X, Y = np.meshgrid(x, x)
+    r2 = (X - c) ** 2 + (Y - c) ** 2
+    omega = math.sqrt(8 * g * h0) / a
+    period = 2 * math.pi / omega
+    bed = h0 * r2 / (a * a)
+
+    def surface(t):
+        q = 1 - A * math.cos(omega * t)
+        return h0 + h0 * (math.sqrt(1 - A * A) / q - 1 - (r2 / (a * a)) * ((1 - A * A) / (q * q) - 1))
+
+    def exact_depth(t):
+        return np.maximum(0.0, surface(t) - bed).ravel()
+
+    h = exact_depth(0.0)
+    z = np.zeros(W * W)
+    params = dict(nx=n, ny=n, dx=dx, g=g, manning=0.0, period=period, bc=("reflect",) * 4)
+    return (bed.ravel(), h, z, z.copy()), params, exact_depth

@@ -170,6 +170,50 @@ int swe_batch_last_advance_ms(swe_batch *b, double *ms);
  * of the current state, restores the state, keeps the fastest */
 int swe_batch_autotune(swe_batch *b, int steps, int *best_rows, int *best_waves);
 
+/* ---- discrete adjoint (no reference counterpart: the reference has no
+ * gradients; these differentiate nsteps self-timed step() calls, swe.mjs:754-814,
+ * with the dt sequence held at the forward's values) ---------------------------
+ * Loss: J = sum over samples s and gauges g of (eta_g(t_s) - obs[s][g])^2, eta
+ * sampled after every `every` steps.  Gradients: dJ/dh0 (initial depth, zero
+ * momentum; dJ/d eta0 on wet cells) and dJ/dn (per-cell Manning n, interior).
+ * Boundaries: reflect, outflow, periodic; no sponge.  The adjoint drives the
+ * context it is created on: its state after any swe_adj_* call is that of the
+ * last forward step run, and a scalar Manning n becomes a per-cell field. */
+typedef struct swe_adj swe_adj;
+/* ckpt: steps between checkpoints (0 = round(sqrt(nsteps))) */
+int swe_adj_create(swe_ctx *ctx, int64_t nsteps, int ckpt, swe_adj **out);
+void swe_adj_destroy(swe_adj *a);
+const char *swe_adj_last_error(const swe_adj *a);
+/* cells: k = j*W + i of interior cells; obs: nsteps/every samples of ng values */
+int swe_adj_set_gauges(swe_adj *a, int ng, const int32_t *cells, int every, const double *obs);
+/* self-timed forward from (h0, hu = hv = 0) with Manning field nfield (NULL keeps
+ * the current one); records dt per step and checkpoints; returns J */
+int swe_adj_forward(swe_adj *a, const double *h0, const double *nfield, double *loss);
+/* J of a forward that replays the recorded dt sequence (finite-difference checks) */
+int swe_adj_loss(swe_adj *a, const double *h0, const double *nfield, double *loss);
+/* dJ/dh0 and dJ/dn of the last swe_adj_forward (W*H each; either may be NULL) */
+int swe_adj_backward(swe_adj *a, double *grad_h0, double *grad_n);
+int swe_adj_get_series(swe_adj *a, double *eta);   /* nsteps/every * ng gauge values */
+int swe_adj_get_dt(swe_adj *a, double *dt);        /* nsteps recorded dt */
+/* HIP-event ms of the last swe_adj_forward and swe_adj_backward */
+int swe_adj_timing(swe_adj *a, double *fwd_ms, double *bwd_ms);
+/* The reverse passes run on host arrays (no GPU needed), for checking them:
+ * lh += (d residual / d state)^T lr for residual(dt) of a state with stored
+ * ghosts (swe.mjs:308-629); BC^T in place (reflect/outflow/periodic,
+ * swe.mjs:289-303); the adjoint of the pre-dryClean state of a stage's end
+ * (stage 1: dryClean; stage 2: dryClean then friction with Manning field nfield
+ * or the scalar, accumulating dJ/dn into lN); hllc^T (swe.mjs:974-1011). */
+int swe_adj_host_residual_rev(const swe_desc *d, const double *h, const double *hu, const double *hv,
+                              const double *b, double dt, const double *lrh, const double *lrhu,
+                              const double *lrhv, double *lh, double *lhu, double *lhv);
+int swe_adj_host_bc_rev(const swe_desc *d, const int32_t bc[4], double *lh, double *lhu, double *lhv);
+int swe_adj_host_epilogue_rev(const swe_desc *d, int stage, const double *pre_h, const double *pre_hu,
+                              const double *pre_hv, const double *nfield, double dt, const double *lout_h,
+                              const double *lout_hu, const double *lout_hv, double *lpre_h, double *lpre_hu,
+                              double *lpre_hv, double *lN);
+void swe_adj_host_hllc_rev(double out[6], double hL, double huL, double hvL, double hR, double huR, double hvR,
+                           double dry, double g, double lm, double ln, double lt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,13 +34,20 @@ def test_thacker256_bit_exact_against_reference():
         dt = sim.step(goldens.hexf(st["dtArg"]))
         assert dt == goldens.hexf(st["dt"]), n
         assert goldens.fingerprint(sim) == int(st["hash"], 16), n
-    # the rest device-resident, self-timed against the end time
+    # the whole run device-resident, self-timed against the end time
     sim2 = goldens.make_sim(gpu_factory, fx)
     T2 = goldens.hexf(fx["final"]["t"])
-    taken = sim2.advance(len(steps) + 10, swe_amd.DT_UNTIL, 0.0, T2)
+    taken = sim2.advance(len(steps), swe_amd.DT_UNTIL, 0.0, T2)
     assert taken == len(steps) == fx["final"]["steps"]
     assert sim2.t == T2
     assert goldens.fingerprint(sim2) == int(steps[-1]["hash"], 16)
+    # at t = T2 every further step is refused (dt = 0): like the reference's
+    # step(), it refreshes the ghost ring and leaves the interior and t alone
+    inner = [sim2.interior(a).copy() for a in (sim2.h, sim2.hu, sim2.hv)]
+    assert sim2.advance(5, swe_amd.DT_UNTIL, 0.0, T2) == 0
+    assert sim2.t == T2
+    for a, b in zip(inner, (sim2.h, sim2.hu, sim2.hv)):
+        assert np.array_equal(a, sim2.interior(b))
 
 
 def test_thacker_closed_form_g981():

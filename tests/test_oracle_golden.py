@@ -101,3 +101,14 @@ def test_planted_one_ulp_state_nudge_is_caught():
     k = sim.W * 10 + 10
     sim.h[k] = np.nextafter(sim.h[k], np.inf)
     assert goldens.replay(sim, fx)
+
+
+def test_oracle_replays_thacker256_two_periods():
+    """The reference's whole 2088-step Thacker run (256x256, dt = min(maxDt, 2T - t)):
+    every step's dt and fingerprint, bit for bit (about a minute of CPU)."""
+    fx = goldens.load("thacker256")
+    sim = goldens.make_sim(oracle.OracleSim, fx)
+    for n, st in enumerate(fx["steps"], start=1):
+        assert sim.step(goldens.hexf(st["dtArg"])) == goldens.hexf(st["dt"]), n
+        assert goldens.fingerprint(sim) == int(st["hash"], 16), n
+    assert sim.t == goldens.hexf(fx["final"]["t"])

@@ -1,0 +1,612 @@
+// Discrete adjoint of the shallow-water step (swe_adj_* in include/swe_amd.h).
+//
+// One forward step at fixed dt (swe.mjs:754-814), with interior values the only
+// ones that carry from step to step:
+//   A' = BC(A); R1 = residual(A'); B = dryClean(A' + dt R1)          (stage 1)
+//   B' = BC(B); R2 = residual(B'); C0 = 0.5 (A' + B' + dt R2)
+//   C  = friction(dryClean(C0), n)                                      (stage 2)
+// Its reverse, given the adjoint of C on the interior: friction^T and dryClean^T,
+// the RK combination^T, residual^T of stage 2, BC^T, then the same for stage 1.
+// residual^T runs as eight one-thread-per-cell passes over the grid with their
+// intermediates in HBM (swe_adjoint.hpp).  The step's stages are recomputed
+// from the step-start state with the forward's own probe sweep.
+//
+// Time: the forward records dt per step on the device and keeps the state every
+// K steps (K ~ sqrt(nsteps)); the reverse sweep re-runs each segment from its
+// checkpoint, keeping the segment's step-start states, then steps back through
+// it.  Memory: (nsteps/K + K) states of 3 W H doubles plus 56 + 27 W H doubles
+// of pass intermediates and stage scratch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "swe_adjoint.hpp"
+#include "swe_ctx.hpp"
+
+using namespace swe;
+
+namespace {
+
+// ---- one-thread-per-cell kernels over the shared bodies -----------------------
+__device__ __forceinline__ bool cell_of(const Params &P, int &i, int &j) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  i = k % P.W;
+  j = k / P.W;
+  return j < P.H;
+}
+
+__global__ void k_adj_cells(Params P, StateP S, const double *b, Work w) {
+  int i, j;
+  if (cell_of(P, i, j)) adj_cells_at(P, S, b, w, i, j);
+}
+__global__ void k_adj_faces(Params P, const double *b, Work w) {
+  int i, j;
+  if (cell_of(P, i, j)) adj_faces_at(P, b, w, i, j);
+}
+__global__ void k_adj_theta(Params P, StateP S, double dt, Work w) {
+  int i, j;
+  if (cell_of(P, i, j)) adj_theta_at(P, S, dt, w, i, j);
+}
+__global__ void k_adj_accum(Params P, StateP lR, Work w) {
+  int i, j;
+  if (cell_of(P, i, j)) adj_accum_at(P, lR, w, i, j);
+}
+__global__ void k_adj_theta_rev(Params P, StateP S, double dt, Work w) {
+  int i, j;
+  if (cell_of(P, i, j)) adj_theta_rev_at(P, S, dt, w, i, j);
+}
+__global__ void k_adj_faces_rev(Params P, const double *b, Work w) {
+  int i, j;
+  if (cell_of(P, i, j)) adj_faces_rev_at(P, b, w, i, j);
+}
+__global__ void k_adj_cells_rev(Params P, StateP S, const double *b, StateP lR, Work w) {
+  int i, j;
+  if (cell_of(P, i, j)) adj_cells_rev_at(P, S, b, lR, w, i, j);
+}
+__global__ void k_adj_slopes_rev(Params P, StateP S, Work w, StateW lS) {
+  int i, j;
+  if (cell_of(P, i, j)) adj_slopes_rev_at(P, S, w, lS, i, j);
+}
+__global__ void k_adj_bc(Params P, StateW l, int pass) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < (pass == 0 ? 4 * P.W : 4 * P.ny)) adj_bc_ghost(P, l, pass, t);
+}
+__global__ void k_adj_combine(Params P, StateP A, StateP B, StateP R, double dt, StateW pre, StateW post, int stage) {
+  int i, j;
+  if (cell_of(P, i, j)) adj_combine_at(P, A, B, R, dt, pre, post, stage, i, j);
+}
+__global__ void k_adj_epilogue(Params P, StateP pre, StateP lOut, const double *nfield, double dt, double *lN,
+                               int stage, StateW lA, StateW lB, StateW lR) {
+  int i, j;
+  if (cell_of(P, i, j)) adj_epilogue_rev_at(P, pre, lOut, nfield, dt, lN, stage, lA, lB, lR, i, j);
+}
+
+__global__ void k_fill(size_t n, double *a, double v) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) a[k] = v;
+}
+// the step about to run takes dt = table[n] (host-chosen dt)
+__global__ void k_ctl_dt(Ctl *ctl, const double *table, int64_t n) {
+  if (threadIdx.x == 0) { ctl->dt = table[n]; ctl->dt_mode = DT_HOST; ctl->cfl_bits = 0ull; }
+}
+// self-timed steps from here on, from t = 0; the max-reduction starts from zero
+__global__ void k_ctl_self(Ctl *ctl) {
+  if (threadIdx.x == 0) { ctl->dt_mode = DT_SELF; ctl->cfl_bits = 0ull; ctl->t = 0.0; ctl->steps = 0; }
+}
+__global__ void k_ctl_record(const Ctl *ctl, double *table, int64_t n) {
+  if (threadIdx.x == 0) table[n] = ctl->dt_last;
+}
+__global__ void k_gauge_sample(int ng, const int *cells, const double *h, const double *b, double *out) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < ng) out[g] = b[cells[g]] + h[cells[g]];
+}
+// d/dh of sum_g (eta_g - obs_g)^2 at one sample; one lane, since gauges may share a cell
+__global__ void k_gauge_inject(int ng, const int *cells, const double *eta, const double *obs, double *lh) {
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (int g = 0; g < ng; ++g) lh[cells[g]] += 2.0 * (eta[g] - obs[g]);
+}
+
+dim3 grid_for(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+StateW st3(double *base, size_t N) { return {base, base + N, base + 2 * N}; }
+StateP sp3(const double *base, size_t N) { return {base, base + N, base + 2 * N}; }
+StateP sp(const StateW &s) { return {s.h, s.hu, s.hv}; }
+
+// residual^T passes on the host: lS += (d residual(S) / d S)^T lR
+void residual_rev_host(const Params &P, const StateP &S, const double *b, double dt, const StateP &lR, const Work &w,
+                       const StateW &lS) {
+  auto each = [&](auto &&f) {
+    for (int j = 0; j < P.H; ++j)
+      for (int i = 0; i < P.W; ++i) f(i, j);
+  };
+  each([&](int i, int j) { adj_cells_at(P, S, b, w, i, j); });
+  each([&](int i, int j) { adj_faces_at(P, b, w, i, j); });
+  each([&](int i, int j) { adj_theta_at(P, S, dt, w, i, j); });
+  each([&](int i, int j) { adj_accum_at(P, lR, w, i, j); });
+  each([&](int i, int j) { adj_theta_rev_at(P, S, dt, w, i, j); });
+  each([&](int i, int j) { adj_faces_rev_at(P, b, w, i, j); });
+  each([&](int i, int j) { adj_cells_rev_at(P, S, b, lR, w, i, j); });
+  each([&](int i, int j) { adj_slopes_rev_at(P, S, w, lS, i, j); });
+}
+
+// Params over host rows for the host entry points (geometry rows in the order
+// of swe_row; the reverse passes use no reciprocals)
+int host_params(const swe_desc *d, const int32_t bc[4], Params &P) {
+  if (!d || d->nx < 2 || d->ny < 1 || d->order < 1 || d->order > 2 || d->limiter < 0 || d->limiter > 2)
+    return SWE_EINVAL;
+  for (int r = 0; r < SWE_ROW_AREA; ++r)
+    if (!d->rows[r]) return SWE_EINVAL;
+  std::memset(&P, 0, sizeof(P));
+  P.nx = d->nx; P.ny = d->ny; P.W = d->nx + 4; P.H = d->ny + 4;
+  P.sphere = d->kind == SWE_SPHERE; P.order = d->order; P.limiter = d->limiter;
+  for (int s = 0; s < 4; ++s) {
+    if (bc && (bc[s] < SWE_BC_REFLECT || bc[s] > SWE_BC_PERIODIC)) return SWE_EINVAL;
+    P.bc[s] = bc ? bc[s] : BC_REFLECT;
+  }
+  P.g = d->g; P.half_g = 0.5 * d->g; P.neg_half_g = -d->g * 0.5; P.rg = 1.0 / d->g;
+  P.min_depth = d->min_depth; P.eps4 = d->eps4; P.cfl = d->cfl; P.R = d->R;
+  P.coriolis = d->coriolis;
+  P.coriolis_on = (d->coriolis != 0.0 && d->coriolis == d->coriolis) ? 1 : 0;
+  P.manning = d->manning;
+  P.friction = (d->manning != 0.0 && d->manning == d->manning) ? 1 : 0;
+  P.geo.dx = d->rows[SWE_ROW_DX]; P.geo.dyN = d->rows[SWE_ROW_DYN]; P.geo.dyS = d->rows[SWE_ROW_DYS];
+  P.geo.dyCFL = d->rows[SWE_ROW_DYCFL]; P.geo.bedPhi = d->rows[SWE_ROW_BEDPHI];
+  P.geo.tanPhi = d->rows[SWE_ROW_TANPHI]; P.geo.fRow = d->rows[SWE_ROW_F]; P.geo.geo = d->rows[SWE_ROW_GEO];
+  return SWE_OK;
+}
+
+}  // namespace
+
+struct swe_adj {
+  swe_ctx *ctx = nullptr;
+  int64_t nsteps = 0;
+  int K = 1;                       // checkpoint interval
+  size_t N = 0;
+  double *work = nullptr;          // S_COUNT * N
+  Work w{};
+  double *ckpt = nullptr;          // ceil(nsteps / K) states of 3N
+  double *seg = nullptr;           // K states of 3N (one segment's step-start states)
+  double *tmp = nullptr;           // stage scratch, T_COUNT states of 3N
+  double *lN = nullptr;            // adjoint of the Manning field
+  double *dt_dev = nullptr;        // recorded dt per step
+  int *gcells = nullptr;
+  double *gobs = nullptr, *geta = nullptr;
+  int ng = 0, every = 1, nsamp = 0;
+  int lam = 0, lap = 0;            // scratch slots holding the running adjoint and lA'
+  bool recorded = false;
+  std::vector<double> h_dt;
+  std::vector<double> h_eta;
+  double loss = 0.0;
+  double fwd_ms = 0.0, bwd_ms = 0.0;
+  std::string err;
+};
+
+namespace {
+
+#define ACHK(a, expr)                                                            \
+  do {                                                                           \
+    hipError_t e_ = (expr);                                                      \
+    if (e_ != hipSuccess) {                                                      \
+      (a)->err = std::string(#expr) + ": " + hipGetErrorString(e_);              \
+      return SWE_EHIP;                                                           \
+    }                                                                            \
+  } while (0)
+
+int afail(swe_adj *a, int code, const std::string &m) {
+  if (a) a->err = m;
+  return code;
+}
+
+// stage scratch slots
+enum { T_AP, T_BP, T_R1, T_R2, T_PRE1, T_PRE2, T_LBP, T_L0, T_L1, T_COUNT };
+StateW tslot(swe_adj *a, int q) { return st3(a->tmp + (size_t)q * 3 * a->N, a->N); }
+StateW cur_state(swe_ctx *c) { return {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]}; }
+
+int copy3(swe_adj *a, const StateW &dst, const StateP &src) {
+  const size_t bytes = a->N * sizeof(double);
+  hipStream_t s = a->ctx->stream;
+  ACHK(a, hipMemcpyAsync(dst.h, src.h, bytes, hipMemcpyDeviceToDevice, s));
+  ACHK(a, hipMemcpyAsync(dst.hu, src.hu, bytes, hipMemcpyDeviceToDevice, s));
+  ACHK(a, hipMemcpyAsync(dst.hv, src.hv, bytes, hipMemcpyDeviceToDevice, s));
+  return SWE_OK;
+}
+
+int enqueue(swe_adj *a) {
+  const int rc = ctx_enqueue_step(a->ctx, 0, 0);
+  if (rc) return afail(a, rc, std::string("forward step: ") + swe_last_error(a->ctx));
+  return SWE_OK;
+}
+
+// forward step n at the recorded dt
+int replay_step(swe_adj *a, int64_t n) {
+  hipLaunchKernelGGL(k_ctl_dt, dim3(1), dim3(64), 0, a->ctx->stream, a->ctx->dctl, a->dt_dev, n);
+  return enqueue(a);
+}
+
+// residual of S (ghosts stored) at dt = table[n] through the forward's probe sweep
+int residual_into(swe_adj *a, const StateP &S, int64_t n, const StateW &R) {
+  swe_ctx *c = a->ctx;
+  hipLaunchKernelGGL(k_ctl_dt, dim3(1), dim3(64), 0, c->stream, c->dctl, a->dt_dev, n);
+  SweepArgs s = ctx_base_args(c);
+  for (int q = 0; q < 4; ++q) s.P.bc[q] = BC_STORED;
+  s.in = S;
+  s.rh = R.h; s.rhu = R.hu; s.rhv = R.hv;
+  s.theta = a->w.s[S_TH];
+  ACHK(a, launch_sweep(MODE_PROBE, s, c->stream));
+  return SWE_OK;
+}
+
+int residual_rev(swe_adj *a, const StateP &S, double dt, const StateP &lR, const StateW &lS) {
+  swe_ctx *c = a->ctx;
+  const Params &P = c->P;
+  hipStream_t st = c->stream;
+  const dim3 g = grid_for(a->N), blk(256);
+  hipLaunchKernelGGL(k_adj_cells, g, blk, 0, st, P, S, c->b, a->w);
+  hipLaunchKernelGGL(k_adj_faces, g, blk, 0, st, P, c->b, a->w);
+  hipLaunchKernelGGL(k_adj_theta, g, blk, 0, st, P, S, dt, a->w);
+  hipLaunchKernelGGL(k_adj_accum, g, blk, 0, st, P, lR, a->w);
+  hipLaunchKernelGGL(k_adj_theta_rev, g, blk, 0, st, P, S, dt, a->w);
+  hipLaunchKernelGGL(k_adj_faces_rev, g, blk, 0, st, P, c->b, a->w);
+  hipLaunchKernelGGL(k_adj_cells_rev, g, blk, 0, st, P, S, c->b, lR, a->w);
+  hipLaunchKernelGGL(k_adj_slopes_rev, g, blk, 0, st, P, S, a->w, lS);
+  ACHK(a, hipGetLastError());
+  return SWE_OK;
+}
+
+int bc_rev(swe_adj *a, const StateW &l) {
+  swe_ctx *c = a->ctx;
+  const Params &P = c->P;
+  hipLaunchKernelGGL(k_adj_bc, grid_for(4 * P.W), dim3(256), 0, c->stream, P, l, 0);
+  hipLaunchKernelGGL(k_adj_bc, grid_for(4 * P.ny), dim3(256), 0, c->stream, P, l, 1);
+  ACHK(a, hipGetLastError());
+  return SWE_OK;
+}
+
+// Step n back: A is the step-start state (interior valid); maps the adjoint of
+// the step's output (slot a->lam) to the adjoint of A (left in slot a->lam).
+int step_rev(swe_adj *a, const StateP &A, int64_t n) {
+  swe_ctx *c = a->ctx;
+  const Params &P = c->P;
+  const double dt = a->h_dt[n];
+  hipStream_t st = c->stream;
+  const dim3 g = grid_for(a->N), blk(256);
+  const StateW Ap = tslot(a, T_AP), Bp = tslot(a, T_BP), R1 = tslot(a, T_R1), R2 = tslot(a, T_R2);
+  const StateW pre1 = tslot(a, T_PRE1), pre2 = tslot(a, T_PRE2), lBp = tslot(a, T_LBP);
+  const StateW lam = tslot(a, a->lam), lAp = tslot(a, a->lap);
+  const StateW none = {nullptr, nullptr, nullptr};
+  int rc;
+  // forward recompute of the two stages
+  if ((rc = copy3(a, Ap, A))) return rc;
+  ACHK(a, launch_apply_bc(P, Ap, c->b, c->dctl, st));
+  if ((rc = residual_into(a, sp(Ap), n, R1))) return rc;
+  hipLaunchKernelGGL(k_adj_combine, g, blk, 0, st, P, sp(Ap), sp(Ap), sp(R1), dt, pre1, Bp, 1);
+  ACHK(a, launch_apply_bc(P, Bp, c->b, c->dctl, st));
+  if ((rc = residual_into(a, sp(Bp), n, R2))) return rc;
+  hipLaunchKernelGGL(k_adj_combine, g, blk, 0, st, P, sp(Ap), sp(Bp), sp(R2), dt, pre2, none, 2);
+  // stage 2 back: R2's slot now holds its adjoint
+  hipLaunchKernelGGL(k_adj_epilogue, g, blk, 0, st, P, sp(pre2), sp(lam), c->mfield, dt, a->lN, 2, lAp, lBp, R2);
+  if ((rc = residual_rev(a, sp(Bp), dt, sp(R2), lBp))) return rc;
+  if ((rc = bc_rev(a, lBp))) return rc;
+  // stage 1 back: R1's slot now holds its adjoint
+  hipLaunchKernelGGL(k_adj_epilogue, g, blk, 0, st, P, sp(pre1), sp(lBp), c->mfield, dt, a->lN, 1, lAp, none, R1);
+  if ((rc = residual_rev(a, sp(Ap), dt, sp(R1), lAp))) return rc;
+  if ((rc = bc_rev(a, lAp))) return rc;
+  std::swap(a->lam, a->lap);
+  return SWE_OK;
+}
+
+int set_initial(swe_adj *a, const double *h0, const double *nfield) {
+  swe_ctx *c = a->ctx;
+  std::vector<double> z(a->N, 0.0);
+  int rc = swe_set_field(c, SWE_F_H, h0);
+  if (!rc) rc = swe_set_field(c, SWE_F_HU, z.data());
+  if (!rc) rc = swe_set_field(c, SWE_F_HV, z.data());
+  if (!rc && nfield) rc = swe_set_field(c, SWE_F_MANNING, nfield);
+  if (rc) return afail(a, rc, std::string("initial state: ") + swe_last_error(c));
+  hipLaunchKernelGGL(k_ctl_self, dim3(1), dim3(64), 0, c->stream, c->dctl);
+  ACHK(a, hipGetLastError());
+  return SWE_OK;
+}
+
+int sample(swe_adj *a, int64_t s) {
+  swe_ctx *c = a->ctx;
+  if (!a->ng) return SWE_OK;
+  hipLaunchKernelGGL(k_gauge_sample, grid_for(a->ng), dim3(256), 0, c->stream, a->ng, a->gcells,
+                     c->st[c->A][0], c->b, a->geta + (size_t)s * a->ng);
+  ACHK(a, hipGetLastError());
+  return SWE_OK;
+}
+
+// forward run from the initial state: self-timed (recording dt and keeping
+// checkpoints) or replaying the recorded dt sequence
+int forward(swe_adj *a, bool record, double *loss) {
+  swe_ctx *c = a->ctx;
+  const size_t N3 = 3 * a->N;
+  int rc;
+  if (record) {
+    // the first dt from the initial state (maxDt, swe.mjs:632-647)
+    ACHK(a, launch_maxdt(c->P, sp(cur_state(c)), c->dctl, c->stream));
+    ACHK(a, launch_finalize(c->dctl, c->mf_part, 0, c->P.cfl, 0, c->stream));
+  }
+  for (int64_t n = 0; n < a->nsteps; ++n) {
+    if (record && n % a->K == 0)
+      if ((rc = copy3(a, st3(a->ckpt + (size_t)(n / a->K) * N3, a->N), sp(cur_state(c))))) return rc;
+    if (record) {
+      if ((rc = enqueue(a))) return rc;
+      hipLaunchKernelGGL(k_ctl_record, dim3(1), dim3(64), 0, c->stream, c->dctl, a->dt_dev, n);
+    } else if ((rc = replay_step(a, n))) {
+      return rc;
+    }
+    if ((n + 1) % a->every == 0 && (n + 1) / a->every <= a->nsamp)
+      if ((rc = sample(a, (n + 1) / a->every - 1))) return rc;
+  }
+  ACHK(a, hipStreamSynchronize(c->stream));
+  if (record) {
+    a->h_dt.assign((size_t)a->nsteps, 0.0);
+    ACHK(a, hipMemcpy(a->h_dt.data(), a->dt_dev, sizeof(double) * a->nsteps, hipMemcpyDeviceToHost));
+    for (int64_t n = 0; n < a->nsteps; ++n)
+      if (!(a->h_dt[n] > 0) || std::isinf(a->h_dt[n]))
+        return afail(a, SWE_ESTATE, "forward step " + std::to_string(n) + " was refused (no wet cell)");
+    a->recorded = true;
+  }
+  const size_t no = (size_t)a->ng * a->nsamp;
+  a->h_eta.assign(no, 0.0);
+  std::vector<double> obs(no);
+  if (no) {
+    ACHK(a, hipMemcpy(a->h_eta.data(), a->geta, sizeof(double) * no, hipMemcpyDeviceToHost));
+    ACHK(a, hipMemcpy(obs.data(), a->gobs, sizeof(double) * no, hipMemcpyDeviceToHost));
+  }
+  double J = 0.0;
+  for (size_t q = 0; q < no; ++q) {
+    const double r = a->h_eta[q] - obs[q];
+    J += r * r;
+  }
+  a->loss = J;
+  if (loss) *loss = J;
+  return SWE_OK;
+}
+
+int inject(swe_adj *a, int64_t n) {   // gauge terms of the state after n steps
+  if (!a->ng || n <= 0 || n % a->every != 0 || n / a->every > a->nsamp) return SWE_OK;
+  const size_t s = (size_t)(n / a->every - 1) * a->ng;
+  hipLaunchKernelGGL(k_gauge_inject, dim3(1), dim3(64), 0, a->ctx->stream, a->ng, a->gcells, a->geta + s,
+                     a->gobs + s, tslot(a, a->lam).h);
+  ACHK(a, hipGetLastError());
+  return SWE_OK;
+}
+
+struct Timer {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ~Timer() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char *swe_adj_last_error(const swe_adj *a) { return a ? a->err.c_str() : "null adjoint"; }
+
+void swe_adj_destroy(swe_adj *a) {
+  if (!a) return;
+  // touches nothing of the context, which may already be gone; hipFree waits
+  // for the device work that uses the buffers
+  (void)hipFree(a->work); (void)hipFree(a->ckpt); (void)hipFree(a->seg); (void)hipFree(a->tmp);
+  (void)hipFree(a->lN); (void)hipFree(a->dt_dev);
+  (void)hipFree(a->gcells); (void)hipFree(a->gobs); (void)hipFree(a->geta);
+  delete a;
+}
+
+int swe_adj_create(swe_ctx *c, int64_t nsteps, int ckpt, swe_adj **out) {
+  if (!c || !out || nsteps < 1 || ckpt < 0) return ctx_fail(c, SWE_EINVAL, "bad adjoint arguments");
+  *out = nullptr;
+  for (int s = 0; s < 4; ++s)
+    if (c->P.bc[s] != BC_REFLECT && c->P.bc[s] != BC_OUTFLOW && c->P.bc[s] != BC_PERIODIC)
+      return ctx_fail(c, SWE_EINVAL, "the adjoint supports reflect, outflow and periodic boundaries");
+  if (c->sponge) return ctx_fail(c, SWE_EINVAL, "the adjoint does not differentiate a sponge forcing");
+  swe_adj *a = new swe_adj();
+  a->ctx = c;
+  a->nsteps = nsteps;
+  a->K = ckpt > 0 ? ckpt : std::max(1, (int)std::lround(std::sqrt((double)nsteps)));
+  a->N = c->N;
+  a->lam = T_L0;
+  a->lap = T_L1;
+  const size_t N = a->N, N3 = 3 * N;
+  const size_t nck = (size_t)((nsteps + a->K - 1) / a->K);
+  auto bail = [&](const char *what, hipError_t e) {
+    const std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    swe_adj_destroy(a);
+    return ctx_fail(c, e == hipErrorOutOfMemory ? SWE_ENOMEM : SWE_EHIP, m);
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&a->work, sizeof(double) * S_COUNT * N)) != hipSuccess) return bail("adjoint passes", e);
+  for (int q = 0; q < S_COUNT; ++q) a->w.s[q] = a->work + (size_t)q * N;
+  if ((e = hipMalloc(&a->ckpt, sizeof(double) * nck * N3)) != hipSuccess) return bail("checkpoints", e);
+  if ((e = hipMalloc(&a->seg, sizeof(double) * (size_t)a->K * N3)) != hipSuccess) return bail("segment", e);
+  if ((e = hipMalloc(&a->tmp, sizeof(double) * T_COUNT * N3)) != hipSuccess) return bail("stage scratch", e);
+  if ((e = hipMalloc(&a->lN, sizeof(double) * N)) != hipSuccess) return bail("Manning adjoint", e);
+  if ((e = hipMalloc(&a->dt_dev, sizeof(double) * (size_t)nsteps)) != hipSuccess) return bail("dt record", e);
+  hipLaunchKernelGGL(k_fill, grid_for(S_COUNT * N), dim3(256), 0, c->stream, S_COUNT * N, a->work, 0.0);
+  hipLaunchKernelGGL(k_fill, grid_for(T_COUNT * N3), dim3(256), 0, c->stream, T_COUNT * N3, a->tmp, 0.0);
+  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail("clear", e);
+  // the differentiated forward always reads a per-cell Manning field
+  if (!c->mfield) {
+    std::vector<double> nf(N, c->manning == c->manning ? c->manning : 0.0);
+    const int rc = swe_set_field(c, SWE_F_MANNING, nf.data());
+    if (rc) {
+      swe_adj_destroy(a);
+      return rc;
+    }
+  }
+  *out = a;
+  return SWE_OK;
+}
+
+int swe_adj_set_gauges(swe_adj *a, int ng, const int32_t *cells, int every, const double *obs) {
+  if (!a || ng < 0 || every < 1 || (ng > 0 && (!cells || !obs))) return afail(a, SWE_EINVAL, "bad gauges");
+  const Params &P = a->ctx->P;
+  for (int q = 0; q < ng; ++q) {
+    const int k = cells[q];
+    if (k < 0 || (size_t)k >= a->N || k % P.W < 2 || k % P.W > P.W - 3 || k / P.W < 2 || k / P.W > P.H - 3)
+      return afail(a, SWE_EINVAL, "gauge " + std::to_string(q) + " is not an interior cell");
+  }
+  (void)hipFree(a->gcells); (void)hipFree(a->gobs); (void)hipFree(a->geta);
+  a->gcells = nullptr; a->gobs = nullptr; a->geta = nullptr;
+  a->ng = ng;
+  a->every = every;
+  a->nsamp = (int)(a->nsteps / every);
+  const size_t no = (size_t)ng * a->nsamp;
+  ACHK(a, hipMalloc(&a->gcells, sizeof(int) * std::max(1, ng)));
+  ACHK(a, hipMalloc(&a->gobs, sizeof(double) * std::max<size_t>(1, no)));
+  ACHK(a, hipMalloc(&a->geta, sizeof(double) * std::max<size_t>(1, no)));
+  if (ng) ACHK(a, hipMemcpy(a->gcells, cells, sizeof(int) * ng, hipMemcpyHostToDevice));
+  if (no) ACHK(a, hipMemcpy(a->gobs, obs, sizeof(double) * no, hipMemcpyHostToDevice));
+  return SWE_OK;
+}
+
+int swe_adj_forward(swe_adj *a, const double *h0, const double *nfield, double *loss) {
+  if (!a || !h0) return afail(a, SWE_EINVAL, "null argument");
+  int rc = set_initial(a, h0, nfield);
+  if (rc) return rc;
+  Timer t;
+  ACHK(a, hipEventCreate(&t.e0));
+  ACHK(a, hipEventCreate(&t.e1));
+  ACHK(a, hipEventRecord(t.e0, a->ctx->stream));
+  a->recorded = false;
+  if ((rc = forward(a, true, loss))) return rc;
+  ACHK(a, hipEventRecord(t.e1, a->ctx->stream));
+  ACHK(a, hipEventSynchronize(t.e1));
+  float ms = 0.f;
+  ACHK(a, hipEventElapsedTime(&ms, t.e0, t.e1));
+  a->fwd_ms = ms;
+  return SWE_OK;
+}
+
+int swe_adj_loss(swe_adj *a, const double *h0, const double *nfield, double *loss) {
+  if (!a || !h0) return afail(a, SWE_EINVAL, "null argument");
+  if (!a->recorded) return afail(a, SWE_ESTATE, "swe_adj_forward has not run");
+  const int rc = set_initial(a, h0, nfield);
+  if (rc) return rc;
+  return forward(a, false, loss);
+}
+
+int swe_adj_get_series(swe_adj *a, double *eta) {
+  if (!a || !eta) return afail(a, SWE_EINVAL, "null argument");
+  std::copy(a->h_eta.begin(), a->h_eta.end(), eta);
+  return SWE_OK;
+}
+
+int swe_adj_get_dt(swe_adj *a, double *dt) {
+  if (!a || !dt) return afail(a, SWE_EINVAL, "null argument");
+  if (!a->recorded) return afail(a, SWE_ESTATE, "swe_adj_forward has not run");
+  std::copy(a->h_dt.begin(), a->h_dt.end(), dt);
+  return SWE_OK;
+}
+
+int swe_adj_backward(swe_adj *a, double *grad_h0, double *grad_n) {
+  if (!a) return SWE_EINVAL;
+  if (!a->recorded) return afail(a, SWE_ESTATE, "swe_adj_forward has not run");
+  swe_ctx *c = a->ctx;
+  const size_t N = a->N, N3 = 3 * N;
+  hipStream_t st = c->stream;
+  Timer t;
+  ACHK(a, hipEventCreate(&t.e0));
+  ACHK(a, hipEventCreate(&t.e1));
+  ACHK(a, hipEventRecord(t.e0, st));
+  int rc;
+  hipLaunchKernelGGL(k_fill, grid_for(N3), dim3(256), 0, st, N3, tslot(a, a->lam).h, 0.0);
+  hipLaunchKernelGGL(k_fill, grid_for(N), dim3(256), 0, st, N, a->lN, 0.0);
+  if ((rc = inject(a, a->nsteps))) return rc;
+  const int64_t nseg = (a->nsteps + a->K - 1) / a->K;
+  for (int64_t sgi = nseg - 1; sgi >= 0; --sgi) {
+    const int64_t n0 = sgi * a->K, n1 = std::min<int64_t>(a->nsteps, n0 + a->K);
+    // re-run the segment from its checkpoint, keeping each step-start state
+    if ((rc = copy3(a, cur_state(c), sp3(a->ckpt + (size_t)sgi * N3, N)))) return rc;
+    for (int64_t n = n0; n < n1; ++n) {
+      if ((rc = copy3(a, st3(a->seg + (size_t)(n - n0) * N3, N), sp(cur_state(c))))) return rc;
+      if (n + 1 < n1 && (rc = replay_step(a, n))) return rc;
+    }
+    for (int64_t n = n1 - 1; n >= n0; --n) {
+      if ((rc = step_rev(a, sp3(a->seg + (size_t)(n - n0) * N3, N), n))) return rc;
+      if ((rc = inject(a, n))) return rc;
+    }
+  }
+  ACHK(a, hipEventRecord(t.e1, st));
+  ACHK(a, hipEventSynchronize(t.e1));
+  float ms = 0.f;
+  ACHK(a, hipEventElapsedTime(&ms, t.e0, t.e1));
+  a->bwd_ms = ms;
+  if (grad_h0) ACHK(a, hipMemcpy(grad_h0, tslot(a, a->lam).h, sizeof(double) * N, hipMemcpyDeviceToHost));
+  if (grad_n) ACHK(a, hipMemcpy(grad_n, a->lN, sizeof(double) * N, hipMemcpyDeviceToHost));
+  return SWE_OK;
+}
+
+int swe_adj_timing(swe_adj *a, double *fwd_ms, double *bwd_ms) {
+  if (!a) return SWE_EINVAL;
+  if (fwd_ms) *fwd_ms = a->fwd_ms;
+  if (bwd_ms) *bwd_ms = a->bwd_ms;
+  return SWE_OK;
+}
+
+// ---- host entry points: the same passes on host arrays ------------------------
+int swe_adj_host_residual_rev(const swe_desc *d, const double *h, const double *hu, const double *hv,
+                              const double *b, double dt, const double *lrh, const double *lrhu,
+                              const double *lrhv, double *lh, double *lhu, double *lhv) {
+  Params P;
+  if (host_params(d, nullptr, P) || !h || !hu || !hv || !b || !lrh || !lrhu || !lrhv || !lh || !lhu || !lhv)
+    return SWE_EINVAL;
+  const size_t N = (size_t)P.W * P.H;
+  std::vector<double> work(S_COUNT * N, 0.0);
+  Work w;
+  for (int q = 0; q < S_COUNT; ++q) w.s[q] = work.data() + (size_t)q * N;
+  residual_rev_host(P, {h, hu, hv}, b, dt, {lrh, lrhu, lrhv}, w, {lh, lhu, lhv});
+  return SWE_OK;
+}
+
+int swe_adj_host_bc_rev(const swe_desc *d, const int32_t bc[4], double *lh, double *lhu, double *lhv) {
+  Params P;
+  if (!bc || host_params(d, bc, P) || !lh || !lhu || !lhv) return SWE_EINVAL;
+  const StateW l = {lh, lhu, lhv};
+  for (int t = 0; t < 4 * P.W; ++t) adj_bc_ghost(P, l, 0, t);
+  for (int t = 0; t < 4 * P.ny; ++t) adj_bc_ghost(P, l, 1, t);
+  return SWE_OK;
+}
+
+int swe_adj_host_epilogue_rev(const swe_desc *d, int stage, const double *pre_h, const double *pre_hu,
+                              const double *pre_hv, const double *nfield, double dt, const double *lout_h,
+                              const double *lout_hu, const double *lout_hv, double *lpre_h, double *lpre_hu,
+                              double *lpre_hv, double *lN) {
+  Params P;
+  if ((stage != 1 && stage != 2) || host_params(d, nullptr, P) || !pre_h || !pre_hu || !pre_hv || !lout_h ||
+      !lout_hu || !lout_hv || !lpre_h || !lpre_hu || !lpre_hv)
+    return SWE_EINVAL;
+  if (nfield) P.friction = 2;
+  const size_t N = (size_t)P.W * P.H;
+  // with lA = 0 and a scratch lR, stage 1 leaves lA = the adjoint of pre;
+  // stage 2 leaves lA = half of it
+  std::vector<double> r(3 * N), z(3 * N);
+  const StateW lA = {lpre_h, lpre_hu, lpre_hv}, lB = st3(z.data(), N), lR = st3(r.data(), N);
+  for (size_t k = 0; k < N; ++k) { lpre_h[k] = 0.0; lpre_hu[k] = 0.0; lpre_hv[k] = 0.0; }
+  for (int j = 0; j < P.H; ++j)
+    for (int i = 0; i < P.W; ++i)
+      adj_epilogue_rev_at(P, {pre_h, pre_hu, pre_hv}, {lout_h, lout_hu, lout_hv}, nfield, dt, lN, stage, lA, lB, lR,
+                          i, j);
+  if (stage == 2)
+    for (size_t k = 0; k < N; ++k) { lpre_h[k] *= 2.0; lpre_hu[k] *= 2.0; lpre_hv[k] *= 2.0; }
+  return SWE_OK;
+}
+
+void swe_adj_host_hllc_rev(double out[6], double hL, double huL, double hvL, double hR, double huR, double hvR,
+                           double dry, double g, double lm, double ln, double lt) {
+  const HllcConst C = {g, 0.5 * g, 1.0 / g, dry};
+  const HllcIn r = hllc_rev(hL, huL, hvL, hR, huR, hvR, C, lm, ln, lt);
+  out[0] = r.hL; out[1] = r.huL; out[2] = r.hvL; out[3] = r.hR; out[4] = r.huR; out[5] = r.hvR;
+}
+
+}  // extern "C"

@@ -8,66 +8,12 @@
 #include <string>
 #include <vector>
 
-#include "../../include/swe_amd.h"
+#include "swe_ctx.hpp"
 #include "swe_device.hpp"
-#include "swe_kernels.hpp"
-
-namespace swe {
-hipError_t launch_sweep(int mode, const SweepArgs &a, hipStream_t st);
-int sweep_waves(const Params &P, int rows_per_chunk);
-const void *k_sweep_stage1_ptr();
-hipError_t launch_apply_bc(const Params &P, StateW S, double *b, const Ctl *ctl, hipStream_t st);
-hipError_t launch_maxdt(const Params &P, StateP S, Ctl *ctl, hipStream_t st);
-hipError_t launch_finalize(Ctl *ctl, const double *mf_part, int nparts, double cfl, int phase, hipStream_t st);
-hipError_t launch_friction(const Params &P, StateW S, const double *mfield, double dt, hipStream_t st);
-hipError_t launch_dry_clean(const Params &P, StateW S, double *part, int blocks, hipStream_t st);
-}  // namespace swe
 
 using namespace swe;
 
-namespace {
-constexpr int MIN_CHUNK = 8;
-enum { R_DX, R_RDX, R_DYN, R_RDYN, R_DYS, R_RDYS, R_DYCFL, R_RDYCFL, R_BED, R_RBED, R_TAN, R_F, R_GEO,
-       R_AREA, R_COUNT };
-}
-
-struct swe_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  Params P{};
-  size_t N = 0;
-  double *rows[R_COUNT] = {};
-  std::vector<double> hrows[SWE_NROWS];
-  double *b = nullptr;
-  double *st[3][3] = {};       // [buffer][h,hu,hv]
-  int A = 0, Bi = 1, C = 2;     // A = current state
-  double *mfield = nullptr, *sponge = nullptr;
-  double *rh = nullptr, *rhu = nullptr, *rhv = nullptr, *theta = nullptr;
-  Ctl *dctl = nullptr;
-  Ctl *hctl = nullptr;          // pinned host shadow
-  double *mf_part = nullptr;
-  int max_parts = 0;
-  int chunk = 0;
-  double manning = 0.0, coriolis = 0.0;
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
-  hipGraph_t graph[2] = {nullptr, nullptr};
-  bool use_graph = false;
-  int graph_chunk = -1;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double last_ms = 0.0;
-  std::string err;
-};
-
 static thread_local std::string g_err;
-
-#define HIPCHK(ctx, expr)                                                        \
-  do {                                                                           \
-    hipError_t e_ = (expr);                                                      \
-    if (e_ != hipSuccess) {                                                      \
-      (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);            \
-      return SWE_EHIP;                                                           \
-    }                                                                            \
-  } while (0)
 
 static int fail(swe_ctx *c, int code, const std::string &msg) {
   if (c) c->err = msg; else g_err = msg;
@@ -749,3 +695,12 @@ int swe_last_advance_ms(swe_ctx *c, double *ms) {
 }
 
 }  // extern "C"
+
+// ---- library-internal entry points used by the adjoint ----------------------
+namespace swe {
+int ctx_fail(swe_ctx *c, int code, const std::string &msg) { return fail(c, code, msg); }
+int ctx_pull(swe_ctx *c) { return pull_ctl(c); }
+int ctx_push(swe_ctx *c) { return push_ctl(c); }
+SweepArgs ctx_base_args(swe_ctx *c) { return base_args(c); }
+int ctx_enqueue_step(swe_ctx *c, int want_theta, int want_rh) { return enqueue_step(c, want_theta, want_rh); }
+}  // namespace swe

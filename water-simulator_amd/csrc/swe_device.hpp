@@ -50,7 +50,7 @@ __host__ __device__ __forceinline__ double vel_den(double h, double eps4) {
   const double h4 = h * h * h * h;
   return __builtin_sqrt(h4 + jmax(h4, eps4));
 }
-__device__ __forceinline__ Vel2 vel2(double hu, double hv, double h, double eps4) {
+__host__ __device__ __forceinline__ Vel2 vel2(double hu, double hv, double h, double eps4) {
   const double den = vel_den(h, eps4);
   const double r = 1.0 / den;
   const double s = SQRT2 * h;
@@ -65,7 +65,7 @@ __host__ __device__ __forceinline__ double vel1(double hq, double h, double eps4
 
 // ---- limiters: swe.mjs:88-141 ---------------------------------------------
 enum { LIM_MINMOD = 0, LIM_MC = 1, LIM_CENTRAL = 2 };
-__device__ __forceinline__ double limit(int lim, double a, double b) {
+__host__ __device__ __forceinline__ double limit(int lim, double a, double b) {
   if (lim == LIM_MC) {
     if (a * b <= 0) return 0.0;
     const double c = 0.5 * (a + b);
@@ -119,7 +119,7 @@ __host__ __device__ __forceinline__ Flux3 hllc(double hL, double huL, double hvL
 }
 
 // ---- Math.hypot (2 args), V8's normalise-by-max Kahan form -----------------
-__device__ __forceinline__ double js_hypot(double a, double b) {
+__host__ __device__ __forceinline__ double js_hypot(double a, double b) {
   if (a != a || b != b) {
     if (__builtin_isinf(a) || __builtin_isinf(b)) return __builtin_inf();
     return __builtin_nan("");

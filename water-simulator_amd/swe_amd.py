@@ -521,3 +521,182 @@ class Batch:
         r, w = ctypes.c_int(), ctypes.c_int()
         self._ck(lib().swe_batch_autotune(self._h, int(steps), ctypes.byref(r), ctypes.byref(w)), "swe_batch_autotune")
         return r.value, w.value
+
+
+# ---- discrete adjoint ----------------------------------------------------------
+
+def _bind_adjoint(L):
+    if getattr(L, "_adj_bound", False):
+        return
+    vp, i, d, dp, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_int64
+    ip = ctypes.POINTER(ctypes.c_int32)
+    D = ctypes.POINTER(Desc)
+    sig = {
+        "swe_adj_create": (i, [vp, i64, i, ctypes.POINTER(vp)]),
+        "swe_adj_destroy": (None, [vp]),
+        "swe_adj_last_error": (ctypes.c_char_p, [vp]),
+        "swe_adj_set_gauges": (i, [vp, i, ip, i, dp]),
+        "swe_adj_forward": (i, [vp, dp, dp, dp]),
+        "swe_adj_loss": (i, [vp, dp, dp, dp]),
+        "swe_adj_backward": (i, [vp, dp, dp]),
+        "swe_adj_get_series": (i, [vp, dp]),
+        "swe_adj_get_dt": (i, [vp, dp]),
+        "swe_adj_timing": (i, [vp, dp, dp]),
+        "swe_adj_host_residual_rev": (i, [D, dp, dp, dp, dp, d, dp, dp, dp, dp, dp, dp]),
+        "swe_adj_host_bc_rev": (i, [D, ip, dp, dp, dp]),
+        "swe_adj_host_epilogue_rev": (i, [D, i, dp, dp, dp, dp, d, dp, dp, dp, dp, dp, dp, dp]),
+        "swe_adj_host_hllc_rev": (None, [dp, d, d, d, d, d, d, d, d, d, d, d]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    L._adj_bound = True
+
+
+def make_desc(nx, ny, rows, *, sphere=False, order=2, limiter="mc", g=9.80665, min_depth=1e-3, eps4=None,
+              cfl=0.45, coriolis=0.0, manning=0.025, R=math.inf):
+    """A swe_desc over host rows; returns (desc, kept arrays) -- keep both alive."""
+    d = Desc()
+    d.nx, d.ny, d.kind, d.order = nx, ny, int(sphere), order
+    d.limiter = LIMITER[limiter] if isinstance(limiter, str) else limiter
+    d.g, d.min_depth = g, min_depth
+    d.eps4 = eps4 if eps4 is not None else js_pow4(max(min_depth, 1e-12))
+    d.cfl, d.coriolis, d.manning, d.R = cfl, coriolis, manning, R
+    keep = {r: np.ascontiguousarray(rows[r], dtype=np.float64) for r in ROWS}
+    for k, r in enumerate(ROWS):
+        d.rows[k] = _dp(keep[r])
+    return d, keep
+
+
+def hllc_rev(hL, huL, hvL, hR, huR, hvR, dry, g, lm, ln, lt):
+    """Adjoint of hllc() (swe.mjs:974-1011) w.r.t. its six state arguments."""
+    L = lib()
+    _bind_adjoint(L)
+    out = np.zeros(6)
+    L.swe_adj_host_hllc_rev(_dp(out), hL, huL, hvL, hR, huR, hvR, dry, g, lm, ln, lt)
+    return out
+
+
+def residual_rev_host(desc, h, hu, hv, b, dt, lrh, lrhu, lrhv):
+    """(d residual(dt) / d (h, hu, hv))^T (lrh, lrhu, lrhv) on the host, for a state with stored ghosts."""
+    L = lib()
+    _bind_adjoint(L)
+    c = [np.ascontiguousarray(x, dtype=np.float64) for x in (h, hu, hv, b, lrh, lrhu, lrhv)]
+    out = [np.zeros_like(c[0]) for _ in range(3)]
+    rc = L.swe_adj_host_residual_rev(ctypes.byref(desc), *[_dp(x) for x in c[:4]], dt, *[_dp(x) for x in c[4:]],
+                                     *[_dp(x) for x in out])
+    if rc != 0:
+        raise SweError(f"swe_adj_host_residual_rev: {rc}")
+    return out
+
+
+def bc_rev_host(desc, bc, lh, lhu, lhv):
+    """BC^T of applyBC() (swe.mjs:289-303) for reflect/outflow/periodic sides, on copies."""
+    L = lib()
+    _bind_adjoint(L)
+    kinds = (ctypes.c_int32 * 4)(*[BC[k] if isinstance(k, str) else k for k in bc])
+    out = [np.array(x, dtype=np.float64, copy=True) for x in (lh, lhu, lhv)]
+    rc = L.swe_adj_host_bc_rev(ctypes.byref(desc), kinds, *[_dp(x) for x in out])
+    if rc != 0:
+        raise SweError(f"swe_adj_host_bc_rev: {rc}")
+    return out
+
+
+def epilogue_rev_host(desc, stage, pre, nfield, dt, lout):
+    """Adjoint of the pre-dryClean state at a stage's end (stage 2 also friction);
+    returns (lpre_h, lpre_hu, lpre_hv, lN)."""
+    L = lib()
+    _bind_adjoint(L)
+    p = [np.ascontiguousarray(x, dtype=np.float64) for x in pre]
+    lo = [np.ascontiguousarray(x, dtype=np.float64) for x in lout]
+    out = [np.zeros_like(p[0]) for _ in range(3)]
+    lN = np.zeros_like(p[0])
+    nf = None if nfield is None else np.ascontiguousarray(nfield, dtype=np.float64)
+    rc = L.swe_adj_host_epilogue_rev(ctypes.byref(desc), stage, *[_dp(x) for x in p],
+                                     None if nf is None else _dp(nf), dt, *[_dp(x) for x in lo],
+                                     *[_dp(x) for x in out], _dp(lN))
+    if rc != 0:
+        raise SweError(f"swe_adj_host_epilogue_rev: {rc}")
+    return out[0], out[1], out[2], lN
+
+
+class Adjoint:
+    """Gradients of a gauge misfit through `nsteps` self-timed steps of a Solver
+    (swe_adj_* of the C ABI): J = sum_s sum_g (eta_g(t_s) - obs[s, g])^2, eta
+    sampled every `every` steps; dJ/dh0 (initial depth with zero momentum) and
+    dJ/dn (per-cell Manning n).  The dt sequence is that of the last forward()."""
+
+    def __init__(self, solver, nsteps, ckpt=0):
+        L = lib()
+        _bind_adjoint(L)
+        self.solver = solver
+        self.nsteps = int(nsteps)
+        h = ctypes.c_void_p()
+        rc = L.swe_adj_create(solver._h, self.nsteps, int(ckpt), ctypes.byref(h))
+        if rc != 0:
+            raise SweError(f"swe_adj_create: {L.swe_last_error(solver._h).decode()}")
+        self._h = h
+        self.ng, self.every = 0, 1
+
+    def _ck(self, rc, what):
+        if rc != 0:
+            raise SweError(f"{what}: {lib().swe_adj_last_error(self._h).decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().swe_adj_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_gauges(self, cells, every, obs):
+        cells = np.ascontiguousarray(cells, dtype=np.int32)
+        obs = np.ascontiguousarray(obs, dtype=np.float64)
+        self.ng, self.every = len(cells), int(every)
+        self._ck(lib().swe_adj_set_gauges(self._h, len(cells), cells.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                          int(every), _dp(obs)), "swe_adj_set_gauges")
+
+    def _args(self, h0, nfield):
+        h0 = np.ascontiguousarray(h0, dtype=np.float64)
+        nf = None if nfield is None else np.ascontiguousarray(nfield, dtype=np.float64)
+        return h0, nf
+
+    def forward(self, h0, nfield=None):
+        h0, nf = self._args(h0, nfield)
+        J = ctypes.c_double()
+        self._ck(lib().swe_adj_forward(self._h, _dp(h0), None if nf is None else _dp(nf), ctypes.byref(J)),
+                 "swe_adj_forward")
+        return J.value
+
+    def loss(self, h0, nfield=None):
+        h0, nf = self._args(h0, nfield)
+        J = ctypes.c_double()
+        self._ck(lib().swe_adj_loss(self._h, _dp(h0), None if nf is None else _dp(nf), ctypes.byref(J)),
+                 "swe_adj_loss")
+        return J.value
+
+    def backward(self):
+        gh = np.zeros(self.solver.N)
+        gn = np.zeros(self.solver.N)
+        self._ck(lib().swe_adj_backward(self._h, _dp(gh), _dp(gn)), "swe_adj_backward")
+        return gh, gn
+
+    def series(self):
+        out = np.zeros((self.nsteps // self.every) * self.ng)
+        self._ck(lib().swe_adj_get_series(self._h, _dp(out)), "swe_adj_get_series")
+        return out.reshape(-1, self.ng) if self.ng else out
+
+    def dts(self):
+        out = np.zeros(self.nsteps)
+        self._ck(lib().swe_adj_get_dt(self._h, _dp(out)), "swe_adj_get_dt")
+        return out
+
+    def timing(self):
+        f, b = ctypes.c_double(), ctypes.c_double()
+        self._ck(lib().swe_adj_timing(self._h, ctypes.byref(f), ctypes.byref(b)), "swe_adj_timing")
+        return f.value, b.value

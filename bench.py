@@ -112,6 +112,72 @@ def batched_measure(steps, warmup, seq_steps):
             "batched_vs_plain": batched / seq}
 
 
+def adjoint_measure(nsteps, every=10, fd_dirs=0):
+    """Config 4: gradient of a gauge misfit w.r.t. the initial surface and the per-cell
+    Manning n on the 1024x1024 shelf (32 gauges every 10 steps), with sqrt(N)
+    checkpointing.  Observations come from a truth run (8 surface humps, Manning
+    0.025 x (1 + 0.3 sin)); the gradient is taken at half the truth surface and the
+    baseline n = 0.025.  Reports device ms of the gradient (recompute + reverse)
+    against one plain forward of the same steps."""
+    import numpy as np
+    import swe_amd
+    import workloads
+    b, ht, p, gauges = workloads.inversion()
+    nx, ny, dx = p["nx"], p["ny"], p["dx"]
+    W, H = nx + 4, ny + 4
+    rows = swe_amd.geometry_cartesian(nx, ny, dx, coriolis=p["coriolis"])
+    s = swe_amd.Solver(nx, ny, rows, manning=p["manning"], coriolis=p["coriolis"], cfl=p["cfl"], dx=dx, dy=dx)
+    z = np.zeros_like(ht)
+    eta_t = np.where(ht > 0, b + ht, 0.0)
+    # first guess: half the truth surface plus 1 cm of seeded noise (the noise keeps
+    # every surface difference far from the limiters' kinks, which finite
+    # differences need)
+    h_eval = np.maximum(0.0, 0.5 * eta_t + 0.01 * workloads.SplitMix64(45).normals(W * H) - b)
+    X, Y = np.meshgrid(np.arange(W), np.arange(H))
+    n_truth = (0.025 * (1 + 0.3 * np.sin(X / 97.0) * np.cos(Y / 131.0))).ravel()
+    n0 = np.full(W * H, 0.025)
+    s.load_state(b, h_eval, z, z)
+    s.set_bc(p["bc"])
+    s.set_manning_field(n0)
+    chunk = s.autotune()
+    s.use_graph(True)
+    s.advance(nsteps)
+    plain_ms = s.last_advance_ms()
+    adj = swe_amd.Adjoint(s, nsteps)
+    nsamp = nsteps // every
+    adj.set_gauges(gauges, every, np.zeros((nsamp, len(gauges))))
+    adj.forward(ht, n_truth)
+    adj.set_gauges(gauges, every, adj.series())
+    J = adj.forward(h_eval, n0)
+    gh, gn = adj.backward()
+    f_ms, b_ms = adj.timing()
+    log(f"[bench] adjoint: plain forward {plain_ms:.1f} ms, recorded forward {f_ms:.1f} ms, gradient {b_ms:.1f} ms "
+        f"({b_ms / plain_ms:.2f}x), J = {J:.4e}")
+    out = {"workload": "inversion_1024x1024_dx50_shelf_seamounts_wetdry_east", "steps": nsteps,
+           "gauges": int(len(gauges)), "sample_every": every, "checkpoint_interval": int(round(math.sqrt(nsteps))),
+           "chunk_rows": chunk, "loss": J, "plain_forward_ms": plain_ms, "recorded_forward_ms": f_ms,
+           "gradient_ms": b_ms, "gradient_over_forward": b_ms / plain_ms,
+           "grad_h0_max": float(np.abs(gh).max()), "grad_n_max": float(np.abs(gn).max())}
+    if fd_dirs > 0:
+        rng = workloads.SplitMix64(44)
+        wet = (h_eval > 1.0)
+        errs = []
+        for q in range(fd_dirs):
+            dh = rng.normals(W * H) * wet * 1e-2
+            dn = rng.normals(W * H) * wet * 1e-3
+            eps = 1e-6
+            fd = (adj.loss(h_eval + eps * dh, n0 + eps * dn) - adj.loss(h_eval - eps * dh, n0 - eps * dn)) / (2 * eps)
+            ad = float(np.dot(gh, dh) + np.dot(gn, dn))
+            errs.append(abs(ad - fd) / abs(fd))
+        out["fd_directions"] = fd_dirs
+        out["fd_rel_err_max"] = max(errs)
+        out["fd_rel_err_median"] = float(np.median(errs))
+        log(f"[bench] adjoint FD over {fd_dirs} directions: max rel err {max(errs):.2e}")
+    adj.close()
+    s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +187,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--batched-steps", type=int, default=100, help="0 skips the batched-ensemble measurement")
     ap.add_argument("--chunk", type=int, default=-1, help="rows per wavefront chunk; -1 autotune, 0 default")
+    ap.add_argument("--adjoint-steps", type=int, default=4000, help="0 skips the adjoint measurement")
+    ap.add_argument("--adjoint-fd", type=int, default=0, help="finite-difference directions checked on the adjoint")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -178,6 +246,8 @@ def main():
     sim.close()
     if args.batched_steps > 0:
         out["batched"] = batched_measure(args.batched_steps, 5, 20)
+    if args.adjoint_steps > 0:
+        out["adjoint"] = adjoint_measure(args.adjoint_steps, fd_dirs=args.adjoint_fd)
     if not args.no_cpu:
         log("[bench] cpu baseline")
         out["cpu_baseline"] = cpu_baseline()

@@ -13,9 +13,12 @@
 //
 // Time: the forward records dt per step on the device and keeps the state every
 // K steps (K ~ sqrt(nsteps)); the reverse sweep re-runs each segment from its
-// checkpoint, keeping the segment's step-start states, then steps back through
-// it.  Memory: (nsteps/K + K) states of 3 W H doubles plus 56 + 27 W H doubles
-// of pass intermediates and stage scratch.
+// checkpoint stage by stage (the forward's probe sweep for the residuals, then
+// the stage combination, dryClean and friction), keeping every step's
+// BC-applied stage inputs and pre-dryClean stage outputs, then steps back
+// through it with no further recomputation.  Memory: nsteps/K states of 3 W H
+// doubles, K steps of 12 W H, plus 56 + 15 W H doubles of pass intermediates
+// and scratch (about 7 GB at 1024x1024 and 4000 steps).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -75,9 +78,10 @@ __global__ void k_adj_bc(Params P, StateW l, int pass) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < (pass == 0 ? 4 * P.W : 4 * P.ny)) adj_bc_ghost(P, l, pass, t);
 }
-__global__ void k_adj_combine(Params P, StateP A, StateP B, StateP R, double dt, StateW pre, StateW post, int stage) {
+__global__ void k_adj_combine(Params P, StateP A, StateP B, StateP R, double dt, StateW pre, StateW post, int stage,
+                              const double *nfield) {
   int i, j;
-  if (cell_of(P, i, j)) adj_combine_at(P, A, B, R, dt, pre, post, stage, i, j);
+  if (cell_of(P, i, j)) adj_combine_at(P, A, B, R, dt, pre, post, stage, nfield, i, j);
 }
 __global__ void k_adj_epilogue(Params P, StateP pre, StateP lOut, const double *nfield, double dt, double *lN,
                                int stage, StateW lA, StateW lB, StateW lR) {
@@ -169,7 +173,7 @@ struct swe_adj {
   double *work = nullptr;          // S_COUNT * N
   Work w{};
   double *ckpt = nullptr;          // ceil(nsteps / K) states of 3N
-  double *seg = nullptr;           // K states of 3N (one segment's step-start states)
+  double *seg = nullptr;           // K steps x {A', pre1, B', pre2}, 12N each
   double *tmp = nullptr;           // stage scratch, T_COUNT states of 3N
   double *lN = nullptr;            // adjoint of the Manning field
   double *dt_dev = nullptr;        // recorded dt per step
@@ -201,8 +205,11 @@ int afail(swe_adj *a, int code, const std::string &m) {
   return code;
 }
 
-// stage scratch slots
-enum { T_AP, T_BP, T_R1, T_R2, T_PRE1, T_PRE2, T_LBP, T_L0, T_L1, T_COUNT };
+// scratch slots
+enum { T_R, T_CUR, T_LBP, T_L0, T_L1, T_COUNT };
+// what one step of a segment keeps: the BC-applied stage inputs and the
+// pre-dryClean stage outputs
+enum { G_AP, G_PRE1, G_BP, G_PRE2, G_COUNT };
 StateW tslot(swe_adj *a, int q) { return st3(a->tmp + (size_t)q * 3 * a->N, a->N); }
 StateW cur_state(swe_ctx *c) { return {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]}; }
 
@@ -266,34 +273,54 @@ int bc_rev(swe_adj *a, const StateW &l) {
   return SWE_OK;
 }
 
-// Step n back: A is the step-start state (interior valid); maps the adjoint of
-// the step's output (slot a->lam) to the adjoint of A (left in slot a->lam).
-int step_rev(swe_adj *a, const StateP &A, int64_t n) {
+StateW gslot(swe_adj *a, int64_t q, int g) { return st3(a->seg + ((size_t)q * G_COUNT + g) * 3 * a->N, a->N); }
+
+// Re-run steps [n0, n1) from the state in slot T_CUR, stage by stage, keeping
+// what the reverse of each step reads (slot q = n - n0).
+int replay_segment(swe_adj *a, int64_t n0, int64_t n1) {
+  swe_ctx *c = a->ctx;
+  const Params &P = c->P;
+  hipStream_t st = c->stream;
+  const dim3 g = grid_for(a->N), blk(256);
+  const StateW R = tslot(a, T_R), cur = tslot(a, T_CUR);
+  const StateW none = {nullptr, nullptr, nullptr};
+  int rc;
+  for (int64_t n = n0; n < n1; ++n) {
+    const double dt = a->h_dt[n];
+    const StateW Ap = gslot(a, n - n0, G_AP), pre1 = gslot(a, n - n0, G_PRE1);
+    const StateW Bp = gslot(a, n - n0, G_BP), pre2 = gslot(a, n - n0, G_PRE2);
+    if ((rc = copy3(a, Ap, sp(cur)))) return rc;
+    ACHK(a, launch_apply_bc(P, Ap, c->b, c->dctl, st));
+    if ((rc = residual_into(a, sp(Ap), n, R))) return rc;
+    hipLaunchKernelGGL(k_adj_combine, g, blk, 0, st, P, sp(Ap), sp(Ap), sp(R), dt, pre1, Bp, 1, c->mfield);
+    ACHK(a, launch_apply_bc(P, Bp, c->b, c->dctl, st));
+    if ((rc = residual_into(a, sp(Bp), n, R))) return rc;
+    hipLaunchKernelGGL(k_adj_combine, g, blk, 0, st, P, sp(Ap), sp(Bp), sp(R), dt, pre2, n + 1 < n1 ? cur : none, 2,
+                       c->mfield);
+    ACHK(a, hipGetLastError());
+  }
+  return SWE_OK;
+}
+
+// Step n back from what replay_segment kept in slot q: maps the adjoint of the
+// step's output (slot a->lam, interior) to the adjoint of its start state.
+int step_rev(swe_adj *a, int64_t q, int64_t n) {
   swe_ctx *c = a->ctx;
   const Params &P = c->P;
   const double dt = a->h_dt[n];
   hipStream_t st = c->stream;
   const dim3 g = grid_for(a->N), blk(256);
-  const StateW Ap = tslot(a, T_AP), Bp = tslot(a, T_BP), R1 = tslot(a, T_R1), R2 = tslot(a, T_R2);
-  const StateW pre1 = tslot(a, T_PRE1), pre2 = tslot(a, T_PRE2), lBp = tslot(a, T_LBP);
-  const StateW lam = tslot(a, a->lam), lAp = tslot(a, a->lap);
+  const StateW Ap = gslot(a, q, G_AP), pre1 = gslot(a, q, G_PRE1), Bp = gslot(a, q, G_BP), pre2 = gslot(a, q, G_PRE2);
+  const StateW lR = tslot(a, T_R), lBp = tslot(a, T_LBP), lam = tslot(a, a->lam), lAp = tslot(a, a->lap);
   const StateW none = {nullptr, nullptr, nullptr};
   int rc;
-  // forward recompute of the two stages
-  if ((rc = copy3(a, Ap, A))) return rc;
-  ACHK(a, launch_apply_bc(P, Ap, c->b, c->dctl, st));
-  if ((rc = residual_into(a, sp(Ap), n, R1))) return rc;
-  hipLaunchKernelGGL(k_adj_combine, g, blk, 0, st, P, sp(Ap), sp(Ap), sp(R1), dt, pre1, Bp, 1);
-  ACHK(a, launch_apply_bc(P, Bp, c->b, c->dctl, st));
-  if ((rc = residual_into(a, sp(Bp), n, R2))) return rc;
-  hipLaunchKernelGGL(k_adj_combine, g, blk, 0, st, P, sp(Ap), sp(Bp), sp(R2), dt, pre2, none, 2);
-  // stage 2 back: R2's slot now holds its adjoint
-  hipLaunchKernelGGL(k_adj_epilogue, g, blk, 0, st, P, sp(pre2), sp(lam), c->mfield, dt, a->lN, 2, lAp, lBp, R2);
-  if ((rc = residual_rev(a, sp(Bp), dt, sp(R2), lBp))) return rc;
+  // stage 2: friction^T, dryClean^T, C0 = 0.5 (A' + B' + dt R2)^T, residual^T, BC^T
+  hipLaunchKernelGGL(k_adj_epilogue, g, blk, 0, st, P, sp(pre2), sp(lam), c->mfield, dt, a->lN, 2, lAp, lBp, lR);
+  if ((rc = residual_rev(a, sp(Bp), dt, sp(lR), lBp))) return rc;
   if ((rc = bc_rev(a, lBp))) return rc;
-  // stage 1 back: R1's slot now holds its adjoint
-  hipLaunchKernelGGL(k_adj_epilogue, g, blk, 0, st, P, sp(pre1), sp(lBp), c->mfield, dt, a->lN, 1, lAp, none, R1);
-  if ((rc = residual_rev(a, sp(Ap), dt, sp(R1), lAp))) return rc;
+  // stage 1: dryClean^T, B0 = A' + dt R1 ^T, residual^T, BC^T
+  hipLaunchKernelGGL(k_adj_epilogue, g, blk, 0, st, P, sp(pre1), sp(lBp), c->mfield, dt, a->lN, 1, lAp, none, lR);
+  if ((rc = residual_rev(a, sp(Ap), dt, sp(lR), lAp))) return rc;
   if ((rc = bc_rev(a, lAp))) return rc;
   std::swap(a->lam, a->lap);
   return SWE_OK;
@@ -428,7 +455,7 @@ int swe_adj_create(swe_ctx *c, int64_t nsteps, int ckpt, swe_adj **out) {
   if ((e = hipMalloc(&a->work, sizeof(double) * S_COUNT * N)) != hipSuccess) return bail("adjoint passes", e);
   for (int q = 0; q < S_COUNT; ++q) a->w.s[q] = a->work + (size_t)q * N;
   if ((e = hipMalloc(&a->ckpt, sizeof(double) * nck * N3)) != hipSuccess) return bail("checkpoints", e);
-  if ((e = hipMalloc(&a->seg, sizeof(double) * (size_t)a->K * N3)) != hipSuccess) return bail("segment", e);
+  if ((e = hipMalloc(&a->seg, sizeof(double) * (size_t)a->K * G_COUNT * N3)) != hipSuccess) return bail("segment", e);
   if ((e = hipMalloc(&a->tmp, sizeof(double) * T_COUNT * N3)) != hipSuccess) return bail("stage scratch", e);
   if ((e = hipMalloc(&a->lN, sizeof(double) * N)) != hipSuccess) return bail("Manning adjoint", e);
   if ((e = hipMalloc(&a->dt_dev, sizeof(double) * (size_t)nsteps)) != hipSuccess) return bail("dt record", e);
@@ -526,14 +553,10 @@ int swe_adj_backward(swe_adj *a, double *grad_h0, double *grad_n) {
   const int64_t nseg = (a->nsteps + a->K - 1) / a->K;
   for (int64_t sgi = nseg - 1; sgi >= 0; --sgi) {
     const int64_t n0 = sgi * a->K, n1 = std::min<int64_t>(a->nsteps, n0 + a->K);
-    // re-run the segment from its checkpoint, keeping each step-start state
-    if ((rc = copy3(a, cur_state(c), sp3(a->ckpt + (size_t)sgi * N3, N)))) return rc;
-    for (int64_t n = n0; n < n1; ++n) {
-      if ((rc = copy3(a, st3(a->seg + (size_t)(n - n0) * N3, N), sp(cur_state(c))))) return rc;
-      if (n + 1 < n1 && (rc = replay_step(a, n))) return rc;
-    }
+    if ((rc = copy3(a, tslot(a, T_CUR), sp3(a->ckpt + (size_t)sgi * N3, N)))) return rc;
+    if ((rc = replay_segment(a, n0, n1))) return rc;
     for (int64_t n = n1 - 1; n >= n0; --n) {
-      if ((rc = step_rev(a, sp3(a->seg + (size_t)(n - n0) * N3, N), n))) return rc;
+      if ((rc = step_rev(a, n - n0, n))) return rc;
       if ((rc = inject(a, n))) return rc;
     }
   }

@@ -499,10 +499,13 @@ __host__ __device__ inline void adj_bc_ghost(const Params &P, const StateW &l, i
 
 // ---- stage combination, dryClean^T and friction^T (swe.mjs:659-704, :734-814) --
 // Forward of a stage's pointwise part at cell k: pre = A + dt R (stage 1) or
-// 0.5 (A + B + dt R) (stage 2), and post = dryClean(pre) on the interior.
+// 0.5 (A + B + dt R) (stage 2), and post = dryClean(pre) on the interior, then
+// for stage 2 applyFriction with the scalar n or the field -- the same
+// expressions, in the same order, as the fused stage kernels, so post is the
+// forward's own next state.
 __host__ __device__ inline void adj_combine_at(const Params &P, const StateP &A, const StateP &B, const StateP &R,
                                                double dt, const StateW &pre, const StateW &post, int stage,
-                                               int i, int j) {
+                                               const double *nfield, int i, int j) {
   const int k = j * P.W + i;
   double h, hu, hv;
   if (stage == 1) {
@@ -517,6 +520,18 @@ __host__ __device__ inline void adj_combine_at(const Params &P, const StateP &A,
     if (i >= 2 && i <= P.W - 3 && j >= 2 && j <= P.H - 3) {
       if (h < 0) h = 0.0;
       if (h <= P.min_depth) { hu = 0.0; hv = 0.0; }
+      if (stage == 2 && P.friction && h > P.min_depth) {
+        const double n = P.friction == 2 ? nfield[k] : P.manning;
+        if (n != 0.0 && n == n) {
+          const Vel2 uv = vel2(hu, hv, h, P.eps4);
+          const double speed = js_hypot(uv.u, uv.v);
+          if (!(speed < 1e-14)) {
+            const double denom = 1 + dt * P.g * n * n * speed / js_pow_pos(h, 4.0 / 3.0);
+            hu = hu / denom;
+            hv = hv / denom;
+          }
+        }
+      }
     }
     post.h[k] = h; post.hu[k] = hu; post.hv[k] = hv;
   }

@@ -134,3 +134,46 @@ def thacker_bowl(n=256, g=G_REF, A=0.5, h0=1.0, a=1000.0):
     z = np.zeros(W * W)
     params = dict(nx=n, ny=n, dx=dx, g=g, manning=0.0, period=period, bc=("reflect",) * 4)
     return (bed.ravel(), h, z, z.copy()), params, exact_depth
+
+
+def inversion(nx=1024, ny=1024, dx=50.0, seed=4, ngauges=32):
+    """Source inversion on a 1024x1024 Cartesian grid, dx = 50 m: an offshore shelf
+    rising from -200 m (west) to -10 m, then a coast over the last 3 % of x that
+    climbs to +5 m (wet/dry on the east edge); 32 Gaussian seamounts (height
+    U(50,180) m, sigma U(1,4) km); N(0,0.5) m roughness.  Hidden truth: a sum of
+    8 Gaussian humps of the surface, amplitude U(-1,1) m, sigma U(1,5) km, u = v = 0.
+    Open (zero-gradient) west side, walls elsewhere; Manning 0.025; f = 1e-4; CFL 0.45.
+    Gauges: 32 seeded wet cells at least 10 cells from the edges.
+    Returns (b, h_truth, params, gauges)."""
+    rng = SplitMix64(seed)
+    W, H = nx + 4, ny + 4
+    x = (np.arange(W) - 2 + 0.5) * dx
+    y = (np.arange(H) - 2 + 0.5) * dx
+    Lx, Ly = nx * dx, ny * dx
+    xf = x / Lx
+    shelf = np.where(xf < 0.97, -200.0 + 190.0 * np.clip(xf, 0, 0.97) / 0.97, -10.0 + 15.0 * (xf - 0.97) / 0.03)
+    b = np.empty((H, W))
+    b[:] = shelf
+
+    def humps(n, amp_lo, amp_hi, s_lo, s_hi, into):
+        for _ in range(n):
+            cx, cy = rng.uniform(0, Lx), rng.uniform(0, Ly)
+            amp, sig = rng.uniform(amp_lo, amp_hi), rng.uniform(s_lo, s_hi)
+            into += amp * np.outer(np.exp(-((y - cy) / sig) ** 2), np.exp(-((x - cx) / sig) ** 2))
+
+    humps(32, 50.0, 180.0, 1000.0, 4000.0, b)
+    b += 0.5 * rng.normals(H * W).reshape(H, W)
+    b = np.minimum(b, np.where(xf < 0.97, -2.0, np.inf)[None, :])   # seamounts stay submerged offshore
+    eta = np.zeros((H, W))
+    humps(8, -1.0, 1.0, 1000.0, 5000.0, eta)
+    h = np.maximum(0.0, eta - b)
+    gauges = []
+    while len(gauges) < ngauges:
+        i = 2 + 10 + int(rng.uniform(0, nx - 20))
+        j = 2 + 10 + int(rng.uniform(0, ny - 20))
+        k = j * W + i
+        if h[j, i] > 5.0 and k not in gauges:
+            gauges.append(k)
+    params = dict(nx=nx, ny=ny, dx=dx, manning=0.025, coriolis=1e-4, cfl=0.45,
+                  bc=("outflow", "reflect", "reflect", "reflect"), nsteps=4000, every=10)
+    return b.ravel(), h.ravel(), params, np.array(gauges, dtype=np.int32)

@@ -178,6 +178,92 @@ def adjoint_measure(nsteps, every=10, fd_dirs=0):
     return out
 
 
+def sphere_measure(steps, adj_steps, every=10):
+    """Config 5: 16-member global-ocean ensemble on the 2160x1080 sphere with the
+    M2/S2/K1 equilibrium tide, one batched launch per phase; throughput over a
+    fixed step count, the 5-day extrapolation, the same member run plain, and the
+    gradient of a gauge misfit (64 open-ocean points) w.r.t. the per-cell Manning n
+    of member 0 over adj_steps steps."""
+    import numpy as np
+    import swe_amd
+    import workloads
+    b, h, p, mann = workloads.sphere_tides()
+    nx, ny = p["nx"], p["ny"]
+    W, H = nx + 4, ny + 4
+    rows = swe_amd.geometry_spherical(nx, ny, R=p["R"], lat0=p["lat0"], lat1=p["lat1"], omega=p["omega"])
+    lat, lon = swe_amd.sphere_lonlat(nx, ny, p["lat0"], p["lat1"])
+    z = np.zeros_like(h)
+    B = swe_amd.Batch([dict(nx=nx, ny=ny, rows=rows, sphere=True, R=p["R"], manning=n, cfl=p["cfl"]) for n in mann])
+    for q in range(len(mann)):
+        B.load_state(q, b, h, z, z)
+        B.set_tide(q, p["tides"], lat, lon)
+    task_rows, waves = B.autotune(steps=2)
+    B.advance(2)
+    t0 = [B.get_scalar(q, swe_amd.S_T) for q in range(len(mann))]
+    B.advance(steps)
+    ms = B.last_advance_ms()
+    dts = [(B.get_scalar(q, swe_amd.S_T) - t0[q]) / steps for q in range(len(mann))]
+    finite = all(B.get_scalar(q, swe_amd.S_NONFINITE) == 0 for q in range(len(mann)))
+    B.close()
+    cells = len(mann) * nx * ny
+    value = cells * steps / (ms * 1e-3)
+    steps_5d = p["days"] * 86400.0 / min(dts)
+    log(f"[bench] sphere ensemble: {value:.4e} cell-updates/s ({len(mann)} x {nx}x{ny}), dt ~ {min(dts):.4f} s, "
+        f"5 days = {steps_5d:.3e} steps = {steps_5d * ms / steps / 3.6e6:.2f} h")
+    # the same member as a plain run
+    s = swe_amd.Solver(nx, ny, rows, sphere=True, R=p["R"], manning=mann[0], cfl=p["cfl"])
+    s.load_state(b, h, z, z)
+    s.set_tide(p["tides"], lat, lon)
+    s.autotune()
+    s.use_graph(True)
+    s.advance(2)
+    s.advance(steps)
+    plain_ms = s.last_advance_ms()
+    plain = nx * ny * steps / (plain_ms * 1e-3)
+    out = {"workload": "sphere_2160x1080_global_ocean_k-2_bathymetry_M2_S2_K1_equilibrium_tide", "members": len(mann),
+           "steps": steps, "value": value, "unit": "cell-updates/s", "task_rows": task_rows,
+           "persistent_wavefronts": waves, "dt_min": min(dts), "steps_for_5_days": steps_5d,
+           "hours_for_5_days_extrapolated": steps_5d * ms / steps / 3.6e6, "plain_member": plain,
+           "finite": finite}
+    if adj_steps > 0:
+        rng = workloads.SplitMix64(51)
+        gauges = []
+        LA = np.repeat(lat, W)
+        while len(gauges) < 64:
+            k = (2 + int(rng.uniform(0, ny))) * W + 2 + int(rng.uniform(0, nx))
+            if h[k] > 1000.0 and abs(LA[k]) < 1.2 and k not in gauges:
+                gauges.append(k)
+        gauges = np.array(gauges, dtype=np.int32)
+        # a surface disturbance to invert for: a 1 m large-scale pattern (truth)
+        # observed from a first guess of half of it, with n = 0.025 (truth)
+        # against member 0's n
+        LO = np.tile(lon, H)
+        eta_t = 1.0 * np.cos(LA) ** 2 * np.cos(3 * LO) * (h > 0)
+        h_t = np.where(h > 0, h + eta_t, 0.0)
+        h_g = np.where(h > 0, h + 0.5 * eta_t, 0.0)
+        n_truth = np.full(W * H, 0.025)
+        n0 = np.full(W * H, mann[0])
+        adj = swe_amd.Adjoint(s, adj_steps)
+        ns = adj_steps // every
+        adj.set_gauges(gauges, every, np.zeros((ns, len(gauges))))
+        adj.forward(h_t, n_truth)
+        adj.set_gauges(gauges, every, adj.series())
+        J = adj.forward(h_g, n0)
+        gh, gn = adj.backward()
+        f_ms, b_ms = adj.timing()
+        s.load_state(b, h, z, z)
+        s.set_manning_field(n0)
+        s.advance(adj_steps)
+        pf_ms = s.last_advance_ms()
+        out["adjoint"] = {"steps": adj_steps, "gauges": 64, "sample_every": every, "loss": J,
+                          "plain_forward_ms": pf_ms, "gradient_ms": b_ms, "gradient_over_forward": b_ms / pf_ms,
+                          "grad_h0_max": float(np.abs(gh).max()), "grad_n_max": float(np.abs(gn).max())}
+        log(f"[bench] sphere adjoint (member 0, {adj_steps} steps): gradient {b_ms:.1f} ms = {b_ms / pf_ms:.2f}x forward")
+        adj.close()
+    s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +275,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=-1, help="rows per wavefront chunk; -1 autotune, 0 default")
     ap.add_argument("--adjoint-steps", type=int, default=4000, help="0 skips the adjoint measurement")
     ap.add_argument("--adjoint-fd", type=int, default=0, help="finite-difference directions checked on the adjoint")
+    ap.add_argument("--sphere-steps", type=int, default=100, help="0 skips the spherical ensemble")
+    ap.add_argument("--sphere-adjoint-steps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -248,6 +336,8 @@ def main():
         out["batched"] = batched_measure(args.batched_steps, 5, 20)
     if args.adjoint_steps > 0:
         out["adjoint"] = adjoint_measure(args.adjoint_steps, fd_dirs=args.adjoint_fd)
+    if args.sphere_steps > 0:
+        out["sphere"] = sphere_measure(args.sphere_steps, args.sphere_adjoint_steps)
     if not args.no_cpu:
         log("[bench] cpu baseline")
         out["cpu_baseline"] = cpu_baseline()

@@ -120,6 +120,16 @@ int swe_step(swe_ctx *ctx, double dt, const double ext[8], int want_theta, int w
  * refused. */
 int swe_advance(swe_ctx *ctx, int64_t nsteps, int dt_mode, double dt_value, double t_end,
                 int64_t *steps_taken);
+/* Equilibrium-tide body force (no reference counterpart: the reference leaves
+ * such forcing to a host hook, swe.mjs:242).  n <= 4 constituents of order m:
+ * 2 (semidiurnal) eta_eq = amp cos^2(lat) cos(omega t + 2 lon + phase), or
+ * 1 (diurnal) eta_eq = amp sin(2 lat) cos(omega t + lon + phase).  Both stages
+ * add g h grad(eta_eq) to the momentum residual, at the stage times t and t + dt,
+ * so a resting ocean relaxes to eta = eta_eq + const.  lat: H cell-centre
+ * latitudes, lon: W cell-centre longitudes, radians.  Spherical grids only;
+ * n = 0 removes it. */
+int swe_set_tide(swe_ctx *ctx, int n, const int32_t *m, const double *amp, const double *omega,
+                 const double *phase, const double *lat, const double *lon);
 /* Flather forcing for swe_advance: eta(t) = msl + sum amp_k cos(omega_k t - phase_k),
  * u(t) = u0 + sum uamp_k cos(uomega_k t - uphase_k), k < 4 (Tide.eta, tide.mjs:103-107). */
 int swe_set_harmonic(swe_ctx *ctx, int side, int n_eta, double msl, const double *amp,
@@ -159,6 +169,9 @@ int swe_batch_set_harmonic(swe_batch *b, int basin, int side, int n_eta, double 
                            const double *omega, const double *phase, int n_u, double u0,
                            const double *uamp, const double *uomega, const double *uphase);
 int swe_batch_get_scalar(swe_batch *b, int basin, int which, double *value);
+/* swe_set_tide for one basin of the batch */
+int swe_batch_set_tide(swe_batch *b, int basin, int n, const int32_t *m, const double *amp, const double *omega,
+                       const double *phase, const double *lat, const double *lon);
 /* rows per task and persistent-grid wavefronts (nwaves <= 0 keeps the current) */
 int swe_batch_set_chunk_rows(swe_batch *b, int rows, int nwaves);
 int swe_batch_use_graph(swe_batch *b, int on);

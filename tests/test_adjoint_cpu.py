@@ -291,15 +291,21 @@ STEP_CASES = [
     dict(id="minmod-mixed", limiter="minmod", bc=("outflow", "reflect", "periodic", "periodic")),
     dict(id="order1-periodic", order=1, limiter="mc", bc=("periodic",) * 4),
     dict(id="mc-wetdry", limiter="mc", bc=("reflect",) * 4, wetdry=True),
+    dict(id="sphere-central", limiter="central", bc=("periodic", "periodic", "reflect", "reflect"), sphere=True),
+    dict(id="sphere-mc", limiter="mc", bc=("periodic", "periodic", "reflect", "reflect"), sphere=True),
 ]
 
 
 @pytest.mark.parametrize("case", STEP_CASES, ids=[c["id"] for c in STEP_CASES])
 def test_step_rev_composition(case):
     nx, ny, dx = 18, 14, 5.0
-    rows = swe_amd.geometry_cartesian(nx, ny, dx)
     order = case.get("order", 2)
     kw = dict(order=order, limiter=case["limiter"], manning=0.0)
+    if case.get("sphere"):
+        rows = swe_amd.geometry_spherical(nx, ny, R=2.0e6, lat0=-50.0, lat1=50.0, omega=7.2921e-5)
+        kw.update(sphere=True, R=2.0e6)
+    else:
+        rows = swe_amd.geometry_cartesian(nx, ny, dx)
     o = oracle.OracleSim(nx, ny, rows, dx=dx, dy=dx, use_manning_field=True, **kw)
     desc, keep = swe_amd.make_desc(nx, ny, rows, **kw)
     b, h, hu, hv = basin(nx, ny, 21, dx=dx, wetdry=case.get("wetdry", False))

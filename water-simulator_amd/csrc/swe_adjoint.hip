@@ -66,9 +66,9 @@ __global__ void k_adj_faces_rev(Params P, const double *b, Work w) {
   int i, j;
   if (cell_of(P, i, j)) adj_faces_rev_at(P, b, w, i, j);
 }
-__global__ void k_adj_cells_rev(Params P, StateP S, const double *b, StateP lR, Work w) {
+__global__ void k_adj_cells_rev(Params P, StateP S, const double *b, StateP lR, Work w, const double *tcs) {
   int i, j;
-  if (cell_of(P, i, j)) adj_cells_rev_at(P, S, b, lR, w, i, j);
+  if (cell_of(P, i, j)) adj_cells_rev_at(P, S, b, lR, w, tcs, i, j);
 }
 __global__ void k_adj_slopes_rev(Params P, StateP S, Work w, StateW lS) {
   int i, j;
@@ -93,9 +93,17 @@ __global__ void k_fill(size_t n, double *a, double v) {
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) a[k] = v;
 }
-// the step about to run takes dt = table[n] (host-chosen dt)
-__global__ void k_ctl_dt(Ctl *ctl, const double *table, int64_t n) {
-  if (threadIdx.x == 0) { ctl->dt = table[n]; ctl->dt_mode = DT_HOST; ctl->cfl_bits = 0ull; }
+// the step about to run takes dt = dts[n] (and starts at t = ts[n] when ts is
+// given); the stage-time tide factors follow, and are also kept in tcs_out
+__global__ void k_ctl_dt(Ctl *ctl, const double *dts, const double *ts, int64_t n, double *tcs_out) {
+  if (threadIdx.x != 0) return;
+  ctl->dt = dts[n];
+  if (ts) ctl->t = ts[n];
+  ctl->dt_mode = DT_HOST;
+  ctl->cfl_bits = 0ull;
+  tide_times(ctl);
+  if (tcs_out)
+    for (int q = 0; q < 2 * MAX_TIDE * 2; ++q) tcs_out[q] = (&ctl->tide_cs[0][0][0])[q];
 }
 // self-timed steps from here on, from t = 0; the max-reduction starts from zero
 __global__ void k_ctl_self(Ctl *ctl) {
@@ -133,7 +141,7 @@ void residual_rev_host(const Params &P, const StateP &S, const double *b, double
   each([&](int i, int j) { adj_accum_at(P, lR, w, i, j); });
   each([&](int i, int j) { adj_theta_rev_at(P, S, dt, w, i, j); });
   each([&](int i, int j) { adj_faces_rev_at(P, b, w, i, j); });
-  each([&](int i, int j) { adj_cells_rev_at(P, S, b, lR, w, i, j); });
+  each([&](int i, int j) { adj_cells_rev_at(P, S, b, lR, w, nullptr, i, j); });
   each([&](int i, int j) { adj_slopes_rev_at(P, S, w, lS, i, j); });
 }
 
@@ -177,6 +185,8 @@ struct swe_adj {
   double *tmp = nullptr;           // stage scratch, T_COUNT states of 3N
   double *lN = nullptr;            // adjoint of the Manning field
   double *dt_dev = nullptr;        // recorded dt per step
+  double *t_dev = nullptr;         // start time of every step
+  double *tcs_seg = nullptr;       // K steps x 2 stages x MAX_TIDE x (cos, sin)
   int *gcells = nullptr;
   double *gobs = nullptr, *geta = nullptr;
   int ng = 0, every = 1, nsamp = 0;
@@ -228,18 +238,21 @@ int enqueue(swe_adj *a) {
   return SWE_OK;
 }
 
-// forward step n at the recorded dt
+// forward step n at the recorded dt (t accumulates on the device as in the
+// recorded run)
 int replay_step(swe_adj *a, int64_t n) {
-  hipLaunchKernelGGL(k_ctl_dt, dim3(1), dim3(64), 0, a->ctx->stream, a->ctx->dctl, a->dt_dev, n);
+  hipLaunchKernelGGL(k_ctl_dt, dim3(1), dim3(64), 0, a->ctx->stream, a->ctx->dctl, a->dt_dev, (const double *)nullptr,
+                     n, (double *)nullptr);
   return enqueue(a);
 }
 
-// residual of S (ghosts stored) at dt = table[n] through the forward's probe sweep
-int residual_into(swe_adj *a, const StateP &S, int64_t n, const StateW &R) {
+// residual of S (ghosts stored) through the forward's probe sweep, at the dt
+// and stage time the control block holds
+int residual_into(swe_adj *a, const StateP &S, int stage, const StateW &R) {
   swe_ctx *c = a->ctx;
-  hipLaunchKernelGGL(k_ctl_dt, dim3(1), dim3(64), 0, c->stream, c->dctl, a->dt_dev, n);
   SweepArgs s = ctx_base_args(c);
   for (int q = 0; q < 4; ++q) s.P.bc[q] = BC_STORED;
+  s.P.probe_stage2 = stage == 2;
   s.in = S;
   s.rh = R.h; s.rhu = R.hu; s.rhv = R.hv;
   s.theta = a->w.s[S_TH];
@@ -247,7 +260,7 @@ int residual_into(swe_adj *a, const StateP &S, int64_t n, const StateW &R) {
   return SWE_OK;
 }
 
-int residual_rev(swe_adj *a, const StateP &S, double dt, const StateP &lR, const StateW &lS) {
+int residual_rev(swe_adj *a, const StateP &S, double dt, const StateP &lR, const StateW &lS, const double *tcs) {
   swe_ctx *c = a->ctx;
   const Params &P = c->P;
   hipStream_t st = c->stream;
@@ -258,7 +271,7 @@ int residual_rev(swe_adj *a, const StateP &S, double dt, const StateP &lR, const
   hipLaunchKernelGGL(k_adj_accum, g, blk, 0, st, P, lR, a->w);
   hipLaunchKernelGGL(k_adj_theta_rev, g, blk, 0, st, P, S, dt, a->w);
   hipLaunchKernelGGL(k_adj_faces_rev, g, blk, 0, st, P, c->b, a->w);
-  hipLaunchKernelGGL(k_adj_cells_rev, g, blk, 0, st, P, S, c->b, lR, a->w);
+  hipLaunchKernelGGL(k_adj_cells_rev, g, blk, 0, st, P, S, c->b, lR, a->w, tcs);
   hipLaunchKernelGGL(k_adj_slopes_rev, g, blk, 0, st, P, S, a->w, lS);
   ACHK(a, hipGetLastError());
   return SWE_OK;
@@ -289,12 +302,14 @@ int replay_segment(swe_adj *a, int64_t n0, int64_t n1) {
     const double dt = a->h_dt[n];
     const StateW Ap = gslot(a, n - n0, G_AP), pre1 = gslot(a, n - n0, G_PRE1);
     const StateW Bp = gslot(a, n - n0, G_BP), pre2 = gslot(a, n - n0, G_PRE2);
+    hipLaunchKernelGGL(k_ctl_dt, dim3(1), dim3(64), 0, st, c->dctl, a->dt_dev, a->t_dev, n,
+                       a->tcs_seg + (size_t)(n - n0) * 2 * MAX_TIDE * 2);
     if ((rc = copy3(a, Ap, sp(cur)))) return rc;
     ACHK(a, launch_apply_bc(P, Ap, c->b, c->dctl, st));
-    if ((rc = residual_into(a, sp(Ap), n, R))) return rc;
+    if ((rc = residual_into(a, sp(Ap), 1, R))) return rc;
     hipLaunchKernelGGL(k_adj_combine, g, blk, 0, st, P, sp(Ap), sp(Ap), sp(R), dt, pre1, Bp, 1, c->mfield);
     ACHK(a, launch_apply_bc(P, Bp, c->b, c->dctl, st));
-    if ((rc = residual_into(a, sp(Bp), n, R))) return rc;
+    if ((rc = residual_into(a, sp(Bp), 2, R))) return rc;
     hipLaunchKernelGGL(k_adj_combine, g, blk, 0, st, P, sp(Ap), sp(Bp), sp(R), dt, pre2, n + 1 < n1 ? cur : none, 2,
                        c->mfield);
     ACHK(a, hipGetLastError());
@@ -313,14 +328,15 @@ int step_rev(swe_adj *a, int64_t q, int64_t n) {
   const StateW Ap = gslot(a, q, G_AP), pre1 = gslot(a, q, G_PRE1), Bp = gslot(a, q, G_BP), pre2 = gslot(a, q, G_PRE2);
   const StateW lR = tslot(a, T_R), lBp = tslot(a, T_LBP), lam = tslot(a, a->lam), lAp = tslot(a, a->lap);
   const StateW none = {nullptr, nullptr, nullptr};
+  const double *tcs = a->tcs_seg + (size_t)q * 2 * MAX_TIDE * 2;
   int rc;
   // stage 2: friction^T, dryClean^T, C0 = 0.5 (A' + B' + dt R2)^T, residual^T, BC^T
   hipLaunchKernelGGL(k_adj_epilogue, g, blk, 0, st, P, sp(pre2), sp(lam), c->mfield, dt, a->lN, 2, lAp, lBp, lR);
-  if ((rc = residual_rev(a, sp(Bp), dt, sp(lR), lBp))) return rc;
+  if ((rc = residual_rev(a, sp(Bp), dt, sp(lR), lBp, tcs + 2 * MAX_TIDE))) return rc;
   if ((rc = bc_rev(a, lBp))) return rc;
   // stage 1: dryClean^T, B0 = A' + dt R1 ^T, residual^T, BC^T
   hipLaunchKernelGGL(k_adj_epilogue, g, blk, 0, st, P, sp(pre1), sp(lBp), c->mfield, dt, a->lN, 1, lAp, none, lR);
-  if ((rc = residual_rev(a, sp(Ap), dt, sp(lR), lAp))) return rc;
+  if ((rc = residual_rev(a, sp(Ap), dt, sp(lR), lAp, tcs))) return rc;
   if ((rc = bc_rev(a, lAp))) return rc;
   std::swap(a->lam, a->lap);
   return SWE_OK;
@@ -378,6 +394,11 @@ int forward(swe_adj *a, bool record, double *loss) {
     for (int64_t n = 0; n < a->nsteps; ++n)
       if (!(a->h_dt[n] > 0) || std::isinf(a->h_dt[n]))
         return afail(a, SWE_ESTATE, "forward step " + std::to_string(n) + " was refused (no wet cell)");
+    // step start times, accumulated as the control block accumulates them
+    std::vector<double> ts(a->nsteps);
+    double t = 0.0;
+    for (int64_t n = 0; n < a->nsteps; ++n) { ts[n] = t; t = t + a->h_dt[n]; }
+    ACHK(a, hipMemcpy(a->t_dev, ts.data(), sizeof(double) * a->nsteps, hipMemcpyHostToDevice));
     a->recorded = true;
   }
   const size_t no = (size_t)a->ng * a->nsamp;
@@ -425,7 +446,7 @@ void swe_adj_destroy(swe_adj *a) {
   // touches nothing of the context, which may already be gone; hipFree waits
   // for the device work that uses the buffers
   (void)hipFree(a->work); (void)hipFree(a->ckpt); (void)hipFree(a->seg); (void)hipFree(a->tmp);
-  (void)hipFree(a->lN); (void)hipFree(a->dt_dev);
+  (void)hipFree(a->lN); (void)hipFree(a->dt_dev); (void)hipFree(a->t_dev); (void)hipFree(a->tcs_seg);
   (void)hipFree(a->gcells); (void)hipFree(a->gobs); (void)hipFree(a->geta);
   delete a;
 }
@@ -459,6 +480,9 @@ int swe_adj_create(swe_ctx *c, int64_t nsteps, int ckpt, swe_adj **out) {
   if ((e = hipMalloc(&a->tmp, sizeof(double) * T_COUNT * N3)) != hipSuccess) return bail("stage scratch", e);
   if ((e = hipMalloc(&a->lN, sizeof(double) * N)) != hipSuccess) return bail("Manning adjoint", e);
   if ((e = hipMalloc(&a->dt_dev, sizeof(double) * (size_t)nsteps)) != hipSuccess) return bail("dt record", e);
+  if ((e = hipMalloc(&a->t_dev, sizeof(double) * (size_t)nsteps)) != hipSuccess) return bail("t record", e);
+  if ((e = hipMalloc(&a->tcs_seg, sizeof(double) * (size_t)a->K * 2 * MAX_TIDE * 2)) != hipSuccess)
+    return bail("tide factors", e);
   hipLaunchKernelGGL(k_fill, grid_for(S_COUNT * N), dim3(256), 0, c->stream, S_COUNT * N, a->work, 0.0);
   hipLaunchKernelGGL(k_fill, grid_for(T_COUNT * N3), dim3(256), 0, c->stream, T_COUNT * N3, a->tmp, 0.0);
   if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail("clear", e);

@@ -357,7 +357,7 @@ __host__ __device__ inline void adj_faces_rev_at(const Params &P, const double *
 // slopes; the bed (and spherical geometric) term^T and Coriolis^T
 // (swe.mjs:570-628)
 __host__ __device__ inline void adj_cells_rev_at(const Params &P, const StateP &S, const double *b, const StateP &lR,
-                                                 const Work &w, int i, int j) {
+                                                 const Work &w, const double *tcs, int i, int j) {
   const int W = P.W, k = j * W + i;
   double leta = 0, lU = 0, lV = 0, lsE[2] = {0, 0}, lsU[2] = {0, 0}, lsV[2] = {0, 0};
   for (int d = 0; d < 2; ++d) {
@@ -399,6 +399,14 @@ __host__ __device__ inline void adj_cells_rev_at(const Params &P, const StateP &
       lU += tR * (S.hv[k] * lrhu - S.hu[k] * lrhv);
     } else if (P.coriolis_on) {
       lhvC = P.coriolis * lrhu; lhuC = -P.coriolis * lrhv;
+    }
+    // equilibrium tide, linear in h: rhu += h X sin(theta), rhv += h Y cos(theta);
+    // tcs holds (cos, sin)(omega t_s + phase) of the stage
+    for (int c = 0; c < P.tide_n; ++c) {
+      const double *T = P.tide + (size_t)c * (2 * P.H + 2 * W);
+      const double cl = T[2 * P.H + i], sl = T[2 * P.H + W + i];
+      const double tc = tcs[2 * c], ts = tcs[2 * c + 1];
+      leta += lrhu * T[j] * (ts * cl + tc * sl) + lrhv * T[P.H + j] * (tc * cl - ts * sl);
     }
   }
   w.s[S_LETA][k] = leta; w.s[S_LU][k] = lU; w.s[S_LV][k] = lV;

@@ -29,6 +29,8 @@ const void *k_sweep_batch_ptr();
 hipError_t launch_apply_bc_batch(const BasinDev *basins, int nb, int max_ghost, int parity, int stage, hipStream_t st);
 hipError_t launch_maxdt_batch(const BasinDev *basins, int nb, int max_cells, int parity, hipStream_t st);
 hipError_t launch_finalize_batch(const BasinDev *basins, int nb, unsigned *qctr, int phase, hipStream_t st);
+int tide_table(const Params &P, int n, const int32_t *m, const double *amp, const double *lat, const double *lon,
+               std::vector<double> &out);
 }  // namespace swe
 
 using namespace swe;
@@ -49,6 +51,7 @@ struct Basin {
   int max_parts = 0;
   double manning = 0.0, coriolis = 0.0;
   bool has_mfield = false, has_sponge = false;
+  double *tide = nullptr;  // equilibrium-tide coefficient table of this basin
 };
 }  // namespace
 
@@ -291,6 +294,7 @@ void swe_batch_destroy(swe_batch *b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   drop_graphs(b);
+  for (auto &B : b->basins) (void)hipFree(B.tide);
   (void)hipFree(b->fields); (void)hipFree(b->rows); (void)hipFree(b->parts);
   (void)hipFree(b->dctl); (void)hipFree(b->dbasins); (void)hipFree(b->dtasks); (void)hipFree(b->qctr);
   if (b->hctl) (void)hipHostFree(b->hctl);
@@ -366,6 +370,36 @@ int swe_batch_set_harmonic(swe_batch *b, int i, int side, int n_eta, double msl,
   for (int k = 0; k < n_eta; ++k) { hm.amp[k] = amp[k]; hm.omega[k] = omega[k]; hm.phase[k] = phase[k]; }
   for (int k = 0; k < n_u; ++k) { hm.uamp[k] = uamp[k]; hm.uomega[k] = uomega[k]; hm.uphase[k] = uphase[k]; }
   b->hctl[i].harm_on = 1;
+  return push(b);
+}
+
+int swe_batch_set_tide(swe_batch *b, int i, int n, const int32_t *m, const double *amp, const double *omega,
+                       const double *phase, const double *lat, const double *lon) {
+  if (!b || i < 0 || i >= (int)b->basins.size()) return bfail(b, SWE_EINVAL, "bad basin");
+  Basin &B = b->basins[i];
+  if (n < 0 || n > MAX_TIDE) return bfail(b, SWE_EINVAL, "tide: 0..4 constituents");
+  if (n > 0 && (!B.P.sphere || !(B.P.R > 0) || std::isinf(B.P.R)))
+    return bfail(b, SWE_EINVAL, "tide: the equilibrium tide needs a spherical grid");
+  std::vector<double> T;
+  if (n > 0 && (!omega || !phase || tide_table(B.P, n, m, amp, lat, lon, T)))
+    return bfail(b, SWE_EINVAL, "tide: bad constituents (m must be 1 or 2, amplitudes and angles finite)");
+  BCHK(b, hipStreamSynchronize(b->stream));
+  drop_graphs(b);
+  BCHK(b, hipFree(B.tide));
+  B.tide = nullptr;
+  B.P.tide = nullptr;
+  B.P.tide_n = 0;
+  if (n > 0) {
+    BCHK(b, hipMalloc(&B.tide, sizeof(double) * T.size()));
+    BCHK(b, hipMemcpy(B.tide, T.data(), sizeof(double) * T.size(), hipMemcpyHostToDevice));
+    B.P.tide = B.tide;
+    B.P.tide_n = n;
+  }
+  b->dirty = true;
+  int rc = pull(b);
+  if (rc) return rc;
+  b->hctl[i].tide_n = n;
+  for (int k = 0; k < n; ++k) { b->hctl[i].tide_om[k] = omega[k]; b->hctl[i].tide_ph[k] = phase[k]; }
   return push(b);
 }
 

@@ -27,6 +27,8 @@ static void refresh_params(swe_ctx *c) {
   P.coriolis = c->coriolis;
   P.coriolis_on = (c->coriolis != 0.0 && c->coriolis == c->coriolis) ? 1 : 0;
   P.sponge = c->sponge ? 1 : 0;
+  P.tide = c->tide;
+  P.tide_n = c->tide_n;
   Geo &G = P.geo;
   G.dx = c->rows[R_DX]; G.rdx = c->rows[R_RDX]; G.dyN = c->rows[R_DYN]; G.rdyN = c->rows[R_RDYN];
   G.dyS = c->rows[R_DYS]; G.rdyS = c->rows[R_RDYS]; G.dyCFL = c->rows[R_DYCFL]; G.rdyCFL = c->rows[R_RDYCFL];
@@ -175,7 +177,7 @@ void swe_destroy(swe_ctx *c) {
   (void)hipFree(c->b);
   for (auto &s : c->st) for (auto *p : s) (void)hipFree(p);
   for (auto *p : c->rows) (void)hipFree(p);
-  (void)hipFree(c->mfield); (void)hipFree(c->sponge);
+  (void)hipFree(c->mfield); (void)hipFree(c->sponge); (void)hipFree(c->tide);
   (void)hipFree(c->rh); (void)hipFree(c->rhu); (void)hipFree(c->rhv); (void)hipFree(c->theta);
   (void)hipFree(c->dctl); (void)hipFree(c->mf_part);
   if (c->hctl) (void)hipHostFree(c->hctl);
@@ -348,6 +350,7 @@ int swe_residual(swe_ctx *c, double dt) {
   if ((rc = pull_ctl(c))) return rc;
   c->hctl->dt = dt;
   if ((rc = push_ctl(c))) return rc;
+  if (c->tide_n) HIPCHK(c, launch_tide_times(c->dctl, c->stream));
   SweepArgs a = base_args(c);
   for (int s = 0; s < 4; ++s) a.P.bc[s] = BC_STORED;      // ghosts as the caller left them
   a.in = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
@@ -446,6 +449,7 @@ int swe_step(swe_ctx *c, double dt, const double ext[8], int want_theta, int wan
   h->dt_mode = self ? DT_SELF : DT_HOST;
   if (!self) h->dt = dt;
   if ((rc = push_ctl(c))) return rc;
+  if (!self && c->tide_n) HIPCHK(c, launch_tide_times(c->dctl, c->stream));
   if (self) {
     StateP S = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
     HIPCHK(c, launch_maxdt(c->P, S, c->dctl, c->stream));
@@ -458,6 +462,36 @@ int swe_step(swe_ctx *c, double dt, const double ext[8], int want_theta, int wan
   if (mass_floored) *mass_floored = h->mf;
   h->dt_mode = DT_HOST;
   return push_ctl(c);
+}
+
+int swe_set_tide(swe_ctx *c, int n, const int32_t *m, const double *amp, const double *omega, const double *phase,
+                 const double *lat, const double *lon) {
+  if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  if (n < 0 || n > MAX_TIDE) return fail(c, SWE_EINVAL, "tide: 0..4 constituents");
+  if (n > 0 && (!c->P.sphere || !(c->P.R > 0) || std::isinf(c->P.R)))
+    return fail(c, SWE_EINVAL, "tide: the equilibrium tide needs a spherical grid");
+  std::vector<double> T;
+  if (n > 0 && (!omega || !phase || tide_table(c->P, n, m, amp, lat, lon, T)))
+    return fail(c, SWE_EINVAL, "tide: bad constituents (m must be 1 or 2, amplitudes and angles finite)");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);
+  HIPCHK(c, hipFree(c->tide));
+  c->tide = nullptr;
+  c->tide_n = 0;
+  if (n > 0) {
+    HIPCHK(c, hipMalloc(&c->tide, sizeof(double) * T.size()));
+    HIPCHK(c, hipMemcpy(c->tide, T.data(), sizeof(double) * T.size(), hipMemcpyHostToDevice));
+    c->tide_n = n;
+  }
+  refresh_params(c);
+  int rc = pull_ctl(c);
+  if (rc) return rc;
+  c->hctl->tide_n = n;
+  for (int k = 0; k < n; ++k) { c->hctl->tide_om[k] = omega[k]; c->hctl->tide_ph[k] = phase[k]; }
+  if ((rc = push_ctl(c))) return rc;
+  if (n) HIPCHK(c, launch_tide_times(c->dctl, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SWE_OK;
 }
 
 int swe_set_harmonic(swe_ctx *c, int side, int n_eta, double msl, const double *amp, const double *omega,
@@ -698,6 +732,32 @@ int swe_last_advance_ms(swe_ctx *c, double *ms) {
 
 // ---- library-internal entry points used by the adjoint ----------------------
 namespace swe {
+int tide_table(const Params &P, int n, const int32_t *m, const double *amp, const double *lat, const double *lon,
+               std::vector<double> &out) {
+  if (n < 0 || n > MAX_TIDE || (n > 0 && (!m || !amp || !lat || !lon))) return SWE_EINVAL;
+  const int W = P.W, H = P.H;
+  out.assign((size_t)n * (2 * H + 2 * W), 0.0);
+  for (int c = 0; c < n; ++c) {
+    if ((m[c] != 1 && m[c] != 2) || !std::isfinite(amp[c])) return SWE_EINVAL;
+    double *X = out.data() + (size_t)c * (2 * H + 2 * W), *Y = X + H, *C = Y + H, *S = C + W;
+    const double gA = P.g * amp[c] / P.R;
+    for (int j = 0; j < H; ++j) {
+      const double phi = lat[j];
+      if (!std::isfinite(phi)) return SWE_EINVAL;
+      // d eta_eq / (R cos(phi) d lon) and d eta_eq / (R d phi), times g, per unit
+      // sin resp. cos of the phase
+      if (m[c] == 2) { X[j] = -2.0 * gA * std::cos(phi); Y[j] = -gA * std::sin(2.0 * phi); }
+      else { X[j] = -2.0 * gA * std::sin(phi); Y[j] = 2.0 * gA * std::cos(2.0 * phi); }
+    }
+    for (int i = 0; i < W; ++i) {
+      if (!std::isfinite(lon[i])) return SWE_EINVAL;
+      C[i] = std::cos(m[c] * lon[i]);
+      S[i] = std::sin(m[c] * lon[i]);
+    }
+  }
+  return SWE_OK;
+}
+
 int ctx_fail(swe_ctx *c, int code, const std::string &msg) { return fail(c, code, msg); }
 int ctx_pull(swe_ctx *c) { return pull_ctl(c); }
 int ctx_push(swe_ctx *c) { return push_ctl(c); }

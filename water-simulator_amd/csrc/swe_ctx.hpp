@@ -18,6 +18,11 @@ hipError_t launch_maxdt(const Params &P, StateP S, Ctl *ctl, hipStream_t st);
 hipError_t launch_finalize(Ctl *ctl, const double *mf_part, int nparts, double cfl, int phase, hipStream_t st);
 hipError_t launch_friction(const Params &P, StateW S, const double *mfield, double dt, hipStream_t st);
 hipError_t launch_dry_clean(const Params &P, StateW S, double *part, int blocks, hipStream_t st);
+hipError_t launch_tide_times(Ctl *ctl, hipStream_t st);
+// the per-constituent coefficient table of the equilibrium-tide body force
+// (2H + 2W doubles each; see Params::tide), SWE_EINVAL on bad arguments
+int tide_table(const Params &P, int n, const int32_t *m, const double *amp, const double *lat, const double *lon,
+               std::vector<double> &out);
 
 constexpr int MIN_CHUNK = 8;
 // device row arrays of a context: the divisors, their correctly rounded
@@ -44,6 +49,8 @@ struct swe_ctx {
   int max_parts = 0;
   int chunk = 0;
   double manning = 0.0, coriolis = 0.0;
+  double *tide = nullptr;      // equilibrium-tide coefficient table (tide_n constituents)
+  int tide_n = 0;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   hipGraph_t graph[2] = {nullptr, nullptr};
   bool use_graph = false;

@@ -182,6 +182,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
     return;
   }
   const bool posOn = dt > 0;
+  const double *tcs = &ctl->tide_cs[(MODE == MODE_STAGE2 || (MODE == MODE_PROBE && P.probe_stage2)) ? 1 : 0][0][0];
   const double *b = A.b;
   const HllcConst HC = {P.g, P.half_g, P.rg, P.min_depth};
   const double md = P.min_depth;
@@ -334,6 +335,15 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
         } else if (P.coriolis_on) {                       // swe.mjs:620-628
           rhu += P.coriolis * cur.hv;
           rhv -= P.coriolis * cur.hu;
+        }
+        // equilibrium tide: g h grad(eta_eq), eta_eq = sum A cos^2(phi) cos(w t + 2 lon + ph)
+        // (semidiurnal) or A sin(2 phi) cos(w t + lon + ph) (diurnal); no reference counterpart
+        for (int c = 0; c < P.tide_n; ++c) {
+          const double *T = P.tide + (size_t)c * (2 * H + 2 * W);
+          const double cl = T[2 * H + gi], sl = T[2 * H + W + gi];
+          const double tc = tcs[2 * c], ts = tcs[2 * c + 1];
+          rhu += cur.h * T[o] * (ts * cl + tc * sl);
+          rhv += cur.h * T[H + o] * (tc * cl - ts * sl);
         }
       }
       if (MODE == MODE_PROBE) {
@@ -553,6 +563,15 @@ hipError_t launch_dry_clean(const Params &P, StateW S, double *part, int blocks,
   return hipGetLastError();
 }
 
+__global__ void k_tide_times(Ctl *ctl) {
+  if (threadIdx.x == 0) tide_times(ctl);
+}
+
+hipError_t launch_tide_times(Ctl *ctl, hipStream_t st) {
+  hipLaunchKernelGGL(k_tide_times, dim3(1), dim3(64), 0, st, ctl);
+  return hipGetLastError();
+}
+
 __device__ __forceinline__ double harm_eval(int n, double c0, const double *amp, const double *om,
                                             const double *ph, double t) {
   double e = c0;
@@ -591,6 +610,7 @@ __device__ __forceinline__ void finalize_body(Ctl *ctl, const double *mf_part, i
   if (ctl->dt_mode == DT_SELF) ctl->dt = dtn;
   else if (ctl->dt_mode == DT_FIXED) ctl->dt = ctl->dt_fixed;
   else if (ctl->dt_mode == DT_UNTIL) ctl->dt = fmin(dtn, ctl->t_end - ctl->t);
+  tide_times(ctl);
   if (ctl->harm_on) {
     for (int side = 0; side < 4; ++side) {
       const Harm &h = ctl->harm[side];

@@ -17,7 +17,7 @@ namespace swe {
 enum BcKind { BC_REFLECT = 0, BC_OUTFLOW = 1, BC_PERIODIC = 2, BC_FLATHER = 3, BC_STORED = 4 };
 enum Mode { MODE_STAGE1 = 1, MODE_STAGE2 = 2, MODE_PROBE = 3 };
 enum DtMode { DT_SELF = 0, DT_FIXED = 1, DT_UNTIL = 2, DT_HOST = 3 };
-enum { MAX_HARM = 4 };
+enum { MAX_HARM = 4, MAX_TIDE = 4 };
 
 struct Geo {
   const double *dx, *rdx, *dyN, *rdyN, *dyS, *rdyS, *dyCFL, *rdyCFL, *bedPhi, *rbedPhi,
@@ -46,6 +46,12 @@ struct Ctl {
   int dt_mode, nonfinite, harm_on, pad;
   double dt_last;      // dt the last step ran with (0 if it was refused)
   Harm harm[4];
+  // equilibrium-tide constituents: angular frequency and phase, and
+  // (cos, sin)(omega t_s + phase) at the two stage times t_s = t, t + dt of the
+  // step about to run (written wherever dt is set on the device)
+  int tide_n, tide_pad;
+  double tide_om[MAX_TIDE], tide_ph[MAX_TIDE];
+  double tide_cs[2][MAX_TIDE][2];
 };
 
 struct Params {
@@ -57,6 +63,11 @@ struct Params {
   int want_theta, want_rh;
   double g, half_g, neg_half_g, rg, min_depth, eps4, cfl, coriolis, manning, R, sponge_ref;
   Geo geo;
+  // equilibrium-tide body force: tide_n constituents, each 2H + 2W doubles in
+  // `tide` (per row X, Y coefficients; per column cos, sin of m lon); a probe
+  // sweep evaluates it at the second stage time when probe_stage2 is set
+  int tide_n, probe_stage2;
+  const double *tide;
 };
 
 struct StateP { const double *h, *hu, *hv; };
@@ -90,6 +101,17 @@ struct BasinDev {
 // A unit of batched work: one wavefront's (strip, chunk) of one basin; `wave`
 // is its massFloored slot in that basin.
 struct Task { int basin, strip, chunk, wave; };
+
+// (cos, sin)(omega t_s + phase) of every tide constituent at the stage times
+// t_s = t, t + dt of the step about to run
+__device__ __forceinline__ void tide_times(Ctl *ctl) {
+  for (int c = 0; c < ctl->tide_n; ++c) {
+    const double a0 = ctl->tide_om[c] * ctl->t + ctl->tide_ph[c];
+    const double a1 = ctl->tide_om[c] * (ctl->t + ctl->dt) + ctl->tide_ph[c];
+    ctl->tide_cs[0][c][0] = cos(a0); ctl->tide_cs[0][c][1] = sin(a0);
+    ctl->tide_cs[1][c][0] = cos(a1); ctl->tide_cs[1][c][1] = sin(a1);
+  }
+}
 
 constexpr int SWEEP_LANES = 64;
 constexpr int SWEEP_HALO = 3;

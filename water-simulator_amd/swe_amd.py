@@ -78,6 +78,7 @@ def lib():
             "swe_last_advance_ms": (i, [vp, dp]),
             "swe_profile_step": (i, [vp, i, dp]),
             "swe_autotune": (i, [vp, i, ctypes.POINTER(i)]),
+            "swe_set_tide": (i, [vp, i, ctypes.POINTER(ctypes.c_int32), dp, dp, dp, dp, dp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -264,6 +265,11 @@ class Solver:
         self._ck(lib().swe_set_harmonic(self._h, side, len(amp), msl, _dp(A), _dp(O), _dp(P), len(uamp), u0,
                                         _dp(UA), _dp(UO), _dp(UP)), "swe_set_harmonic")
 
+    def set_tide(self, constituents, lat, lon):
+        """Equilibrium-tide body force (spherical grids): constituents = [(m, amp, omega, phase), ...],
+        m = 2 semidiurnal, 1 diurnal; lat (H) and lon (W) cell-centre angles in radians."""
+        self._ck(lib().swe_set_tide(self._h, *_tide_args(constituents, lat, lon)), "swe_set_tide")
+
     def profile_step(self, nsteps):
         """Mean kernel durations (ms) [stage 1, stage 2, finalize] over nsteps device steps."""
         out = np.zeros(3)
@@ -304,6 +310,43 @@ class Solver:
 # ---- geometry (src/geometry.mjs) -----------------------------------------------
 R_EARTH = 6371e3
 OMEGA_EARTH = 7.292115e-5
+
+
+# equilibrium-tide constituents: (m, angular frequency rad/s)
+TIDE_M2 = (2, 1.405189025e-4)
+TIDE_S2 = (2, 1.454441043e-4)
+TIDE_K1 = (1, 7.292115e-5)
+
+
+def _tide_args(constituents, lat, lon):
+    n = len(constituents)
+    m = (ctypes.c_int32 * max(n, 1))(*[int(c[0]) for c in constituents])
+    keep = [np.ascontiguousarray([float(c[k]) for c in constituents] or [0.0]) for k in (1, 2, 3)]
+    lat = np.ascontiguousarray(lat, dtype=np.float64)
+    lon = np.ascontiguousarray(lon, dtype=np.float64)
+    _tide_args.keep = (keep, lat, lon)
+    return n, m, _dp(keep[0]), _dp(keep[1]), _dp(keep[2]), _dp(lat), _dp(lon)
+
+
+def sphere_lonlat(nx, ny, lat0=-90.0, lat1=90.0, ng=2):
+    """Cell-centre latitudes (H rows) and longitudes (W columns) in radians, as
+    cellLonLat (swe.mjs:262-272, geometry.mjs:314) places them: lon = (i + 0.5) dlam - pi,
+    lat = the centre of the row's (clamped) faces."""
+    H, W = ny + 2 * ng, nx + 2 * ng
+    dlam = 2 * math.pi / nx
+    p0, p1 = lat0 * math.pi / 180, lat1 * math.pi / 180
+    dphi = (p1 - p0) / ny
+    half_pi = math.pi / 2
+    lat = np.empty(H)
+    for j in range(H):
+        jj = j - ng
+        pS = p0 if jj == 0 else p1 if jj == ny else p0 + jj * dphi
+        pN = p0 if jj + 1 == 0 else p1 if jj + 1 == ny else p0 + (jj + 1) * dphi
+        pS = min(half_pi, max(-half_pi, pS))
+        pN = min(half_pi, max(-half_pi, pN))
+        lat[j] = 0.5 * (pS + pN)
+    lon = (np.arange(W) - ng + 0.5) * dlam - math.pi
+    return lat, lon
 
 
 def geometry_cartesian(nx, ny, dx, dy=None, coriolis=0.0, ng=2):
@@ -438,6 +481,7 @@ class Batch:
             "swe_batch_advance": (i, [vp, i64, ctypes.POINTER(i64)]),
             "swe_batch_last_advance_ms": (i, [vp, dp]),
             "swe_batch_autotune": (i, [vp, i, ctypes.POINTER(i), ctypes.POINTER(i)]),
+            "swe_batch_set_tide": (i, [vp, i, i, ctypes.POINTER(ctypes.c_int32), dp, dp, dp, dp, dp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -495,6 +539,9 @@ class Batch:
         A, O, P, UA, UO, UP = (arr(x) for x in (amp, omega, phase, uamp, uomega, uphase))
         self._ck(lib().swe_batch_set_harmonic(self._h, basin, side, len(amp), msl, _dp(A), _dp(O), _dp(P),
                                               len(uamp), u0, _dp(UA), _dp(UO), _dp(UP)), "swe_batch_set_harmonic")
+
+    def set_tide(self, basin, constituents, lat, lon):
+        self._ck(lib().swe_batch_set_tide(self._h, basin, *_tide_args(constituents, lat, lon)), "swe_batch_set_tide")
 
     def get_scalar(self, basin, which):
         v = ctypes.c_double()

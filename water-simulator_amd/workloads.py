@@ -177,3 +177,36 @@ def inversion(nx=1024, ny=1024, dx=50.0, seed=4, ngauges=32):
     params = dict(nx=nx, ny=ny, dx=dx, manning=0.025, coriolis=1e-4, cfl=0.45,
                   bc=("outflow", "reflect", "reflect", "reflect"), nsteps=4000, every=10)
     return b.ravel(), h.ravel(), params, np.array(gauges, dtype=np.int32)
+
+
+def sphere_tides(nx=2160, ny=1080, members=16, seed=5):
+    """Global ocean on the 10-arc-minute lat-lon sphere (2160x1080, R = 6371 km,
+    Omega = 7.292115e-5 s^-1, the globe's metric rows): bathymetry from spectral
+    noise with k^-2 amplitude (seeded normals shaped in Fourier space), scaled
+    linearly to -6000 .. +2000 m, land dry; an ocean at rest (eta0 = 0).
+    Equilibrium tide M2 0.5 m, S2 0.2 m, K1 0.3 m.  Members differ only in their
+    Manning n, U(0.015, 0.035).  Returns (b, h, params, manning per member)."""
+    rng = SplitMix64(seed)
+    W, H = nx + 4, ny + 4
+    wn = rng.normals(ny * nx).reshape(ny, nx)
+    F = np.fft.rfft2(wn)
+    ky = np.fft.fftfreq(ny)[:, None] * ny
+    kx = np.fft.rfftfreq(nx)[None, :] * nx
+    k = np.sqrt(kx * kx + ky * ky)
+    k[0, 0] = 1.0
+    F *= k ** -2.0
+    F[0, 0] = 0.0
+    z = np.fft.irfft2(F, s=(ny, nx))
+    z = -6000.0 + 8000.0 * (z - z.min()) / (z.max() - z.min())
+    b = np.empty((H, W))
+    b[2:-2, 2:-2] = z
+    b[2:-2, :2] = z[:, -2:]                  # periodic seam
+    b[2:-2, -2:] = z[:, :2]
+    b[:2] = b[3:1:-1]                        # mirrored beyond the poles
+    b[-2:] = b[-3:-5:-1]
+    h = np.maximum(0.0, -b)
+    manning = [rng.uniform(0.015, 0.035) for _ in range(members)]
+    params = dict(nx=nx, ny=ny, R=6371e3, omega=7.292115e-5, lat0=-90.0, lat1=90.0, cfl=0.45,
+                  tides=[(2, 0.5, 1.405189025e-4, 0.0), (2, 0.2, 1.454441043e-4, 0.0), (1, 0.3, 7.292115e-5, 0.0)],
+                  days=5.0)
+    return b.ravel(), h.ravel(), params, manning

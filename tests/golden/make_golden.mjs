@@ -10,8 +10,9 @@
 // 16-hex-digit IEEE-754 bit patterns and arrays as base64 of little-endian
 // Float64Array bytes, so nothing is lost to decimal printing.
 //
-// This script is the only place that executes the reference, and only in the build
-// container; nothing that runs on a GPU machine reads /root/reference.
+// This script and oracle/time_reference.mjs (the CPU timing of the reference) are
+// the only places that execute the reference, and only in the build container;
+// nothing that runs on a GPU machine reads /root/reference.
 import fs from 'fs';
 import zlib from 'zlib';
 import path from 'path';

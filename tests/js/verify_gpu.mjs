@@ -1,0 +1,312 @@
+// Physics checks of the GPU backend through the Node drop-in, restating the
+// kinds of test the reference's suites make (tools/verify.mjs,
+// tools/verify-physics.mjs, tools/verify-tide.mjs) with this file's own closed
+// forms and set-ups: every target is computed here from G_REF and the problem's
+// constants, never from the solver under test.
+//
+//   node tests/js/verify_gpu.mjs        (needs the addon and a GPU)
+//
+// Exit status 0 when every check passes.
+import { ShallowWater, G, reflect, outflow, periodic, flather } from '../../water-simulator_amd/js/swe.mjs';
+
+const G_REF = 9.80665;   // standard gravity, exact by definition
+let fails = 0, checks = 0;
+function report(label, ok, note) {
+  checks++;
+  if (!ok) fails++;
+  console.log(`${ok ? 'PASS' : 'FAIL'}  ${label}${note ? '  -- ' + note : ''}`);
+}
+const rel = (got, want) => Math.abs(got - want) / Math.abs(want);
+const fmt = (v) => (Math.abs(v) < 1e-3 || Math.abs(v) >= 1e4 ? v.toExponential(3) : v.toFixed(4));
+// xorshift32 stream for seeded beds
+function xorshift(seed) {
+  let s = seed >>> 0 || 1;
+  return () => { s ^= s << 13; s >>>= 0; s ^= s >>> 17; s ^= s << 5; s >>>= 0; return s / 4294967296; };
+}
+// run to exactly T with dt = min(maxDt, T - t), the reference suites' stepping
+function runTo(sim, T, each = null) {
+  while (sim.t < T) {
+    const dt = Math.min(sim.maxDt(), T - sim.t);
+    if (!(dt > 0)) break;
+    sim.step(dt);
+    if (each) each(sim);
+  }
+}
+// row j of the interior as plain arrays
+function rowOf(sim, f, j) {
+  const out = new Float64Array(sim.nx);
+  for (let i = 0; i < sim.nx; i++) out[i] = sim[f][sim.idx(i, j)];
+  return out;
+}
+// sub-cell position of a maximum by a parabola through the three top samples
+function peakPos(a, dx, from = 0) {
+  let m = from;
+  for (let i = from; i < a.length; i++) if (a[i] > a[m]) m = i;
+  const l = a[Math.max(m - 1, 0)], c = a[m], r = a[Math.min(m + 1, a.length - 1)];
+  const den = l - 2 * c + r;
+  const off = den !== 0 ? 0.5 * (l - r) / den : 0;
+  return (m + 0.5 + off) * dx;
+}
+// times at which x(t) crosses its mean upward (linear interpolation)
+function upCrossings(ts, xs) {
+  const mean = xs.reduce((s, v) => s + v, 0) / xs.length;
+  const out = [];
+  for (let k = 1; k < xs.length; k++) {
+    const a = xs[k - 1] - mean, b = xs[k] - mean;
+    if (a < 0 && b >= 0) out.push(ts[k - 1] + (ts[k] - ts[k - 1]) * (-a) / (b - a));
+  }
+  return out;
+}
+
+report('gravity: the drop-in exports the reference constant', G === G_REF, `G = ${G}`);
+
+// ---- 1. lake at rest over four beds (well-balancing, wet/dry island guard) -----------
+{
+  const ETA = 1.0, L = 480;
+  const beds = {
+    flat: () => -4,
+    bump: (x, y) => -4 + 3 * Math.exp(-((x - L / 2) ** 2 + (y - L / 2) ** 2) / 80 ** 2),
+    island: (x, y) => -3 + 6 * Math.exp(-((x - 0.4 * L) ** 2 + (y - 0.6 * L) ** 2) / 60 ** 2),
+    rough: (() => { const r = xorshift(7); const z = new Map(); return (x, y) => { const k = `${x},${y}`; if (!z.has(k)) z.set(k, -4 + 2 * r()); return z.get(k); }; })(),
+  };
+  for (const [name, bed] of Object.entries(beds)) {
+    const sim = new ShallowWater({ nx: 48, ny: 48, dx: 10, bed, eta0: ETA, manning: 0.025 });
+    const v0 = sim.volume();
+    sim.advance(400);
+    const dev = sim.maxSurfaceDeviation(ETA), spd = sim.maxSpeed(), dV = Math.abs(sim.volume() - v0) / v0;
+    report(`lake at rest, ${name}: surface`, dev < 1e-12, `max |eta - ${ETA}| = ${fmt(dev)} m over 400 steps`);
+    report(`lake at rest, ${name}: velocity`, spd < 1e-12, `max speed ${fmt(spd)} m/s`);
+    report(`lake at rest, ${name}: volume`, dV < 1e-13, `relative drift ${fmt(dV)}`);
+  }
+}
+
+// ---- 2. closed basin: exact mass conservation with waves and wet/dry --------------------
+{
+  const r = xorshift(11);
+  const sim = new ShallowWater({
+    nx: 64, ny: 48, dx: 5, manning: 0.025,
+    bed: (x, y) => -2 + 3 * x / 320 + 0.2 * r(),
+    eta0: (x, y) => 0.5 * Math.exp(-((x - 80) ** 2 + (y - 120) ** 2) / 30 ** 2),
+  });
+  const v0 = sim.volume();
+  let floored = 0;
+  for (let n = 0; n < 600; n++) { sim.step(); floored += sim.massFloored; }
+  const drift = Math.abs(sim.volume() - v0) / v0;
+  report('closed basin: volume drift over 600 steps', drift < 1e-12, `${fmt(drift)} (massFloored total ${floored})`);
+  report('closed basin: finite', sim.finite());
+}
+
+// ---- 3. Ritter dam break onto a dry bed ---------------------------------------------------
+{
+  const h0 = 1, L = 200, x0 = 100, T = 10;
+  const c0 = Math.sqrt(G_REF * h0);
+  const exactH = (x) => {
+    const xi = (x - x0) / T;
+    if (xi <= -c0) return h0;
+    if (xi >= 2 * c0) return 0;
+    return (2 * c0 - xi) ** 2 / (9 * G_REF);
+  };
+  const errs = [], deep = [];
+  let contourErr = 0;
+  for (const nx of [400, 800, 1600]) {
+    const dx = L / nx;
+    const sim = new ShallowWater({ nx, ny: 4, dx, manning: 0, bed: () => 0, eta0: (x) => (x < x0 ? h0 : 0) });
+    runTo(sim, T);
+    const h = rowOf(sim, 'h', 1);
+    let e = 0, ed = 0, nd = 0;
+    for (let i = 0; i < nx; i++) {
+      const he = exactH((i + 0.5) * dx), d = Math.abs(h[i] - he);
+      e += d;
+      if (he > 0.1 * h0) { ed += d; nd++; }
+    }
+    errs.push(e / nx);
+    deep.push(ed / nd);
+    if (nx === 1600) {
+      for (const lvl of [0.8, 0.5, 0.2]) {
+        // exact contour: x0 + T (2 c0 - 3 sqrt(g lvl)); measured: first crossing downstream of the dam
+        const want = x0 + T * (2 * c0 - 3 * Math.sqrt(G_REF * lvl));
+        let got = NaN;
+        for (let i = 1; i < nx; i++) {
+          if (h[i - 1] >= lvl && h[i] < lvl) { got = (i - 0.5 + (h[i - 1] - lvl) / (h[i - 1] - h[i])) * dx; break; }
+        }
+        const r2 = Math.abs((got - x0) - (want - x0)) / Math.abs(want - x0);
+        contourErr = Math.max(contourErr, r2);
+        report(`Ritter: h = ${lvl} m contour`, r2 < 0.02, `${fmt(got)} m vs ${fmt(want)} m`);
+      }
+    }
+  }
+  // the smeared dry-bed front dominates the whole-channel error and converges
+  // slowly; where the exact depth exceeds h0/10 the scheme converges at first order
+  const p = Math.log(deep[0] / deep[2]) / Math.log(4);
+  report('Ritter: mean |h - exact| at nx = 1600', errs[2] < 0.004 * h0, `${fmt(errs[2])} m`);
+  report('Ritter: the error falls under refinement', errs[2] < errs[1] && errs[1] < errs[0],
+    `L1 ${errs.map(fmt).join(' -> ')}`);
+  report('Ritter: first-order convergence where h > h0/10', deep[2] < deep[1] && deep[1] < deep[0] && p > 0.8,
+    `L1 ${deep.map(fmt).join(' -> ')}, order ${fmt(p)}`);
+}
+
+// ---- 4. linear celerity sqrt(g h) ----------------------------------------------------------
+{
+  const speeds = [];
+  for (const h0 of [2, 10, 50]) {
+    const c0 = Math.sqrt(G_REF * h0);
+    const dx = h0 / 2, nx = 1600, x0 = 0.25 * nx * dx, sig = 10 * dx, a = 1e-4 * h0;
+    const sim = new ShallowWater({ nx, ny: 4, dx, manning: 0, bed: () => -h0,
+      eta0: (x) => a * Math.exp(-(((x - x0) / sig) ** 2)) });
+    // a purely right-going pulse: hu = c eta
+    for (let k = 0; k < sim.h.length; k++) sim.hu[k] = c0 * (sim.h[k] - h0);
+    const t1 = 0.25 * nx * dx / c0;
+    runTo(sim, 0.2 * t1);
+    const tA = sim.t, xA = peakPos(rowOf(sim, 'h', 1), dx);
+    runTo(sim, t1);
+    const c = (peakPos(rowOf(sim, 'h', 1), dx) - xA) / (sim.t - tA);
+    speeds.push(c);
+    report(`celerity at h = ${h0} m`, rel(c, c0) < 0.01, `${fmt(c)} vs sqrt(g h) = ${fmt(c0)} m/s`);
+  }
+  report('celerity: the measurement tells depths apart', speeds[1] / speeds[0] > 2 && speeds[2] / speeds[1] > 2);
+}
+
+// ---- 5. Thacker's radially symmetric bowl (closed form, wetting and drying) ----------------
+{
+  // dx = diameter / 200 as in Thacker's test; the shoreline swings out to
+  // r = a (q / sqrt(1 - A^2))^(1/2) <= 1.32 a, so the domain is 280 cells wide
+  const a = 1000, h0 = 1, A = 0.5, dx = 2 * a / 200, n = 280, c = n * dx / 2;
+  const omega = Math.sqrt(8 * G_REF * h0) / a, period = 2 * Math.PI / omega;
+  const surf = (x, y, t) => {
+    const q = 1 - A * Math.cos(omega * t), r2 = (x - c) ** 2 + (y - c) ** 2;
+    return h0 + h0 * (Math.sqrt(1 - A * A) / q - 1 - (r2 / (a * a)) * ((1 - A * A) / (q * q) - 1));
+  };
+  const bed = (x, y) => h0 * ((x - c) ** 2 + (y - c) ** 2) / (a * a);
+  const sim = new ShallowWater({ nx: n, ny: n, dx, manning: 0, bed, eta0: (x, y) => surf(x, y, 0) });
+  const kc = sim.idx(n / 2, n / 2);
+  const ts = [], hs = [];
+  runTo(sim, 2 * period, (s) => { ts.push(s.t); hs.push(s.h[kc]); });
+  let l1 = 0, cnt = 0;
+  for (let j = 0; j < n; j++) {
+    for (let i = 0; i < n; i++) {
+      const x = (i + 0.5) * dx, y = (j + 0.5) * dx;
+      const he = Math.max(0, surf(x, y, sim.t) - bed(x, y));
+      if (he > 0.05) { l1 += Math.abs(sim.h[sim.idx(i, j)] - he); cnt++; }
+    }
+  }
+  report('Thacker: mean |h - exact| after 2 periods', l1 / cnt < 0.02 * h0, `${fmt(l1 / cnt)} m over ${cnt} wet cells`);
+  const zc = upCrossings(ts, hs);
+  const measured = (zc[zc.length - 1] - zc[0]) / (zc.length - 1);
+  report('Thacker: at least two oscillations observed', zc.length >= 2, `${zc.length} up-crossings`);
+  report('Thacker: oscillation period', rel(measured, period) < 0.01, `${fmt(measured)} vs ${fmt(period)} s`);
+}
+
+// ---- 6. Merian seiche in a closed basin -------------------------------------------------------
+{
+  const h0 = 10, L = 1000, nx = 200, dx = L / nx, amp = 0.01;
+  const want = 2 * L / Math.sqrt(G_REF * h0);
+  const sim = new ShallowWater({ nx, ny: 4, dx, manning: 0, bed: () => -h0,
+    eta0: (x) => amp * Math.cos(Math.PI * x / L) });
+  const k = sim.idx(0, 1);
+  const ts = [], es = [];
+  runTo(sim, 7.5 * want, (s) => { ts.push(s.t); es.push(s.b[k] + s.h[k]); });
+  const zc = upCrossings(ts, es);
+  const measured = (zc[zc.length - 1] - zc[0]) / (zc.length - 1);
+  report('seiche: at least six periods observed', zc.length >= 6, `${zc.length} up-crossings`);
+  report('seiche period (Merian)', rel(measured, want) < 0.01, `${fmt(measured)} vs 2L/sqrt(gh) = ${fmt(want)} s`);
+}
+
+// ---- 7. positivity limiter at a wet/dry face -------------------------------------------------
+{
+  // a resting column meeting a dry bed: the only mass flux is the dry-face flux
+  // m = 2 c h / 3 (HLLC with a dry right state), so the last wet cell would hand
+  // out h dx / (m dt) of its content; at dt = 9 dx / (4 c) that is 3/2 of it and
+  // the limiter scales its outflow by theta = 2/3
+  const h0 = 2, dx = 1, nx = 20;
+  const sim = new ShallowWater({ nx, ny: 4, dx, manning: 0, bed: () => 0, eta0: (x) => (x < 10 ? h0 : 0) });
+  const cc = Math.sqrt(G_REF * h0);
+  sim.applyBC();
+  sim.residual(9 * dx / (4 * cc));
+  const th = sim.theta[sim.idx(9, 1)];
+  const ulps = Math.abs(th - 2 / 3) / (2 ** -53);
+  report('positivity: theta at the wet/dry face = 2/3', ulps <= 4, `${th} (${fmt(ulps)} ulp of 2/3)`);
+  let others = true;
+  for (let i = 0; i < nx; i++) if (i !== 9 && sim.theta[sim.idx(i, 1)] !== 1) others = false;
+  report('positivity: every other cell keeps theta = 1', others);
+}
+
+// ---- 8. Manning spin-down: the semi-implicit update is exact in 1/u ------------------------
+{
+  // uniform flow in a periodic channel: each step divides hu by 1 + dt g n^2 u / h^(4/3),
+  // so 1/u grows by exactly dt g n^2 / h^(4/3) per step
+  const h0 = 2, u0 = 1.5, n = 0.03;
+  const sim = new ShallowWater({ nx: 8, ny: 8, dx: 10, manning: n, bed: () => -h0, eta0: 0 });
+  for (const s of ['west', 'east', 'south', 'north']) sim.boundaries[s] = periodic;
+  for (let k = 0; k < sim.hu.length; k++) sim.hu[k] = h0 * u0;
+  const A = G_REF * n * n / Math.pow(h0, 4 / 3);
+  for (let s = 0; s < 200; s++) sim.step(0.5);
+  const u = sim.hu[sim.idx(3, 3)] / h0;
+  const lhs = 1 / u - 1 / u0, rhs = A * sim.t;
+  report('Manning spin-down: 1/u - 1/u0 = g n^2 t / h^(4/3)', rel(lhs, rhs) < 1e-10, `${fmt(lhs)} vs ${fmt(rhs)} (rel ${fmt(rel(lhs, rhs))})`);
+}
+
+// ---- 9. Coriolis: the SSP-RK2 rotation map, exactly ---------------------------------------------
+{
+  // uniform current in a periodic box: per step Heun's method maps (hu, hv) by
+  // [[1 - q^2/2, q], [-q, 1 - q^2/2]] with q = f dt, a rotation by atan2(q, 1 - q^2/2)
+  // and a growth of sqrt(1 + q^4/4) -- (f dt)^2/6 phase and (f dt)^4/8 amplitude error
+  const f = 1e-3, dt = 40, N = 300, h0 = 5;
+  const sim = new ShallowWater({ nx: 8, ny: 8, dx: 100, manning: 0, coriolis: f, bed: () => -h0, eta0: 0 });
+  for (const s of ['west', 'east', 'south', 'north']) sim.boundaries[s] = periodic;
+  for (let k = 0; k < sim.hu.length; k++) sim.hu[k] = h0 * 0.3;
+  for (let s = 0; s < N; s++) sim.step(dt);
+  const q = f * dt;
+  const ang = N * Math.atan2(q, 1 - q * q / 2), gain = Math.pow(1 + q ** 4 / 4, N / 2);
+  const wantU = h0 * 0.3 * gain * Math.cos(ang), wantV = -h0 * 0.3 * gain * Math.sin(ang);
+  const k = sim.idx(4, 4);
+  const err = Math.hypot(sim.hu[k] - wantU, sim.hv[k] - wantV) / (h0 * 0.3 * gain);
+  report('Coriolis: SSP-RK2 rotation map after 300 steps', err < 1e-12, `relative error ${fmt(err)}`);
+  const exactAng = N * q;
+  report('Coriolis: the map, not the exact rotation, is what the solver integrates',
+    Math.abs(ang - exactAng) > 1e3 * err, `phase lead ${fmt(ang - exactAng)} rad`);
+}
+
+// ---- 10. CFL: maxDt bit for bit against its closed form --------------------------------------
+{
+  const h0 = 3, dx = 7, dy = 5, U = 0.8, V = -0.3;
+  const sim = new ShallowWater({ nx: 12, ny: 9, dx, dy, manning: 0, bed: () => -h0, eta0: 0, cfl: 0.45 });
+  for (let k = 0; k < sim.hu.length; k++) { sim.hu[k] = h0 * U; sim.hv[k] = h0 * V; }
+  const u = Math.abs(sim.vel(h0 * U, h0)), v = Math.abs(sim.vel(h0 * V, h0)), c = Math.sqrt(G_REF * h0);
+  const want = 0.45 / ((u + c) / dx + (v + c) / dy);
+  const got = sim.maxDt();
+  report('CFL: maxDt equals cfl / ((|u| + c)/dx + (|v| + c)/dy) exactly', Object.is(got, want), `${got} vs ${want}`);
+}
+
+// ---- 11. run-up on a dry beach: no mass invented, none lost ------------------------------------
+{
+  const sim = new ShallowWater({ nx: 120, ny: 8, dx: 2, manning: 0.02,
+    bed: (x) => -1 + 2 * x / 240, eta0: (x) => (x < 40 ? 0.3 : 0) });
+  const v0 = sim.volume();
+  let floored = 0;
+  for (let n = 0; n < 400; n++) { sim.step(); floored += sim.massFloored; }
+  const drift = Math.abs(sim.volume() - v0) / v0;
+  report('run-up: volume conserved over a moving shoreline', drift < 1e-12, `drift ${fmt(drift)}`);
+  report('run-up: massFloored stays zero at CFL 0.45', floored === 0, `${floored}`);
+}
+
+// ---- 12. Flather: an outgoing pulse leaves; a wall sends it back ---------------------------------
+{
+  const h0 = 10, dx = 10, nx = 400, a = 0.05, c0 = Math.sqrt(G_REF * h0);
+  const left = (bc) => {
+    const sim = new ShallowWater({ nx, ny: 4, dx, manning: 0, bed: () => -h0,
+      eta0: (x) => a * Math.exp(-(((x - 0.5 * nx * dx) / (15 * dx)) ** 2)) });
+    for (let k = 0; k < sim.h.length; k++) sim.hu[k] = c0 * (sim.h[k] - h0);    // right-going
+    sim.boundaries.east = bc;
+    // long enough to reach the boundary and come back to the middle
+    runTo(sim, 0.5 * nx * dx / c0 + 0.5 * nx * dx / c0 + 0.15 * nx * dx / c0);
+    let m = 0;
+    for (let i = 0; i < nx; i++) m = Math.max(m, Math.abs(sim.b[sim.idx(i, 1)] + sim.h[sim.idx(i, 1)]));
+    return m / a;
+  };
+  const rFl = left(flather(() => 0)), rWall = left(reflect), rOut = left(outflow);
+  report('Flather: outgoing pulse reflection below 2%', rFl < 0.02, `${fmt(100 * rFl)}% of the amplitude`);
+  report('Flather: the wall control reflects above 90%', rWall > 0.9, `${fmt(100 * rWall)}% (zero-gradient outflow ${fmt(100 * rOut)}%)`);
+}
+
+console.log(`\n${fails === 0 ? 'ALL PASS' : 'FAILURES'} ${checks - fails}/${checks}`);
+process.exit(fails === 0 ? 0 : 1);

@@ -37,3 +37,17 @@ def test_js_host_pieces():
 @pytest.mark.gpu
 def test_js_golden_replay_on_gpu():
     assert run_node("tests/js/golden_replay.mjs") == 0
+
+
+@needs_node
+@pytest.mark.gpu
+def test_js_restated_verification_suites_on_gpu():
+    """Lake at rest, mass, Ritter, celerity, Thacker, Merian, positivity, Manning
+    spin-down, Coriolis map, CFL, run-up and Flather, through the drop-in."""
+    assert run_node("tests/js/verify_gpu.mjs", timeout=900) == 0
+
+
+@needs_node
+@pytest.mark.gpu
+def test_js_adjoint_and_tide_on_gpu():
+    assert run_node("tests/js/adjoint_gpu.mjs") == 0

@@ -257,6 +257,24 @@ export class ShallowWater {
     return taken;
   }
 
+  /**
+   * Equilibrium-tide body force (no counterpart in the reference, which leaves
+   * such forcing to `sim.forcing`): constituents [{m, amp, omega, phase}], m = 2
+   * semidiurnal (amp cos^2(lat) cos(omega t + 2 lon + phase)), m = 1 diurnal
+   * (amp sin(2 lat) cos(omega t + lon + phase)); both RK stages add
+   * g h grad(eta_eq) on the device.  Spherical grids only; [] removes it.
+   */
+  setTide(constituents) {
+    const g = this.geom;
+    if (g.kind !== 'sphere') throw new Error('setTide: the equilibrium tide needs a spherical grid');
+    const lon = new Float64Array(this.W);
+    for (let i = 0; i < this.W; i++) lon[i] = (i - this.ng + 0.5) * g.dlam - Math.PI;
+    const pick = (k) => Float64Array.from(constituents.map((c) => c[k]));
+    addon.setTide(this._dev, Int32Array.from(constituents.map((c) => c.m)), pick('amp'), pick('omega'),
+      pick('phase'), Float64Array.from(g.phiC), lon);
+    return this;
+  }
+
   // ---- diagnostics (host, over the coherent mirror) ---------------------------
   volume() {
     const { W, ng, nx, ny } = this;
@@ -329,6 +347,74 @@ export class ShallowWater {
 }
 
 /** HLLC interface flux with Einfeldt speeds and dry states: the kernels' own code. */
+/**
+ * Discrete adjoint of `nsteps` self-timed steps of `sim` (no counterpart in the
+ * reference): the forward/backward pair of a differentiable function, the JS
+ * analogue of a torch.autograd.Function.
+ *
+ *   const adj = new Adjoint(sim, 4000);
+ *   adj.setGauges(cells, 10, obs);        // obs: (nsteps/10) x cells.length
+ *   const J = adj.forward(h0, nField);    // runs the steps, records dt, checkpoints
+ *   const { dh0, dn } = adj.backward();   // dJ/dh0, dJ/dn over W*H cells
+ *
+ * J = sum over samples and gauges of (eta - obs)^2.  The dt sequence is the
+ * forward's (held fixed); branches take the subgradient the forward took.
+ * Boundaries reflect/outflow/periodic; no sponge.  The adjoint drives the
+ * solver's device state, so `sim.h` etc. are re-uploaded by its next call.
+ */
+export class Adjoint {
+  constructor(sim, nsteps, { checkpointEvery = 0 } = {}) {
+    this.sim = sim;
+    this.nsteps = nsteps;
+    this.ng = 0;
+    this.every = 1;
+    this._prepare();
+    this._adj = addon.adjCreate(sim._dev, nsteps, checkpointEvery);
+  }
+
+  _prepare() {
+    const sim = this.sim;
+    sim._boundaries();
+    sim._syncParams();
+    sim._upload(['B']);
+  }
+
+  setGauges(cells, every, obs) {
+    this.ng = cells.length;
+    this.every = every;
+    addon.adjSetGauges(this._adj, Int32Array.from(cells), every, Float64Array.from(obs));
+    return this;
+  }
+
+  forward(h0, manningField = null) {
+    this._prepare();
+    return addon.adjForward(this._adj, Float64Array.from(h0), manningField ? Float64Array.from(manningField) : null);
+  }
+
+  /** J of the forward that replays the recorded dt sequence (for finite-difference checks). */
+  loss(h0, manningField = null) {
+    return addon.adjLoss(this._adj, Float64Array.from(h0), manningField ? Float64Array.from(manningField) : null);
+  }
+
+  backward() {
+    const N = this.sim.W * this.sim.H;
+    const dh0 = new Float64Array(N), dn = new Float64Array(N);
+    addon.adjBackward(this._adj, dh0, dn);
+    return { dh0, dn };
+  }
+
+  series() {
+    const out = new Float64Array(Math.floor(this.nsteps / this.every) * this.ng);
+    addon.adjSeries(this._adj, out);
+    return out;
+  }
+
+  timing() {
+    const [forwardMs, backwardMs] = addon.adjTiming(this._adj);
+    return { forwardMs, backwardMs };
+  }
+}
+
 export function hllc(out, hL, huL, hvL, hR, huR, hvR, dry = 1e-4) {
   const f = addon.hllc(hL, huL, hvL, hR, huR, hvR, dry, G);
   out[0] = f[0]; out[1] = f[1]; out[2] = f[2];

@@ -295,6 +295,149 @@ napi_value JsPow(napi_env env, napi_callback_info info) {
   return num(env, swe_js_pow(get_double(env, argv[0]), get_double(env, argv[1])));
 }
 
+// ---- equilibrium tide and the discrete adjoint ----------------------------------
+bool get_i32(napi_env env, napi_value v, int32_t **data, size_t *len) {
+  bool is_ta = false;
+  napi_is_typedarray(env, v, &is_ta);
+  if (!is_ta) return false;
+  napi_typedarray_type tt;
+  void *p = nullptr;
+  napi_value ab;
+  size_t off = 0;
+  if (napi_get_typedarray_info(env, v, &tt, len, &p, &ab, &off) != napi_ok || tt != napi_int32_array) return false;
+  *data = static_cast<int32_t *>(p);
+  return true;
+}
+
+// setTide(h, Int32Array m, Float64Array amp, omega, phase, lat(H), lon(W))
+napi_value SetTide(napi_env env, napi_callback_info info) {
+  napi_value argv[7];
+  if (!get_args(env, info, 7, argv)) return nullptr;
+  swe_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  int32_t *m = nullptr;
+  size_t n = 0, k;
+  double *a[5];
+  if (!get_i32(env, argv[1], &m, &n)) { napi_throw_type_error(env, nullptr, "setTide: m Int32Array"); return nullptr; }
+  for (int q = 0; q < 5; ++q)
+    if (!get_f64(env, argv[2 + q], &a[q], &k)) { napi_throw_type_error(env, nullptr, "setTide: Float64Array"); return nullptr; }
+  if (swe_set_tide(c, (int)n, m, a[0], a[1], a[2], a[3], a[4]) != SWE_OK) return throw_swe(env, c, "swe_set_tide");
+  return nullptr;
+}
+
+swe_adj *get_adj(napi_env env, napi_value v) {
+  void *p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, nullptr, "swe_napi: expected an adjoint handle");
+    return nullptr;
+  }
+  return static_cast<swe_adj *>(p);
+}
+
+napi_value throw_adj(napi_env env, swe_adj *a, const char *what) {
+  std::string m = std::string(what) + ": " + swe_adj_last_error(a);
+  napi_throw_error(env, nullptr, m.c_str());
+  return nullptr;
+}
+
+void finalize_adj(napi_env, void *data, void *) { swe_adj_destroy(static_cast<swe_adj *>(data)); }
+
+// adjCreate(h, nsteps, checkpointEvery) -> adjoint handle
+napi_value AdjCreate(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  swe_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  swe_adj *a = nullptr;
+  if (swe_adj_create(c, (int64_t)get_double(env, argv[1]), get_int(env, argv[2]), &a) != SWE_OK)
+    return throw_swe(env, c, "swe_adj_create");
+  napi_value ext;
+  NAPI_OK(env, napi_create_external(env, a, finalize_adj, nullptr, &ext));
+  return ext;
+}
+
+// adjSetGauges(a, Int32Array cells, every, Float64Array obs)
+napi_value AdjSetGauges(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  if (!get_args(env, info, 4, argv)) return nullptr;
+  swe_adj *a = get_adj(env, argv[0]);
+  if (!a) return nullptr;
+  int32_t *cells = nullptr;
+  size_t n = 0, no = 0;
+  double *obs = nullptr;
+  if (!get_i32(env, argv[1], &cells, &n) || !get_f64(env, argv[3], &obs, &no)) {
+    napi_throw_type_error(env, nullptr, "adjSetGauges: Int32Array cells, Float64Array obs");
+    return nullptr;
+  }
+  if (swe_adj_set_gauges(a, (int)n, cells, get_int(env, argv[2]), obs) != SWE_OK)
+    return throw_adj(env, a, "swe_adj_set_gauges");
+  return nullptr;
+}
+
+// adjForward(a, h0, nfield|null) -> J ; adjLoss likewise (replays the recorded dt)
+napi_value adj_run(napi_env env, napi_callback_info info, bool record) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  swe_adj *a = get_adj(env, argv[0]);
+  if (!a) return nullptr;
+  double *h0 = nullptr, *nf = nullptr;
+  size_t n1 = 0, n2 = 0;
+  if (!get_f64(env, argv[1], &h0, &n1) || !h0 || !get_f64(env, argv[2], &nf, &n2)) {
+    napi_throw_type_error(env, nullptr, "adjoint: Float64Array h0, Float64Array|null n");
+    return nullptr;
+  }
+  double J = 0;
+  const int rc = record ? swe_adj_forward(a, h0, nf, &J) : swe_adj_loss(a, h0, nf, &J);
+  if (rc != SWE_OK) return throw_adj(env, a, record ? "swe_adj_forward" : "swe_adj_loss");
+  return num(env, J);
+}
+napi_value AdjForward(napi_env env, napi_callback_info info) { return adj_run(env, info, true); }
+napi_value AdjLoss(napi_env env, napi_callback_info info) { return adj_run(env, info, false); }
+
+// adjBackward(a, Float64Array gradH0|null, Float64Array gradN|null)
+napi_value AdjBackward(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  swe_adj *a = get_adj(env, argv[0]);
+  if (!a) return nullptr;
+  double *gh = nullptr, *gn = nullptr;
+  size_t n1 = 0, n2 = 0;
+  if (!get_f64(env, argv[1], &gh, &n1) || !get_f64(env, argv[2], &gn, &n2)) {
+    napi_throw_type_error(env, nullptr, "adjBackward: Float64Array or null");
+    return nullptr;
+  }
+  if (swe_adj_backward(a, gh, gn) != SWE_OK) return throw_adj(env, a, "swe_adj_backward");
+  return nullptr;
+}
+
+// adjSeries(a, Float64Array out)
+napi_value AdjSeries(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  swe_adj *a = get_adj(env, argv[0]);
+  if (!a) return nullptr;
+  double *p = nullptr;
+  size_t n = 0;
+  if (!get_f64(env, argv[1], &p, &n) || !p) { napi_throw_type_error(env, nullptr, "adjSeries: Float64Array"); return nullptr; }
+  if (swe_adj_get_series(a, p) != SWE_OK) return throw_adj(env, a, "swe_adj_get_series");
+  return nullptr;
+}
+
+// adjTiming(a) -> [forwardMs, backwardMs]
+napi_value AdjTiming(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  swe_adj *a = get_adj(env, argv[0]);
+  if (!a) return nullptr;
+  double f = 0, b = 0;
+  if (swe_adj_timing(a, &f, &b) != SWE_OK) return throw_adj(env, a, "swe_adj_timing");
+  napi_value arr;
+  NAPI_OK(env, napi_create_array_with_length(env, 2, &arr));
+  NAPI_OK(env, napi_set_element(env, arr, 0, num(env, f)));
+  NAPI_OK(env, napi_set_element(env, arr, 1, num(env, b)));
+  return arr;
+}
+
 napi_value BuildInfo(napi_env env, napi_callback_info) {
   napi_value s;
   napi_create_string_utf8(env, swe_build_info(), NAPI_AUTO_LENGTH, &s);
@@ -306,7 +449,9 @@ napi_value Init(napi_env env, napi_value exports) {
       {"create", Create}, {"setField", SetField}, {"getField", GetField}, {"setRow", SetRow},
       {"setBc", SetBc}, {"setScalar", SetScalar}, {"getScalar", GetScalar}, {"applyBc", ApplyBc},
       {"residual", Residual}, {"maxDt", MaxDt}, {"applyFriction", ApplyFriction}, {"dryClean", DryClean}, {"step", Step}, {"advance", Advance}, {"hllc", Hllc},
-      {"jsPow", JsPow}, {"buildInfo", BuildInfo},
+      {"jsPow", JsPow}, {"buildInfo", BuildInfo}, {"setTide", SetTide},
+      {"adjCreate", AdjCreate}, {"adjSetGauges", AdjSetGauges}, {"adjForward", AdjForward}, {"adjLoss", AdjLoss},
+      {"adjBackward", AdjBackward}, {"adjSeries", AdjSeries}, {"adjTiming", AdjTiming},
   };
   for (auto &f : fns) {
     napi_value fn;

@@ -308,5 +308,44 @@ report('gravity: the drop-in exports the reference constant', G === G_REF, `G = 
   report('Flather: the wall control reflects above 90%', rWall > 0.9, `${fmt(100 * rWall)}% (zero-gradient outflow ${fmt(100 * rOut)}%)`);
 }
 
+// ---- 13. the sphere: metric identities, a resting ocean, the periodic seam ------------------
+{
+  const R = 6371e3, nx = 72, ny = 36;
+  const sphere = { R, lat0: -90, lat1: 90, omega: 7.292115e-5 };
+  // rough bathymetry with dry land, a resting ocean (eta = 0)
+  const r = xorshift(21);
+  const bed = (lon, lat) => -4000 + 3000 * Math.sin(3 * lon * Math.PI / 180) * Math.cos(2 * lat * Math.PI / 180) + 200 * r();
+  const sim = new ShallowWater({ nx, ny, sphere, bed, eta0: 0, manning: 0.025 });
+  const g = sim.geom;
+  let area = 0;
+  for (let j = 0; j < ny; j++) area += nx * g.area[j + 2];
+  report('sphere: cell areas sum to 4 pi R^2', rel(area, 4 * Math.PI * R * R) < 1e-12, `rel ${fmt(rel(area, 4 * Math.PI * R * R))}`);
+  report('sphere: the pole faces have zero length (infinite divisor)',
+    g.dyRowS[2] === Infinity && g.dyRowN[ny + 1] === Infinity);
+  const wantF = 2 * sphere.omega * Math.sin(g.phiC[2 + ny / 2]);
+  report('sphere: f = 2 Omega sin(phi) per row', Object.is(g.fRow[2 + ny / 2], wantF));
+  const v0 = sim.volume();
+  sim.advance(300);
+  // rounding-level: eta = b + h with |b| up to 7 km, so 16 ulp of 7000 m
+  const ulp7k = 2 ** (Math.floor(Math.log2(7000)) - 52);
+  report('sphere: a resting ocean stays at rest', sim.maxSpeed() < 1e-9 && sim.maxSurfaceDeviation(0) < 16 * ulp7k,
+    `max speed ${fmt(sim.maxSpeed())} m/s, max |eta| ${fmt(sim.maxSurfaceDeviation(0))} m over 300 steps`);
+  report('sphere: volume conserved', rel(sim.volume(), v0) < 1e-13, `rel ${fmt(rel(sim.volume(), v0))}`);
+  // a zonally uniform disturbance on a zonally uniform ocean stays zonally uniform
+  // bit for bit: the seam at lon = +-180 is the same water as every other meridian
+  const z = new ShallowWater({ nx, ny, sphere, bed: () => -3000,
+    eta0: (lon, lat) => 2 * Math.exp(-(((lat - 20) / 15) ** 2)), manning: 0.025 });
+  z.advance(200);
+  let same = true;
+  for (let j = 0; j < ny && same; j++) {
+    const k0 = z.idx(0, j);
+    for (let i = 1; i < nx; i++) {
+      const k = z.idx(i, j);
+      if (!Object.is(z.h[k], z.h[k0]) || !Object.is(z.hv[k], z.hv[k0])) { same = false; break; }
+    }
+  }
+  report('sphere: zonal symmetry kept bit for bit across the periodic seam', same && z.finite());
+}
+
 console.log(`\n${fails === 0 ? 'ALL PASS' : 'FAILURES'} ${checks - fails}/${checks}`);
 process.exit(fails === 0 ? 0 : 1);

@@ -13,12 +13,12 @@
 //
 // Time: the forward records dt per step on the device and keeps the state every
 // K steps (K ~ sqrt(nsteps)); the reverse sweep re-runs each segment from its
-// checkpoint stage by stage (the forward's probe sweep for the residuals, then
-// the stage combination, dryClean and friction), keeping every step's
-// BC-applied stage inputs and pre-dryClean stage outputs, then steps back
-// through it with no further recomputation.  Memory: nsteps/K states of 3 W H
-// doubles, K steps of 12 W H, plus 56 + 15 W H doubles of pass intermediates
-// and scratch (about 7 GB at 1024x1024 and 4000 steps).
+// checkpoint with the forward's own stage kernels, writing each step's
+// BC-applied stage inputs A', B' and stage 2's dryCleaned, pre-friction state
+// straight into the segment store, then steps back through the segment with no
+// further recomputation.  Memory: nsteps/K states of 3 W H doubles, K steps of
+// 9 W H, plus 56 + 15 W H doubles of pass intermediates and scratch (about 6 GB
+// at 1024x1024 and 4000 steps).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -77,11 +77,6 @@ __global__ void k_adj_slopes_rev(Params P, StateP S, Work w, StateW lS) {
 __global__ void k_adj_bc(Params P, StateW l, int pass) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < (pass == 0 ? 4 * P.W : 4 * P.ny)) adj_bc_ghost(P, l, pass, t);
-}
-__global__ void k_adj_combine(Params P, StateP A, StateP B, StateP R, double dt, StateW pre, StateW post, int stage,
-                              const double *nfield) {
-  int i, j;
-  if (cell_of(P, i, j)) adj_combine_at(P, A, B, R, dt, pre, post, stage, nfield, i, j);
 }
 __global__ void k_adj_epilogue(Params P, StateP pre, StateP lOut, const double *nfield, double dt, double *lN,
                                int stage, StateW lA, StateW lB, StateW lR) {
@@ -181,12 +176,15 @@ struct swe_adj {
   double *work = nullptr;          // S_COUNT * N
   Work w{};
   double *ckpt = nullptr;          // ceil(nsteps / K) states of 3N
-  double *seg = nullptr;           // K steps x {A', pre1, B', pre2}, 12N each
+  double *seg[2] = {nullptr, nullptr};      // two segment stores, K steps x 9N each
+  double *tcs_seg[2] = {nullptr, nullptr};  // K steps x 2 stages x MAX_TIDE x (cos, sin), per store
+  double *bed = nullptr;           // the bed with its ghost ring, read by the reverse passes
+  hipStream_t rs = nullptr;        // the replay stream
+  hipEvent_t replayed[2] = {nullptr, nullptr}, reversed[2] = {nullptr, nullptr};
   double *tmp = nullptr;           // stage scratch, T_COUNT states of 3N
   double *lN = nullptr;            // adjoint of the Manning field
   double *dt_dev = nullptr;        // recorded dt per step
   double *t_dev = nullptr;         // start time of every step
-  double *tcs_seg = nullptr;       // K steps x 2 stages x MAX_TIDE x (cos, sin)
   int *gcells = nullptr;
   double *gobs = nullptr, *geta = nullptr;
   int ng = 0, every = 1, nsamp = 0;
@@ -217,11 +215,20 @@ int afail(swe_adj *a, int code, const std::string &m) {
 
 // scratch slots
 enum { T_R, T_CUR, T_LBP, T_L0, T_L1, T_COUNT };
-// what one step of a segment keeps: the BC-applied stage inputs and the
-// pre-dryClean stage outputs
-enum { G_AP, G_PRE1, G_BP, G_PRE2, G_COUNT };
+// what one step of a segment keeps: the BC-applied stage inputs (B' holds stage
+// 1's dryCleaned output in its interior) and stage 2's dryCleaned, pre-friction
+// state
+enum { G_AP, G_BP, G_DC2, G_COUNT };
 StateW tslot(swe_adj *a, int q) { return st3(a->tmp + (size_t)q * 3 * a->N, a->N); }
 StateW cur_state(swe_ctx *c) { return {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]}; }
+
+int copy3_on(swe_adj *a, const StateW &dst, const StateP &src, hipStream_t s) {
+  const size_t bytes = a->N * sizeof(double);
+  ACHK(a, hipMemcpyAsync(dst.h, src.h, bytes, hipMemcpyDeviceToDevice, s));
+  ACHK(a, hipMemcpyAsync(dst.hu, src.hu, bytes, hipMemcpyDeviceToDevice, s));
+  ACHK(a, hipMemcpyAsync(dst.hv, src.hv, bytes, hipMemcpyDeviceToDevice, s));
+  return SWE_OK;
+}
 
 int copy3(swe_adj *a, const StateW &dst, const StateP &src) {
   const size_t bytes = a->N * sizeof(double);
@@ -246,32 +253,18 @@ int replay_step(swe_adj *a, int64_t n) {
   return enqueue(a);
 }
 
-// residual of S (ghosts stored) through the forward's probe sweep, at the dt
-// and stage time the control block holds
-int residual_into(swe_adj *a, const StateP &S, int stage, const StateW &R) {
-  swe_ctx *c = a->ctx;
-  SweepArgs s = ctx_base_args(c);
-  for (int q = 0; q < 4; ++q) s.P.bc[q] = BC_STORED;
-  s.P.probe_stage2 = stage == 2;
-  s.in = S;
-  s.rh = R.h; s.rhu = R.hu; s.rhv = R.hv;
-  s.theta = a->w.s[S_TH];
-  ACHK(a, launch_sweep(MODE_PROBE, s, c->stream));
-  return SWE_OK;
-}
-
 int residual_rev(swe_adj *a, const StateP &S, double dt, const StateP &lR, const StateW &lS, const double *tcs) {
   swe_ctx *c = a->ctx;
   const Params &P = c->P;
   hipStream_t st = c->stream;
   const dim3 g = grid_for(a->N), blk(256);
-  hipLaunchKernelGGL(k_adj_cells, g, blk, 0, st, P, S, c->b, a->w);
-  hipLaunchKernelGGL(k_adj_faces, g, blk, 0, st, P, c->b, a->w);
+  hipLaunchKernelGGL(k_adj_cells, g, blk, 0, st, P, S, a->bed, a->w);
+  hipLaunchKernelGGL(k_adj_faces, g, blk, 0, st, P, a->bed, a->w);
   hipLaunchKernelGGL(k_adj_theta, g, blk, 0, st, P, S, dt, a->w);
   hipLaunchKernelGGL(k_adj_accum, g, blk, 0, st, P, lR, a->w);
   hipLaunchKernelGGL(k_adj_theta_rev, g, blk, 0, st, P, S, dt, a->w);
-  hipLaunchKernelGGL(k_adj_faces_rev, g, blk, 0, st, P, c->b, a->w);
-  hipLaunchKernelGGL(k_adj_cells_rev, g, blk, 0, st, P, S, c->b, lR, a->w, tcs);
+  hipLaunchKernelGGL(k_adj_faces_rev, g, blk, 0, st, P, a->bed, a->w);
+  hipLaunchKernelGGL(k_adj_cells_rev, g, blk, 0, st, P, S, a->bed, lR, a->w, tcs);
   hipLaunchKernelGGL(k_adj_slopes_rev, g, blk, 0, st, P, S, a->w, lS);
   ACHK(a, hipGetLastError());
   return SWE_OK;
@@ -286,56 +279,67 @@ int bc_rev(swe_adj *a, const StateW &l) {
   return SWE_OK;
 }
 
-StateW gslot(swe_adj *a, int64_t q, int g) { return st3(a->seg + ((size_t)q * G_COUNT + g) * 3 * a->N, a->N); }
+StateW gslot(swe_adj *a, int buf, int64_t q, int g) {
+  return st3(a->seg[buf] + ((size_t)q * G_COUNT + g) * 3 * a->N, a->N);
+}
+const double *tcs_of(swe_adj *a, int buf, int64_t q) { return a->tcs_seg[buf] + (size_t)q * 2 * MAX_TIDE * 2; }
 
-// Re-run steps [n0, n1) from the state in slot T_CUR, stage by stage, keeping
-// what the reverse of each step reads (slot q = n - n0).
-int replay_segment(swe_adj *a, int64_t n0, int64_t n1) {
+// Re-run steps [n0, n1) of checkpoint segment sgi into segment store `buf` on
+// the replay stream, with the forward's stage kernels at the recorded dt: step
+// q = n - n0 reads A' = slot q (BC applied in place), writes B' = its stage-1
+// output (BC applied in place), stage 2's dryCleaned pre-friction state, and its
+// output straight into slot q + 1.  The replay touches only the store, the
+// control block and the context's bed, never what the reverse sweep reads.
+int replay_segment(swe_adj *a, int64_t sgi, int64_t n0, int64_t n1, int buf) {
   swe_ctx *c = a->ctx;
   const Params &P = c->P;
-  hipStream_t st = c->stream;
-  const dim3 g = grid_for(a->N), blk(256);
-  const StateW R = tslot(a, T_R), cur = tslot(a, T_CUR);
-  const StateW none = {nullptr, nullptr, nullptr};
-  int rc;
+  hipStream_t st = a->rs;
+  const int np = sweep_waves(P, c->chunk);
+  if (int rc = copy3_on(a, gslot(a, buf, 0, G_AP), sp3(a->ckpt + (size_t)sgi * 3 * a->N, a->N), st)) return rc;
   for (int64_t n = n0; n < n1; ++n) {
-    const double dt = a->h_dt[n];
-    const StateW Ap = gslot(a, n - n0, G_AP), pre1 = gslot(a, n - n0, G_PRE1);
-    const StateW Bp = gslot(a, n - n0, G_BP), pre2 = gslot(a, n - n0, G_PRE2);
+    const int64_t q = n - n0;
+    const StateW Ap = gslot(a, buf, q, G_AP), Bp = gslot(a, buf, q, G_BP), dc = gslot(a, buf, q, G_DC2);
+    const StateW next = n + 1 < n1 ? gslot(a, buf, q + 1, G_AP) : tslot(a, T_CUR);
     hipLaunchKernelGGL(k_ctl_dt, dim3(1), dim3(64), 0, st, c->dctl, a->dt_dev, a->t_dev, n,
-                       a->tcs_seg + (size_t)(n - n0) * 2 * MAX_TIDE * 2);
-    if ((rc = copy3(a, Ap, sp(cur)))) return rc;
+                       a->tcs_seg[buf] + (size_t)q * 2 * MAX_TIDE * 2);
     ACHK(a, launch_apply_bc(P, Ap, c->b, c->dctl, st));
-    if ((rc = residual_into(a, sp(Ap), 1, R))) return rc;
-    hipLaunchKernelGGL(k_adj_combine, g, blk, 0, st, P, sp(Ap), sp(Ap), sp(R), dt, pre1, Bp, 1, c->mfield);
+    SweepArgs s1 = ctx_base_args(c);
+    s1.in = sp(Ap);
+    s1.out = Bp;
+    ACHK(a, launch_sweep(MODE_STAGE1, s1, st));
     ACHK(a, launch_apply_bc(P, Bp, c->b, c->dctl, st));
-    if ((rc = residual_into(a, sp(Bp), 2, R))) return rc;
-    hipLaunchKernelGGL(k_adj_combine, g, blk, 0, st, P, sp(Ap), sp(Bp), sp(R), dt, pre2, n + 1 < n1 ? cur : none, 2,
-                       c->mfield);
-    ACHK(a, hipGetLastError());
+    SweepArgs s2 = ctx_base_args(c);
+    s2.in = sp(Bp);
+    s2.h0 = sp(Ap);
+    s2.out = next;
+    s2.rec = dc.h;
+    s2.mf_part = c->mf_part + np;
+    ACHK(a, launch_sweep(MODE_STAGE2, s2, st));
   }
   return SWE_OK;
 }
 
-// Step n back from what replay_segment kept in slot q: maps the adjoint of the
-// step's output (slot a->lam, interior) to the adjoint of its start state.
-int step_rev(swe_adj *a, int64_t q, int64_t n) {
+// Step n back from what replay_segment kept in slot q of store buf: maps the
+// adjoint of the step's output (slot a->lam, interior) to the adjoint of its
+// start state.
+int step_rev(swe_adj *a, int buf, int64_t q, int64_t n) {
   swe_ctx *c = a->ctx;
   const Params &P = c->P;
   const double dt = a->h_dt[n];
   hipStream_t st = c->stream;
   const dim3 g = grid_for(a->N), blk(256);
-  const StateW Ap = gslot(a, q, G_AP), pre1 = gslot(a, q, G_PRE1), Bp = gslot(a, q, G_BP), pre2 = gslot(a, q, G_PRE2);
+  const StateW Ap = gslot(a, buf, q, G_AP), Bp = gslot(a, buf, q, G_BP), dc = gslot(a, buf, q, G_DC2);
   const StateW lR = tslot(a, T_R), lBp = tslot(a, T_LBP), lam = tslot(a, a->lam), lAp = tslot(a, a->lap);
   const StateW none = {nullptr, nullptr, nullptr};
-  const double *tcs = a->tcs_seg + (size_t)q * 2 * MAX_TIDE * 2;
+  const double *tcs = tcs_of(a, buf, q);
   int rc;
   // stage 2: friction^T, dryClean^T, C0 = 0.5 (A' + B' + dt R2)^T, residual^T, BC^T
-  hipLaunchKernelGGL(k_adj_epilogue, g, blk, 0, st, P, sp(pre2), sp(lam), c->mfield, dt, a->lN, 2, lAp, lBp, lR);
+  hipLaunchKernelGGL(k_adj_epilogue, g, blk, 0, st, P, sp(dc), sp(lam), c->mfield, dt, a->lN, 2, lAp, lBp, lR);
   if ((rc = residual_rev(a, sp(Bp), dt, sp(lR), lBp, tcs + 2 * MAX_TIDE))) return rc;
   if ((rc = bc_rev(a, lBp))) return rc;
-  // stage 1: dryClean^T, B0 = A' + dt R1 ^T, residual^T, BC^T
-  hipLaunchKernelGGL(k_adj_epilogue, g, blk, 0, st, P, sp(pre1), sp(lBp), c->mfield, dt, a->lN, 1, lAp, none, lR);
+  // stage 1: dryClean^T (B's interior is stage 1's dryCleaned output), B0 = A' + dt R1 ^T,
+  // residual^T, BC^T
+  hipLaunchKernelGGL(k_adj_epilogue, g, blk, 0, st, P, sp(Bp), sp(lBp), c->mfield, dt, a->lN, 1, lAp, none, lR);
   if ((rc = residual_rev(a, sp(Ap), dt, sp(lR), lAp, tcs))) return rc;
   if ((rc = bc_rev(a, lAp))) return rc;
   std::swap(a->lam, a->lap);
@@ -445,8 +449,16 @@ void swe_adj_destroy(swe_adj *a) {
   if (!a) return;
   // touches nothing of the context, which may already be gone; hipFree waits
   // for the device work that uses the buffers
-  (void)hipFree(a->work); (void)hipFree(a->ckpt); (void)hipFree(a->seg); (void)hipFree(a->tmp);
-  (void)hipFree(a->lN); (void)hipFree(a->dt_dev); (void)hipFree(a->t_dev); (void)hipFree(a->tcs_seg);
+  if (a->rs) (void)hipStreamSynchronize(a->rs);
+  (void)hipFree(a->work); (void)hipFree(a->ckpt); (void)hipFree(a->tmp);
+  for (int q = 0; q < 2; ++q) {
+    (void)hipFree(a->seg[q]); (void)hipFree(a->tcs_seg[q]);
+    if (a->replayed[q]) (void)hipEventDestroy(a->replayed[q]);
+    if (a->reversed[q]) (void)hipEventDestroy(a->reversed[q]);
+  }
+  (void)hipFree(a->bed);
+  if (a->rs) (void)hipStreamDestroy(a->rs);
+  (void)hipFree(a->lN); (void)hipFree(a->dt_dev); (void)hipFree(a->t_dev);
   (void)hipFree(a->gcells); (void)hipFree(a->gobs); (void)hipFree(a->geta);
   delete a;
 }
@@ -476,13 +488,20 @@ int swe_adj_create(swe_ctx *c, int64_t nsteps, int ckpt, swe_adj **out) {
   if ((e = hipMalloc(&a->work, sizeof(double) * S_COUNT * N)) != hipSuccess) return bail("adjoint passes", e);
   for (int q = 0; q < S_COUNT; ++q) a->w.s[q] = a->work + (size_t)q * N;
   if ((e = hipMalloc(&a->ckpt, sizeof(double) * nck * N3)) != hipSuccess) return bail("checkpoints", e);
-  if ((e = hipMalloc(&a->seg, sizeof(double) * (size_t)a->K * G_COUNT * N3)) != hipSuccess) return bail("segment", e);
+  for (int q = 0; q < 2; ++q) {
+    if ((e = hipMalloc(&a->seg[q], sizeof(double) * (size_t)a->K * G_COUNT * N3)) != hipSuccess)
+      return bail("segment store", e);
+    if ((e = hipMalloc(&a->tcs_seg[q], sizeof(double) * (size_t)a->K * 2 * MAX_TIDE * 2)) != hipSuccess)
+      return bail("tide factors", e);
+    if ((e = hipEventCreateWithFlags(&a->replayed[q], hipEventDisableTiming)) != hipSuccess) return bail("event", e);
+    if ((e = hipEventCreateWithFlags(&a->reversed[q], hipEventDisableTiming)) != hipSuccess) return bail("event", e);
+  }
+  if ((e = hipMalloc(&a->bed, sizeof(double) * N)) != hipSuccess) return bail("bed copy", e);
+  if ((e = hipStreamCreateWithFlags(&a->rs, hipStreamNonBlocking)) != hipSuccess) return bail("replay stream", e);
   if ((e = hipMalloc(&a->tmp, sizeof(double) * T_COUNT * N3)) != hipSuccess) return bail("stage scratch", e);
   if ((e = hipMalloc(&a->lN, sizeof(double) * N)) != hipSuccess) return bail("Manning adjoint", e);
   if ((e = hipMalloc(&a->dt_dev, sizeof(double) * (size_t)nsteps)) != hipSuccess) return bail("dt record", e);
   if ((e = hipMalloc(&a->t_dev, sizeof(double) * (size_t)nsteps)) != hipSuccess) return bail("t record", e);
-  if ((e = hipMalloc(&a->tcs_seg, sizeof(double) * (size_t)a->K * 2 * MAX_TIDE * 2)) != hipSuccess)
-    return bail("tide factors", e);
   hipLaunchKernelGGL(k_fill, grid_for(S_COUNT * N), dim3(256), 0, c->stream, S_COUNT * N, a->work, 0.0);
   hipLaunchKernelGGL(k_fill, grid_for(T_COUNT * N3), dim3(256), 0, c->stream, T_COUNT * N3, a->tmp, 0.0);
   if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail("clear", e);
@@ -574,15 +593,39 @@ int swe_adj_backward(swe_adj *a, double *grad_h0, double *grad_n) {
   hipLaunchKernelGGL(k_fill, grid_for(N3), dim3(256), 0, st, N3, tslot(a, a->lam).h, 0.0);
   hipLaunchKernelGGL(k_fill, grid_for(N), dim3(256), 0, st, N, a->lN, 0.0);
   if ((rc = inject(a, a->nsteps))) return rc;
+  // the reverse passes read a private copy of the bed (ghost ring as the forward
+  // leaves it); the replay stream keeps writing the context's own
+  ACHK(a, hipMemcpyAsync(a->bed, c->b, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+  // Segments are reversed last to first.  While the reverse passes of segment s
+  // (HBM-bound gathers, main stream) run, the replay of segment s - 1 (VALU-bound
+  // stage sweeps, replay stream) fills the other store.
   const int64_t nseg = (a->nsteps + a->K - 1) / a->K;
+  auto bounds = [&](int64_t sgi, int64_t &n0, int64_t &n1) {
+    n0 = sgi * a->K;
+    n1 = std::min<int64_t>(a->nsteps, n0 + a->K);
+  };
+  for (int q = 0; q < 2; ++q) ACHK(a, hipEventRecord(a->reversed[q], st));
+  ACHK(a, hipStreamWaitEvent(a->rs, a->reversed[0], 0));
+  int64_t n0, n1;
+  bounds(nseg - 1, n0, n1);
+  if ((rc = replay_segment(a, nseg - 1, n0, n1, (int)((nseg - 1) % 2)))) return rc;
+  ACHK(a, hipEventRecord(a->replayed[(nseg - 1) % 2], a->rs));
   for (int64_t sgi = nseg - 1; sgi >= 0; --sgi) {
-    const int64_t n0 = sgi * a->K, n1 = std::min<int64_t>(a->nsteps, n0 + a->K);
-    if ((rc = copy3(a, tslot(a, T_CUR), sp3(a->ckpt + (size_t)sgi * N3, N)))) return rc;
-    if ((rc = replay_segment(a, n0, n1))) return rc;
+    const int buf = (int)(sgi % 2);
+    ACHK(a, hipStreamWaitEvent(st, a->replayed[buf], 0));
+    if (sgi > 0) {
+      int64_t m0, m1;
+      bounds(sgi - 1, m0, m1);
+      ACHK(a, hipStreamWaitEvent(a->rs, a->reversed[1 - buf], 0));
+      if ((rc = replay_segment(a, sgi - 1, m0, m1, 1 - buf))) return rc;
+      ACHK(a, hipEventRecord(a->replayed[1 - buf], a->rs));
+    }
+    bounds(sgi, n0, n1);
     for (int64_t n = n1 - 1; n >= n0; --n) {
-      if ((rc = step_rev(a, n - n0, n))) return rc;
+      if ((rc = step_rev(a, buf, n - n0, n))) return rc;
       if ((rc = inject(a, n))) return rc;
     }
+    ACHK(a, hipEventRecord(a->reversed[buf], st));
   }
   ACHK(a, hipEventRecord(t.e1, st));
   ACHK(a, hipEventSynchronize(t.e1));
@@ -635,6 +678,17 @@ int swe_adj_host_epilogue_rev(const swe_desc *d, int stage, const double *pre_h,
     return SWE_EINVAL;
   if (nfield) P.friction = 2;
   const size_t N = (size_t)P.W * P.H;
+  // dryClean (swe.mjs:734-745) of the given state on the interior
+  std::vector<double> dc(3 * N);
+  for (size_t k = 0; k < N; ++k) {
+    double h = pre_h[k], hu = pre_hu[k], hv = pre_hv[k];
+    const int i = (int)(k % P.W), j = (int)(k / P.W);
+    if (i >= 2 && i <= P.W - 3 && j >= 2 && j <= P.H - 3) {
+      if (h < 0) h = 0.0;
+      if (h <= P.min_depth) { hu = 0.0; hv = 0.0; }
+    }
+    dc[k] = h; dc[N + k] = hu; dc[2 * N + k] = hv;
+  }
   // with lA = 0 and a scratch lR, stage 1 leaves lA = the adjoint of pre;
   // stage 2 leaves lA = half of it
   std::vector<double> r(3 * N), z(3 * N);
@@ -642,8 +696,7 @@ int swe_adj_host_epilogue_rev(const swe_desc *d, int stage, const double *pre_h,
   for (size_t k = 0; k < N; ++k) { lpre_h[k] = 0.0; lpre_hu[k] = 0.0; lpre_hv[k] = 0.0; }
   for (int j = 0; j < P.H; ++j)
     for (int i = 0; i < P.W; ++i)
-      adj_epilogue_rev_at(P, {pre_h, pre_hu, pre_hv}, {lout_h, lout_hu, lout_hv}, nfield, dt, lN, stage, lA, lB, lR,
-                          i, j);
+      adj_epilogue_rev_at(P, sp3(dc.data(), N), {lout_h, lout_hu, lout_hv}, nfield, dt, lN, stage, lA, lB, lR, i, j);
   if (stage == 2)
     for (size_t k = 0; k < N; ++k) { lpre_h[k] *= 2.0; lpre_hu[k] *= 2.0; lpre_hv[k] *= 2.0; }
   return SWE_OK;

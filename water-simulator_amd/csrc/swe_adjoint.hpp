@@ -505,64 +505,26 @@ __host__ __device__ inline void adj_bc_ghost(const Params &P, const StateW &l, i
   adj_add(&l.hv[s], flipV ? -lhv : lhv);
 }
 
-// ---- stage combination, dryClean^T and friction^T (swe.mjs:659-704, :734-814) --
-// Forward of a stage's pointwise part at cell k: pre = A + dt R (stage 1) or
-// 0.5 (A + B + dt R) (stage 2), and post = dryClean(pre) on the interior, then
-// for stage 2 applyFriction with the scalar n or the field -- the same
-// expressions, in the same order, as the fused stage kernels, so post is the
-// forward's own next state.
-__host__ __device__ inline void adj_combine_at(const Params &P, const StateP &A, const StateP &B, const StateP &R,
-                                               double dt, const StateW &pre, const StateW &post, int stage,
-                                               const double *nfield, int i, int j) {
-  const int k = j * P.W + i;
-  double h, hu, hv;
-  if (stage == 1) {
-    h = A.h[k] + dt * R.h[k]; hu = A.hu[k] + dt * R.hu[k]; hv = A.hv[k] + dt * R.hv[k];
-  } else {
-    h = 0.5 * (A.h[k] + B.h[k] + dt * R.h[k]);
-    hu = 0.5 * (A.hu[k] + B.hu[k] + dt * R.hu[k]);
-    hv = 0.5 * (A.hv[k] + B.hv[k] + dt * R.hv[k]);
-  }
-  pre.h[k] = h; pre.hu[k] = hu; pre.hv[k] = hv;
-  if (post.h) {
-    if (i >= 2 && i <= P.W - 3 && j >= 2 && j <= P.H - 3) {
-      if (h < 0) h = 0.0;
-      if (h <= P.min_depth) { hu = 0.0; hv = 0.0; }
-      if (stage == 2 && P.friction && h > P.min_depth) {
-        const double n = P.friction == 2 ? nfield[k] : P.manning;
-        if (n != 0.0 && n == n) {
-          const Vel2 uv = vel2(hu, hv, h, P.eps4);
-          const double speed = js_hypot(uv.u, uv.v);
-          if (!(speed < 1e-14)) {
-            const double denom = 1 + dt * P.g * n * n * speed / js_pow_pos(h, 4.0 / 3.0);
-            hu = hu / denom;
-            hv = hv / denom;
-          }
-        }
-      }
-    }
-    post.h[k] = h; post.hu[k] = hu; post.hv[k] = hv;
-  }
-}
-
+// ---- dryClean^T and friction^T (swe.mjs:659-704, :734-814) -------------------------
 // Reverse of a stage's pointwise part at cell k, given the adjoint lOut of its
-// output (interior): friction^T (stage 2, accumulating the Manning adjoint into
-// lN), dryClean^T, then the stage combination^T:
+// output (interior) and the stage's dryCleaned (pre-friction) state `dc`:
+// friction^T (stage 2, accumulating the Manning adjoint into lN), dryClean^T,
+// then the stage combination^T:
 //   stage 2: lA = 0.5 lC0, lB = 0.5 lC0, lR = 0.5 dt lC0
 //   stage 1: lA += lB0,    lR = dt lB0
-// Ghost cells of the output carry no adjoint.
-__host__ __device__ inline void adj_epilogue_rev_at(const Params &P, const StateP &pre, const StateP &lOut,
+// dryClean floored the cells whose depth it left at exactly 0 (where the
+// combination gave 0 exactly the subgradient 0 is taken), and zeroed the
+// momentum of the cells at or below minDepth.  Ghost cells carry no adjoint.
+__host__ __device__ inline void adj_epilogue_rev_at(const Params &P, const StateP &dc, const StateP &lOut,
                                                     const double *nfield, double dt, double *lN, int stage,
                                                     const StateW &lA, const StateW &lB, const StateW &lR,
                                                     int i, int j) {
   const int k = j * P.W + i;
   double lh = 0.0, lhu = 0.0, lhv = 0.0;
   if (i >= 2 && i <= P.W - 3 && j >= 2 && j <= P.H - 3) {
-    double h = pre.h[k], hu = pre.hu[k], hv = pre.hv[k];
-    const bool floored = h < 0;
-    if (floored) h = 0.0;
+    const double h = dc.h[k], hu = dc.hu[k], hv = dc.hv[k];
+    const bool floored = h == 0.0;
     const bool dry = h <= P.min_depth;
-    if (dry) { hu = 0.0; hv = 0.0; }
     lh = lOut.h[k]; lhu = lOut.hu[k]; lhv = lOut.hv[k];
     if (stage == 2 && P.friction && !dry) {
       const double n = P.friction == 2 ? nfield[k] : P.manning;

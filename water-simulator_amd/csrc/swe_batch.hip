@@ -233,6 +233,8 @@ int swe_batch_create(const swe_desc *descs, int nb, swe_batch **out) {
     P.min_depth = d.min_depth; P.eps4 = d.eps4; P.cfl = d.cfl; P.R = d.R;
     B.manning = d.manning; B.coriolis = d.coriolis;
     B.N = (size_t)P.W * P.H;
+    P.fstride = (int)up(B.N);        // buffer fields and rows sit up(N), up(H) apart in the pools
+    P.rstride = (int)up(P.H);
     B.off_field = fpool; fpool += FS_COUNT * up(B.N);
     B.off_rows = rpool; rpool += R_COUNT * up(P.H);
     B.max_parts = strips_of(P) * chunks_of(P, MIN_CHUNK);

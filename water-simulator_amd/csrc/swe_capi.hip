@@ -138,11 +138,17 @@ int swe_create(const swe_desc *d, swe_ctx **out) {
   if ((e = hipGetDevice(&c->device)) != hipSuccess) return bail("hipGetDevice", e);
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return bail("stream", e);
   if ((e = hipMalloc(&c->b, bytes)) != hipSuccess) return bail("hipMalloc b", e);
-  for (int i = 0; i < 3; ++i)
-    for (int f = 0; f < 3; ++f)
-      if ((e = hipMalloc(&c->st[i][f], bytes)) != hipSuccess) return bail("hipMalloc state", e);
-  for (int r = 0; r < R_COUNT; ++r)
-    if ((e = hipMalloc(&c->rows[r], sizeof(double) * P.H)) != hipSuccess) return bail("hipMalloc rows", e);
+  // each state buffer's h, hu, hv contiguous, and all rows in one block: the
+  // stage kernels' layout contract (Params::fstride, rstride)
+  for (int i = 0; i < 3; ++i) {
+    if ((e = hipMalloc(&c->st[i][0], 3 * bytes)) != hipSuccess) return bail("hipMalloc state", e);
+    c->st[i][1] = c->st[i][0] + c->N;
+    c->st[i][2] = c->st[i][0] + 2 * c->N;
+  }
+  if ((e = hipMalloc(&c->rows[0], sizeof(double) * P.H * R_COUNT)) != hipSuccess) return bail("hipMalloc rows", e);
+  for (int r = 1; r < R_COUNT; ++r) c->rows[r] = c->rows[0] + (size_t)r * P.H;
+  P.fstride = (int)c->N;
+  P.rstride = P.H;
   if ((e = hipMalloc(&c->dctl, sizeof(Ctl))) != hipSuccess) return bail("hipMalloc ctl", e);
   if ((e = hipHostMalloc((void **)&c->hctl, sizeof(Ctl), hipHostMallocDefault)) != hipSuccess) return bail("pinned ctl", e);
   const int strips = (P.W + SWEEP_OUT - 1) / SWEEP_OUT;
@@ -175,10 +181,10 @@ void swe_destroy(swe_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graphs(c);
   (void)hipFree(c->b);
-  for (auto &s : c->st) for (auto *p : s) (void)hipFree(p);
-  for (auto *p : c->rows) (void)hipFree(p);
+  for (auto &s : c->st) (void)hipFree(s[0]);
+  (void)hipFree(c->rows[0]);
   (void)hipFree(c->mfield); (void)hipFree(c->sponge); (void)hipFree(c->tide);
-  (void)hipFree(c->rh); (void)hipFree(c->rhu); (void)hipFree(c->rhv); (void)hipFree(c->theta);
+  (void)hipFree(c->rh); (void)hipFree(c->theta);
   (void)hipFree(c->dctl); (void)hipFree(c->mf_part);
   if (c->hctl) (void)hipHostFree(c->hctl);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -189,12 +195,15 @@ void swe_destroy(swe_ctx *c) {
 
 static int ensure_probe(swe_ctx *c) {
   const size_t bytes = c->N * sizeof(double);
-  double **arr[4] = {&c->rh, &c->rhu, &c->rhv, &c->theta};
-  for (auto **p : arr) {
-    if (!*p) {
-      HIPCHK(c, hipMalloc(p, bytes));
-      HIPCHK(c, hipMemsetAsync(*p, 0, bytes, c->stream));
-    }
+  if (!c->rh) {                       // rh, rhu, rhv contiguous (Params::fstride)
+    HIPCHK(c, hipMalloc(&c->rh, 3 * bytes));
+    HIPCHK(c, hipMemsetAsync(c->rh, 0, 3 * bytes, c->stream));
+    c->rhu = c->rh + c->N;
+    c->rhv = c->rh + 2 * c->N;
+  }
+  if (!c->theta) {
+    HIPCHK(c, hipMalloc(&c->theta, bytes));
+    HIPCHK(c, hipMemsetAsync(c->theta, 0, bytes, c->stream));
   }
   return SWE_OK;
 }

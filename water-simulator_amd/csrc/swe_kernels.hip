@@ -169,13 +169,17 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
   const int y1 = min(H, y0 + A.rows_per_chunk);
   const Ctl *ctl = A.ctl;
   const double dt = ctl->dt;
+  const int fs = P.fstride;
+  const double *inp = A.in.h;          // in.hu = inp + fs, in.hv = inp + 2 fs
+  const double *h0p = A.h0.h;
+  double *outp = A.out.h;
   if (MODE != MODE_PROBE && (!(dt > 0) || __builtin_isinf(dt))) {
     // a refused step (swe.mjs:784) leaves the state as it was: stage 2 hands the
     // step-start state through to its output buffer unchanged
     if (MODE == MODE_STAGE2 && colIn && outLane) {
       for (int o = y0; o < y1; ++o) {
         const int k = o * W + gi;
-        A.out.h[k] = A.h0.h[k]; A.out.hu[k] = A.h0.hu[k]; A.out.hv[k] = A.h0.hv[k];
+        outp[k] = h0p[k]; outp[k + fs] = h0p[k + fs]; outp[k + 2 * fs] = h0p[k + 2 * fs];
       }
     }
     if (lane == 0) A.mf_part[wave_id] = 0.0;
@@ -188,14 +192,19 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
   const double md = P.min_depth;
   const bool o2 = P.order >= 2;
   const bool bedOn = o2 || P.sphere;
-  const Geo &G = P.geo;
+  // the per-row arrays, rstride apart from dx (Params' layout contract)
+  const double *Rw = P.geo.dx;
+  const int rs = P.rstride;
+  auto rowv = [&](int which, int j) -> double { return Rw[which * rs + j]; };
+  enum { DX, RDX, DYN, RDYN, DYS, RDYS, DYCFL, RDYCFL, BED, RBED, TAN, FROW, GEOC };
 
   auto load_row = [&](int r) -> Row {
     Row c;
     if (r >= 0 && r < H && colIn) {
-      const Cell x = ld(A.in, b, r * W + gi);
-      c.h = x.h; c.b = x.b; c.eta = x.b + x.h;
-      const Vel2 uv = vel2(x.hu, x.hv, x.h, P.eps4);
+      const int k = r * W + gi;
+      const double xh = inp[k], xb = b[k];
+      c.h = xh; c.b = xb; c.eta = xb + xh;
+      const Vel2 uv = vel2(inp[k + fs], inp[k + 2 * fs], xh, P.eps4);
       c.U = uv.u; c.V = uv.v;
     } else {
       c.h = 0.0; c.b = 0.0; c.eta = 0.0; c.U = 0.0; c.V = 0.0;
@@ -255,9 +264,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
     if (bedOn && xin && s >= 2 && s <= H - 3) {
       const double hPx = jmax0(fE.eta - fE.b), hMx = jmax0(fWst.eta - fWst.b);
       const double hPy = jmax0(fN.eta - fN.b), hMy = jmax0(fS.eta - fS.b);
-      bedCur.hu = div_r(P.neg_half_g * (hPx + hMx) * sBx, G.dx[s], G.rdx[s]);
-      bedCur.h = div_r(P.neg_half_g * (hPy + hMy) * sBy, G.bedPhi[s], G.rbedPhi[s]);   // rhv bed term
-      bedCur.hv = P.sphere ? G.geo[s] * P.g * (hPy * hPy + hMy * hMy) : 0.0;           // rhv geometric term
+      bedCur.hu = div_r(P.neg_half_g * (hPx + hMx) * sBx, rowv(DX, s), rowv(RDX, s));
+      bedCur.h = div_r(P.neg_half_g * (hPy + hMy) * sBy, rowv(BED, s), rowv(RBED, s));   // rhv bed term
+      bedCur.hv = P.sphere ? rowv(GEOC, s) * P.g * (hPy * hPy + hMy * hMy) : 0.0;        // rhv geometric term
     }
     // ---- y-face (s-1|s): v normal, u tangential (swe.mjs:448-475) ----------
     FluxF fyCur = {0.0, 0.0, 0.0, 0.0};
@@ -279,10 +288,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
       const double eM = fxPrev.M, wM = shup(fxPrev.M);
       if (posOn && slopeCol && q >= 1 && q <= H - 2) {
         double out = 0.0;
-        if (eM > 0) out += div_r(eM, G.dx[q], G.rdx[q]);
-        if (wM < 0) out -= div_r(wM, G.dx[q], G.rdx[q]);
-        if (fyCur.M > 0) out += div_r(fyCur.M, G.dyN[q], G.rdyN[q]);
-        if (fyPrev.M < 0) out -= div_r(fyPrev.M, G.dyS[q], G.rdyS[q]);
+        if (eM > 0) out += div_r(eM, rowv(DX, q), rowv(RDX, q));
+        if (wM < 0) out -= div_r(wM, rowv(DX, q), rowv(RDX, q));
+        if (fyCur.M > 0) out += div_r(fyCur.M, rowv(DYN, q), rowv(RDYN, q));
+        if (fyPrev.M < 0) out -= div_r(fyPrev.M, rowv(DYS, q), rowv(RDYS, q));
         if (out * dt > hq) th = hq > 0 ? hq / (out * dt) : 0.0;
       }
     }
@@ -292,7 +301,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
       const double thE = shdn(th);
       const double the = fxPrev.M >= 0 ? th : thE;
       double inv = 0.0;
-      if (q >= 0 && q < H) inv = (the == 1.0) ? G.rdx[q] : the / G.dx[q];
+      if (q >= 0 && q < H) inv = (the == 1.0) ? rowv(RDX, q) : the / rowv(DX, q);
       const double eMc = fxPrev.M * inv, eNL = fxPrev.NL * inv, eNR = fxPrev.NR * inv, eT = fxPrev.T * inv;
       const double wMc = shup(eMc), wNR = shup(eNR), wT = shup(eT);
       const bool rowX = q >= 2 && q <= H - 3;
@@ -306,8 +315,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
     const bool fyPrevOn = fyCol && q - 1 >= 1 && q - 1 <= H - 3;
     if (fyPrevOn) {
       const double thy = fyPrev.M >= 0 ? thPrev : th;
-      const double invL = (thy == 1.0) ? G.rdyN[q - 1] : thy / G.dyN[q - 1];
-      const double invR = (thy == 1.0) ? G.rdyS[q] : thy / G.dyS[q];
+      const double invL = (thy == 1.0) ? rowv(RDYN, q - 1) : thy / rowv(DYN, q - 1);
+      const double invR = (thy == 1.0) ? rowv(RDYS, q) : thy / rowv(DYS, q);
       yn.h = fyPrev.M * invL; yn.hu = fyPrev.T * invL; yn.hv = fyPrev.NL * invL;
       ysCur.h = fyPrev.M * invR; ysCur.hu = fyPrev.T * invR; ysCur.hv = fyPrev.NR * invR;
     }
@@ -321,7 +330,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
       if (fySouth) { rh += ysPrev.h; rhu += ysPrev.hu; rhv += ysPrev.hv; }
       if (fyPrevOn) { rh -= yn.h; rhu -= yn.hu; rhv -= yn.hv; }
       const bool interior = xin && o >= 2 && o <= H - 3;
-      const Cell cur = ld(A.in, b, k);
+      const Cell cur = {b[k], inp[k], inp[k + fs], inp[k + 2 * fs]};
       if (interior) {
         if (bedOn) {
           rhu += bedPrev2.hu;
@@ -329,7 +338,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
           if (P.sphere) rhv += bedPrev2.hv;
         }
         if (P.sphere) {                                   // swe.mjs:600-619
-          const double fe = G.fRow[o] + Uo * (G.tanPhi[o] / P.R);
+          const double fe = rowv(FROW, o) + Uo * (rowv(TAN, o) / P.R);
           rhu += fe * cur.hv;
           rhv -= fe * cur.hu;
         } else if (P.coriolis_on) {                       // swe.mjs:620-628
@@ -347,22 +356,23 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
         }
       }
       if (MODE == MODE_PROBE) {
-        A.rh[k] = rh; A.rhu[k] = rhu; A.rhv[k] = rhv; A.theta[k] = thPrev;
+        A.rh[k] = rh; A.rh[k + fs] = rhu; A.rh[k + 2 * fs] = rhv; A.theta[k] = thPrev;
       } else if (MODE == MODE_STAGE1) {
         if (interior) {                                   // swe.mjs:790-796
           double nh = cur.h + dt * rh, nhu = cur.hu + dt * rhu, nhv = cur.hv + dt * rhv;
           if (nh < 0) { mf -= nh; nh = 0.0; }
           if (nh <= md) { nhu = 0.0; nhv = 0.0; }
-          A.out.h[k] = nh; A.out.hu[k] = nhu; A.out.hv[k] = nhv;
+          outp[k] = nh; outp[k + fs] = nhu; outp[k + 2 * fs] = nhv;
         }
       } else {                                            // stage 2, swe.mjs:799-812
-        const double h0 = A.h0.h[k], hu0 = A.h0.hu[k], hv0 = A.h0.hv[k];
+        const double h0 = h0p[k], hu0 = h0p[k + fs], hv0 = h0p[k + 2 * fs];
         double nh = 0.5 * (h0 + cur.h + dt * rh);
         double nhu = 0.5 * (hu0 + cur.hu + dt * rhu);
         double nhv = 0.5 * (hv0 + cur.hv + dt * rhv);
         if (interior) {
           if (nh < 0) { mf -= nh; nh = 0.0; }
           if (nh <= md) { nhu = 0.0; nhv = 0.0; }
+          if (A.rec) { A.rec[k] = nh; A.rec[k + fs] = nhu; A.rec[k + 2 * fs] = nhv; }
           // applyFriction (swe.mjs:659-704)
           if (P.friction) {
             if (nh <= md) { nhu = 0.0; nhv = 0.0; }
@@ -394,8 +404,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
           if (nh > md) {
             const double c = __builtin_sqrt(P.g * nh);
             const Vel2 uv = vel2(nhu, nhv, nh, P.eps4);
-            const double sp = div_r(__builtin_fabs(uv.u) + c, G.dx[o], G.rdx[o]) +
-                              div_r(__builtin_fabs(uv.v) + c, G.dyCFL[o], G.rdyCFL[o]);
+            const double sp = div_r(__builtin_fabs(uv.u) + c, rowv(DX, o), rowv(RDX, o)) +
+                              div_r(__builtin_fabs(uv.v) + c, rowv(DYCFL, o), rowv(RDYCFL, o));
             if (sp > 0) {
               const unsigned long long bits = (unsigned long long)__double_as_longlong(sp);
               if (bits > cflb) cflb = bits;
@@ -403,9 +413,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
           }
         }
         if (!(__builtin_isfinite(nh) && __builtin_isfinite(nhu) && __builtin_isfinite(nhv))) bad = true;
-        A.out.h[k] = nh; A.out.hu[k] = nhu; A.out.hv[k] = nhv;
+        outp[k] = nh; outp[k + fs] = nhu; outp[k + 2 * fs] = nhv;
         if (A.P.want_theta) A.theta[k] = thPrev;
-        if (A.P.want_rh) { A.rh[k] = rh; A.rhu[k] = rhu; A.rhv[k] = rhv; }
+        if (A.P.want_rh) { A.rh[k] = rh; A.rh[k + fs] = rhu; A.rh[k + 2 * fs] = rhv; }
       }
     }
 

@@ -68,6 +68,13 @@ struct Params {
   // sweep evaluates it at the second stage time when probe_stage2 is set
   int tide_n, probe_stage2;
   const double *tide;
+  // Layout contract of the stage kernels: the hu and hv of a state buffer (and
+  // rhu, rhv of the residual outputs) sit fstride doubles after its h; the
+  // per-row arrays sit rstride doubles apart in the order dx, rdx, dyN, rdyN,
+  // dyS, rdyS, dyCFL, rdyCFL, bedPhi, rbedPhi, tanPhi, fRow, geo starting at
+  // geo.dx.  The kernels then carry one base pointer per buffer instead of
+  // three, and one for all the rows.
+  int fstride, rstride;
 };
 
 struct StateP { const double *h, *hu, *hv; };
@@ -83,6 +90,7 @@ struct SweepArgs {
   const double *mfield, *sponge;
   Ctl *ctl;
   double *mf_part;    // one partial per wave
+  double *rec;        // stage 2, optional: the dryCleaned, pre-friction interior state (adjoint replay)
   int rows_per_chunk;
 };
 

@@ -7,18 +7,19 @@
 //   C  = friction(dryClean(C0), n)                                      (stage 2)
 // Its reverse, given the adjoint of C on the interior: friction^T and dryClean^T,
 // the RK combination^T, residual^T of stage 2, BC^T, then the same for stage 1.
-// residual^T runs as eight one-thread-per-cell passes over the grid with their
-// intermediates in HBM (swe_adjoint.hpp).  The step's stages are recomputed
-// from the step-start state with the forward's own probe sweep.
+// residual^T runs as seven one-thread-per-cell passes over the grid with their
+// intermediates in HBM (swe_adjoint.hpp).
 //
 // Time: the forward records dt per step on the device and keeps the state every
 // K steps (K ~ sqrt(nsteps)); the reverse sweep re-runs each segment from its
 // checkpoint with the forward's own stage kernels, writing each step's
 // BC-applied stage inputs A', B' and stage 2's dryCleaned, pre-friction state
-// straight into the segment store, then steps back through the segment with no
-// further recomputation.  Memory: nsteps/K states of 3 W H doubles, K steps of
-// 9 W H, plus 56 + 15 W H doubles of pass intermediates and scratch (about 6 GB
-// at 1024x1024 and 4000 steps).
+// straight into a segment store, then steps back through the segment with no
+// further recomputation.  The replay of segment s - 1 runs on a second stream
+// into the other of two stores while segment s is reversed.  Memory: nsteps/K
+// states of 3 W H doubles, two stores of K steps of 9 W H, plus 54 + 15 W H
+// doubles of pass intermediates and scratch (about 12 GB at 1024x1024 and 4000
+// steps).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,8 +36,16 @@ using namespace swe;
 namespace {
 
 // ---- one-thread-per-cell kernels over the shared bodies -----------------------
+// Workgroups are dealt to the 8 XCDs round-robin (block b to XCD b mod 8), and
+// each XCD has its own L2.  The grids are a multiple of 8 blocks, and block b
+// takes the cells of logical block (b mod 8) * (nblocks / 8) + b / 8, so each XCD
+// sweeps one contiguous band of rows and finds the rows above and below a cell
+// (which every gather pass reads) in its own L2.
+constexpr int NXCD = 8;
 __device__ __forceinline__ bool cell_of(const Params &P, int &i, int &j) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int per = gridDim.x / NXCD;
+  const int blk = (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
+  const int k = blk * blockDim.x + threadIdx.x;
   i = k % P.W;
   j = k / P.W;
   return j < P.H;
@@ -54,13 +63,9 @@ __global__ void k_adj_theta(Params P, StateP S, double dt, Work w) {
   int i, j;
   if (cell_of(P, i, j)) adj_theta_at(P, S, dt, w, i, j);
 }
-__global__ void k_adj_accum(Params P, StateP lR, Work w) {
+__global__ void k_adj_accum(Params P, StateP S, double dt, StateP lR, Work w) {
   int i, j;
-  if (cell_of(P, i, j)) adj_accum_at(P, lR, w, i, j);
-}
-__global__ void k_adj_theta_rev(Params P, StateP S, double dt, Work w) {
-  int i, j;
-  if (cell_of(P, i, j)) adj_theta_rev_at(P, S, dt, w, i, j);
+  if (cell_of(P, i, j)) adj_accum_at(P, S, dt, lR, w, i, j);
 }
 __global__ void k_adj_faces_rev(Params P, const double *b, Work w) {
   int i, j;
@@ -117,7 +122,11 @@ __global__ void k_gauge_inject(int ng, const int *cells, const double *eta, cons
     for (int g = 0; g < ng; ++g) lh[cells[g]] += 2.0 * (eta[g] - obs[g]);
 }
 
-dim3 grid_for(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
+// 256-thread blocks covering n, rounded up to a multiple of the XCD count
+dim3 grid_for(size_t n) {
+  const size_t nb = (n + 255) / 256;
+  return dim3((unsigned)((nb + NXCD - 1) / NXCD * NXCD));
+}
 
 StateW st3(double *base, size_t N) { return {base, base + N, base + 2 * N}; }
 StateP sp3(const double *base, size_t N) { return {base, base + N, base + 2 * N}; }
@@ -133,8 +142,7 @@ void residual_rev_host(const Params &P, const StateP &S, const double *b, double
   each([&](int i, int j) { adj_cells_at(P, S, b, w, i, j); });
   each([&](int i, int j) { adj_faces_at(P, b, w, i, j); });
   each([&](int i, int j) { adj_theta_at(P, S, dt, w, i, j); });
-  each([&](int i, int j) { adj_accum_at(P, lR, w, i, j); });
-  each([&](int i, int j) { adj_theta_rev_at(P, S, dt, w, i, j); });
+  each([&](int i, int j) { adj_accum_at(P, S, dt, lR, w, i, j); });
   each([&](int i, int j) { adj_faces_rev_at(P, b, w, i, j); });
   each([&](int i, int j) { adj_cells_rev_at(P, S, b, lR, w, nullptr, i, j); });
   each([&](int i, int j) { adj_slopes_rev_at(P, S, w, lS, i, j); });
@@ -261,8 +269,7 @@ int residual_rev(swe_adj *a, const StateP &S, double dt, const StateP &lR, const
   hipLaunchKernelGGL(k_adj_cells, g, blk, 0, st, P, S, a->bed, a->w);
   hipLaunchKernelGGL(k_adj_faces, g, blk, 0, st, P, a->bed, a->w);
   hipLaunchKernelGGL(k_adj_theta, g, blk, 0, st, P, S, dt, a->w);
-  hipLaunchKernelGGL(k_adj_accum, g, blk, 0, st, P, lR, a->w);
-  hipLaunchKernelGGL(k_adj_theta_rev, g, blk, 0, st, P, S, dt, a->w);
+  hipLaunchKernelGGL(k_adj_accum, g, blk, 0, st, P, S, dt, lR, a->w);
   hipLaunchKernelGGL(k_adj_faces_rev, g, blk, 0, st, P, a->bed, a->w);
   hipLaunchKernelGGL(k_adj_cells_rev, g, blk, 0, st, P, S, a->bed, lR, a->w, tcs);
   hipLaunchKernelGGL(k_adj_slopes_rev, g, blk, 0, st, P, S, a->w, lS);

@@ -20,10 +20,11 @@ __host__ __device__ __forceinline__ VelD vel_deriv(double hq, double h, double e
   const double h4 = h * h * h * h;
   const bool self = h4 > eps4;                       // max(h4, eps4) picks h4
   const double D = __builtin_sqrt(h4 + (self ? h4 : eps4));
-  const double dD = (4.0 * h * h * h * (self ? 2.0 : 1.0)) / (2.0 * D);
+  const double rD = 1.0 / D;
+  const double dD = (4.0 * h * h * h * (self ? 2.0 : 1.0)) * (0.5 * rD);
   VelD r;
-  r.dhq = SQRT2 * h / D;
-  r.dh = SQRT2 * hq / D - SQRT2 * h * hq * dD / (D * D);
+  r.dhq = SQRT2 * h * rD;
+  r.dh = SQRT2 * hq * rD - SQRT2 * h * hq * dD * (rD * rD);
   return r;
 }
 
@@ -120,32 +121,38 @@ __host__ __device__ inline HllcIn hllc_rev(double hL, double huL, double hvL, do
     else {
       luL += lsL; lcL -= lsL * qL; const double lqL = -lsL * cL;
       luR += lsR; lcR += lsR * qR; const double lqR = lsR * cR;
+      // X = (hStar + h) hStar / (2 h^2) and qL = sqrt(X) (qR likewise)
       double lhStar = 0.0;
       if (hStar > hL) {
-        const double X = 0.5 * (hStar + hL) * hStar / (hL * hL);
-        const double lX = lqL / (2.0 * qL);
-        lhStar += lX * 0.5 * (2.0 * hStar + hL) / (hL * hL);
-        g.hL += lX * (0.5 * hStar / (hL * hL) - 2.0 * X / hL);
+        const double r = 1.0 / hL, r2 = r * r;
+        const double X = 0.5 * (hStar + hL) * hStar * r2;
+        const double lX = lqL * (0.5 / qL);
+        lhStar += lX * 0.5 * (2.0 * hStar + hL) * r2;
+        g.hL += lX * (0.5 * hStar * r2 - 2.0 * X * r);
       }
       if (hStar > hR) {
-        const double X = 0.5 * (hStar + hR) * hStar / (hR * hR);
-        const double lX = lqR / (2.0 * qR);
-        lhStar += lX * 0.5 * (2.0 * hStar + hR) / (hR * hR);
-        g.hR += lX * (0.5 * hStar / (hR * hR) - 2.0 * X / hR);
+        const double r = 1.0 / hR, r2 = r * r;
+        const double X = 0.5 * (hStar + hR) * hStar * r2;
+        const double lX = lqR * (0.5 / qR);
+        lhStar += lX * 0.5 * (2.0 * hStar + hR) * r2;
+        g.hR += lX * (0.5 * hStar * r2 - 2.0 * X * r);
       }
-      const double la = lhStar * 2.0 * a / G;
+      const double la = lhStar * 2.0 * a * C.rg;
       lcL += 0.5 * la; lcR += 0.5 * la; luL += 0.25 * la; luR -= 0.25 * la;
     }
   }
+  // u = hu / h, c = sqrt(G h)
   if (wetL) {
-    g.hL += lcL * G / (2.0 * cL);
-    g.huL += luL / hL; g.hvL += lvL / hL;
-    g.hL -= (luL * uL + lvL * vL) / hL;
+    const double r = 1.0 / hL;
+    g.hL += lcL * G * (0.5 / cL);
+    g.huL += luL * r; g.hvL += lvL * r;
+    g.hL -= (luL * uL + lvL * vL) * r;
   }
   if (wetR) {
-    g.hR += lcR * G / (2.0 * cR);
-    g.huR += luR / hR; g.hvR += lvR / hR;
-    g.hR -= (luR * uR + lvR * vR) / hR;
+    const double r = 1.0 / hR;
+    g.hR += lcR * G * (0.5 / cR);
+    g.huR += luR * r; g.hvR += lvR * r;
+    g.hR -= (luR * uR + lvR * vR) * r;
   }
   return g;
 }
@@ -182,8 +189,8 @@ enum Slot {
   // forward recompute: velocities, surface, slopes (x, y), faces, positivity
   S_U, S_V, S_ETA, S_SEX, S_SEY, S_SUX, S_SUY, S_SVX, S_SVY, S_SBX, S_SBY,
   S_FXM, S_FXNL, S_FXNR, S_FXT, S_FYM, S_FYNL, S_FYNR, S_FYT, S_TH, S_OUT,
-  // adjoints: face flux terms and positivity factor per face, theta^T per cell
-  S_LXM, S_LXNL, S_LXNR, S_LXT, S_LYM, S_LYNL, S_LYNR, S_LYT, S_LXTH, S_LYTH, S_LOUT, S_LHTH,
+  // adjoints: face flux terms per face, theta^T per cell
+  S_LXM, S_LXNL, S_LXNR, S_LXT, S_LYM, S_LYNL, S_LYNR, S_LYT, S_LOUT, S_LHTH,
   // adjoints of the reconstructed face states (grid frame: eta, u, v; L and R)
   S_LXETAL, S_LXUL, S_LXVL, S_LXETAR, S_LXUR, S_LXVR,
   S_LYETAL, S_LYUL, S_LYVL, S_LYETAR, S_LYUR, S_LYVR,
@@ -208,17 +215,20 @@ __host__ __device__ inline void adj_cells_at(const Params &P, const StateP &S, c
   const Vel2 uv = vel2(S.hu[k], S.hv[k], h, P.eps4);
   w.s[S_U][k] = uv.u; w.s[S_V][k] = uv.v; w.s[S_ETA][k] = b[k] + h;
   double sl[8] = {0, 0, 0, 0, 0, 0, 0, 0};        // sEx sEy sUx sUy sVx sVy sBx sBy
-  if (P.order >= 2 && i >= 1 && i <= P.W - 2 && j >= 1 && j <= P.H - 2) {
-    for (int d = 0; d < 2; ++d) {
-      const int kp = d == 0 ? k + 1 : k + P.W, km = d == 0 ? k - 1 : k - P.W;
-      if (h <= P.min_depth || S.h[kp] <= P.min_depth || S.h[km] <= P.min_depth) continue;
-      const double eC = b[k] + h, eP = b[kp] + S.h[kp], eM = b[km] + S.h[km];
-      const Vel2 up = vel2(S.hu[kp], S.hv[kp], S.h[kp], P.eps4), um = vel2(S.hu[km], S.hv[km], S.h[km], P.eps4);
-      sl[0 + d] = limit(P.limiter, eC - eM, eP - eC);
-      sl[6 + d] = limit(P.limiter, b[k] - b[km], b[kp] - b[k]);
-      sl[2 + d] = limit(P.limiter, uv.u - um.u, up.u - uv.u);
-      sl[4 + d] = limit(P.limiter, uv.v - um.v, up.v - uv.v);
-    }
+  const bool in = P.order >= 2 && i >= 1 && i <= P.W - 2 && j >= 1 && j <= P.H - 2;
+  const double bk = b[k];
+  for (int d = 0; d < 2; ++d) {
+    // neighbours loaded before the branches (cell k stands in off the grid)
+    const int kp = in ? (d == 0 ? k + 1 : k + P.W) : k, km = in ? (d == 0 ? k - 1 : k - P.W) : k;
+    const double hp = S.h[kp], hm = S.h[km], bp = b[kp], bm = b[km];
+    const double hup = S.hu[kp], hvp = S.hv[kp], hum = S.hu[km], hvm = S.hv[km];
+    if (!in || h <= P.min_depth || hp <= P.min_depth || hm <= P.min_depth) continue;
+    const double eC = bk + h, eP = bp + hp, eM = bm + hm;
+    const Vel2 up = vel2(hup, hvp, hp, P.eps4), um = vel2(hum, hvm, hm, P.eps4);
+    sl[0 + d] = limit(P.limiter, eC - eM, eP - eC);
+    sl[6 + d] = limit(P.limiter, bk - bm, bp - bk);
+    sl[2 + d] = limit(P.limiter, uv.u - um.u, up.u - uv.u);
+    sl[4 + d] = limit(P.limiter, uv.v - um.v, up.v - uv.v);
   }
   for (int q = 0; q < 8; ++q) w.s[S_SEX + q][k] = sl[q];
 }
@@ -241,8 +251,10 @@ __host__ __device__ inline void adj_faces_at(const Params &P, const double *b, c
   const HllcConst HC = {P.g, P.half_g, P.rg, P.min_depth};
   double f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int d = 0; d < 2; ++d) {
-    if (!(d == 0 ? xface_on(P, i, j) : yface_on(P, i, j))) continue;
-    const FaceSide L = face_side(w, b, k, d, 1.0), R = face_side(w, b, d == 0 ? k + 1 : k + P.W, d, -1.0);
+    // both sides loaded before the branch (cell k stands in for a face off the grid)
+    const bool on = d == 0 ? xface_on(P, i, j) : yface_on(P, i, j);
+    const FaceSide L = face_side(w, b, k, d, 1.0), R = face_side(w, b, on ? (d == 0 ? k + 1 : k + P.W) : k, d, -1.0);
+    if (!on) continue;
     const double nL = d == 0 ? L.u : L.v, tL = d == 0 ? L.v : L.u;
     const double nR = d == 0 ? R.u : R.v, tR = d == 0 ? R.v : R.u;
     const double hLf = jmax0(L.eta - L.b), hRf = jmax0(R.eta - R.b);
@@ -261,72 +273,99 @@ __host__ __device__ inline void adj_faces_at(const Params &P, const double *b, c
 __host__ __device__ inline void adj_theta_at(const Params &P, const StateP &S, double dt, const Work &w, int i, int j) {
   const int k = j * P.W + i;
   double th = 1.0, out = 0.0;
-  if (dt > 0 && i >= 1 && i <= P.W - 2 && j >= 1 && j <= P.H - 2) {
-    const double *fxM = w.s[S_FXM], *fyM = w.s[S_FYM];
-    if (fxM[k] > 0) out += fxM[k] / P.geo.dx[j];
-    if (fxM[k - 1] < 0) out -= fxM[k - 1] / P.geo.dx[j];
-    if (fyM[k] > 0) out += fyM[k] / P.geo.dyN[j];
-    if (fyM[k - P.W] < 0) out -= fyM[k - P.W] / P.geo.dyS[j];
-    const double avail = S.h[k];
+  const bool in = dt > 0 && i >= 1 && i <= P.W - 2 && j >= 1 && j <= P.H - 2;
+  const double *fxM = w.s[S_FXM], *fyM = w.s[S_FYM];
+  const double fe = fxM[k], fw = fxM[in ? k - 1 : k], fn = fyM[k], fs = fyM[in ? k - P.W : k];
+  const double avail = S.h[k];
+  if (in) {
+    if (fe > 0) out += fe / P.geo.dx[j];
+    if (fw < 0) out -= fw / P.geo.dx[j];
+    if (fn > 0) out += fn / P.geo.dyN[j];
+    if (fs < 0) out -= fs / P.geo.dyS[j];
     if (out * dt > avail) th = avail > 0 ? avail / (out * dt) : 0.0;
   }
   w.s[S_TH][k] = th;
   w.s[S_OUT][k] = out;
 }
 
-// pass 4: accumulation^T per face (swe.mjs:530-555): the adjoints of its flux
-// terms and of the positivity factor that scaled it
-__host__ __device__ inline void adj_accum_at(const Params &P, const StateP &lR, const Work &w, int i, int j) {
-  const int k = j * P.W + i;
-  double lx[5] = {0, 0, 0, 0, 0}, ly[5] = {0, 0, 0, 0, 0};
-  const double *th = w.s[S_TH];
-  if (xface_on(P, i, j)) {
-    const int kL = k, kR = k + 1;
-    const double M = w.s[S_FXM][k], NL = w.s[S_FXNL][k], NR = w.s[S_FXNR][k], T = w.s[S_FXT][k];
-    const double t = M >= 0 ? th[kL] : th[kR];
-    const double inv = t / P.geo.dx[j];
-    const double dh = lR.h[kR] - lR.h[kL], dv = lR.hv[kR] - lR.hv[kL];
-    lx[0] = inv * dh; lx[1] = -inv * lR.hu[kL]; lx[2] = inv * lR.hu[kR]; lx[3] = inv * dv;
-    lx[4] = (M * dh + NR * lR.hu[kR] - NL * lR.hu[kL] + T * dv) / P.geo.dx[j];
-  }
-  if (yface_on(P, i, j)) {
-    const int kL = k, kR = k + P.W;
-    const double M = w.s[S_FYM][k], NL = w.s[S_FYNL][k], NR = w.s[S_FYNR][k], T = w.s[S_FYT][k];
-    const double t = M >= 0 ? th[kL] : th[kR];
-    const double iL = t / P.geo.dyN[j], iR = t / P.geo.dyS[j + 1];
-    ly[0] = lR.h[kR] * iR - lR.h[kL] * iL;
-    ly[1] = -lR.hv[kL] * iL;
-    ly[2] = lR.hv[kR] * iR;
-    ly[3] = lR.hu[kR] * iR - lR.hu[kL] * iL;
-    ly[4] = (M * lR.h[kR] + NR * lR.hv[kR] + T * lR.hu[kR]) / P.geo.dyS[j + 1] -
-            (M * lR.h[kL] + NL * lR.hv[kL] + T * lR.hu[kL]) / P.geo.dyN[j];
-  }
-  w.s[S_LXM][k] = lx[0]; w.s[S_LXNL][k] = lx[1]; w.s[S_LXNR][k] = lx[2]; w.s[S_LXT][k] = lx[3]; w.s[S_LXTH][k] = lx[4];
-  w.s[S_LYM][k] = ly[0]; w.s[S_LYNL][k] = ly[1]; w.s[S_LYNR][k] = ly[2]; w.s[S_LYT][k] = ly[3]; w.s[S_LYTH][k] = ly[4];
+// The positivity-factor adjoint of one face: d/dtheta of its contributions
+// (the face's flux terms scaled by theta over the row divisors), given the
+// adjoint lR of the residual.  x-face owned by cell kf in row j:
+__host__ __device__ inline double face_lth_x(const Params &P, const StateP &lR, const Work &w, int kf, int j) {
+  const int kL = kf, kR = kf + 1;
+  const double M = w.s[S_FXM][kf], NL = w.s[S_FXNL][kf], NR = w.s[S_FXNR][kf], T = w.s[S_FXT][kf];
+  const double dh = lR.h[kR] - lR.h[kL], dv = lR.hv[kR] - lR.hv[kL];
+  return (M * dh + NR * lR.hu[kR] - NL * lR.hu[kL] + T * dv) / P.geo.dx[j];
+}
+// y-face owned by cell kf in row j (between rows j and j + 1)
+__host__ __device__ inline double face_lth_y(const Params &P, const StateP &lR, const Work &w, int kf, int j) {
+  const int kL = kf, kR = kf + P.W;
+  const double M = w.s[S_FYM][kf], NL = w.s[S_FYNL][kf], NR = w.s[S_FYNR][kf], T = w.s[S_FYT][kf];
+  return (M * lR.h[kR] + NR * lR.hv[kR] + T * lR.hu[kR]) / P.geo.dyS[j + 1] -
+         (M * lR.h[kL] + NL * lR.hv[kL] + T * lR.hu[kL]) / P.geo.dyN[j];
 }
 
-// pass 5: positivity^T per cell: the factor's adjoint gathered from the faces
-// it scaled (as their upwind cell), then through theta = h / (out dt)
-__host__ __device__ inline void adj_theta_rev_at(const Params &P, const StateP &S, double dt, const Work &w,
-                                                 int i, int j) {
+// pass 4: accumulation^T per face (swe.mjs:530-555) and positivity^T per cell
+// (swe.mjs:507-529).  The adjoints of the flux terms of the cell's east and north
+// faces; then the positivity factor's adjoint gathered from the four faces it
+// scaled as their upwind cell (the west and south faces' shares recomputed from
+// their owners' data) and taken through theta = h / (out dt).
+__host__ __device__ inline void adj_accum_at(const Params &P, const StateP &S, double dt, const StateP &lR,
+                                             const Work &w, int i, int j) {
   const int k = j * P.W + i;
+  double lx[4] = {0, 0, 0, 0}, ly[4] = {0, 0, 0, 0}, lthE = 0.0, lthN = 0.0, lthW = 0.0, lthS = 0.0;
+  const double *th = w.s[S_TH];
+  // everything loaded before the branches (cell k stands in for a face off the grid)
+  const double lhk = lR.h[k], lhuk = lR.hu[k], lhvk = lR.hv[k], thk = th[k];
+  const bool xon = xface_on(P, i, j), yon = yface_on(P, i, j);
+  const int kx = xon ? k + 1 : k, ky = yon ? k + P.W : k;
+  const double xM = w.s[S_FXM][k], xNL = w.s[S_FXNL][k], xNR = w.s[S_FXNR][k], xT = w.s[S_FXT][k];
+  const double yM = w.s[S_FYM][k], yNL = w.s[S_FYNL][k], yNR = w.s[S_FYNR][k], yT = w.s[S_FYT][k];
+  const double xth = th[kx], xlh = lR.h[kx], xlhu = lR.hu[kx], xlhv = lR.hv[kx];
+  const double yth = th[ky], ylh = lR.h[ky], ylhu = lR.hu[ky], ylhv = lR.hv[ky];
+  const double dxj = P.geo.dx[j], dyNj = P.geo.dyN[j], dySj = P.geo.dyS[yon ? j + 1 : j];
+  const bool won = i >= 1 && xface_on(P, i - 1, j), son = j >= 1 && yface_on(P, i, j - 1);
+  const double fw = w.s[S_FXM][i >= 1 ? k - 1 : k], fs = w.s[S_FYM][j >= 1 ? k - P.W : k];
+  const double thc = w.s[S_TH][k], out = w.s[S_OUT][k], h = S.h[k];
+  if (xon) {
+    const double M = xM, NL = xNL, NR = xNR, T = xT;
+    const double t = M >= 0 ? thk : xth;
+    const double inv = t / dxj;
+    const double dh = xlh - lhk, dv = xlhv - lhvk;
+    lx[0] = inv * dh; lx[1] = -inv * lhuk; lx[2] = inv * xlhu; lx[3] = inv * dv;
+    lthE = (M * dh + NR * xlhu - NL * lhuk + T * dv) / dxj;
+  }
+  if (yon) {
+    const double M = yM, NL = yNL, NR = yNR, T = yT;
+    const double t = M >= 0 ? thk : yth;
+    const double iL = t / dyNj, iR = t / dySj;
+    ly[0] = ylh * iR - lhk * iL;
+    ly[1] = -lhvk * iL;
+    ly[2] = ylhv * iR;
+    ly[3] = ylhu * iR - lhuk * iL;
+    lthN = (M * ylh + NR * ylhv + T * ylhu) / dySj -
+           (M * lhk + NL * lhvk + T * lhuk) / dyNj;
+  }
+  if (won) lthW = face_lth_x(P, lR, w, k - 1, j);
+  if (son) lthS = face_lth_y(P, lR, w, k - P.W, j - 1);
+  w.s[S_LXM][k] = lx[0]; w.s[S_LXNL][k] = lx[1]; w.s[S_LXNR][k] = lx[2]; w.s[S_LXT][k] = lx[3];
+  w.s[S_LYM][k] = ly[0]; w.s[S_LYNL][k] = ly[1]; w.s[S_LYNR][k] = ly[2]; w.s[S_LYT][k] = ly[3];
+  // a face off the grid has zero flux and zero share
   double lth = 0.0;
-  const double *fxM = w.s[S_FXM], *fyM = w.s[S_FYM];
-  if (fxM[k] >= 0) lth += w.s[S_LXTH][k];                      // east face
-  if (i >= 1 && fxM[k - 1] < 0) lth += w.s[S_LXTH][k - 1];     // west face
-  if (fyM[k] >= 0) lth += w.s[S_LYTH][k];                      // north face
-  if (j >= 1 && fyM[k - P.W] < 0) lth += w.s[S_LYTH][k - P.W]; // south face
+  if (xM >= 0) lth += lthE;                    // east face
+  if (i >= 1 && fw < 0) lth += lthW;           // west face
+  if (yM >= 0) lth += lthN;                    // north face
+  if (j >= 1 && fs < 0) lth += lthS;           // south face
   double lh = 0.0, lout = 0.0;
-  const double th = w.s[S_TH][k], out = w.s[S_OUT][k], h = S.h[k];
-  if (th < 1.0 && h > 0 && out * dt > h) {
+  if (thc < 1.0 && h > 0 && out * dt > h) {
     lh = lth / (out * dt);
-    lout = -lth * th / out;
+    lout = -lth * thc / out;
   }
   w.s[S_LHTH][k] = lh;
   w.s[S_LOUT][k] = lout;
 }
 
-// pass 6: face^T: the adjoints of both sides' reconstructed states of the
+// pass 5: face^T: the adjoints of both sides' reconstructed states of the
 // east x-face and north y-face of cell (i, j)
 __host__ __device__ inline void adj_faces_rev_at(const Params &P, const double *b, const Work &w, int i, int j) {
   const int k = j * P.W + i;
@@ -334,18 +373,23 @@ __host__ __device__ inline void adj_faces_rev_at(const Params &P, const double *
   const double *lout = w.s[S_LOUT];
   double r[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int d = 0; d < 2; ++d) {
-    if (!(d == 0 ? xface_on(P, i, j) : yface_on(P, i, j))) continue;
-    const int kR = d == 0 ? k + 1 : k + P.W;
-    const double divL = d == 0 ? P.geo.dx[j] : P.geo.dyN[j], divR = d == 0 ? P.geo.dx[j] : P.geo.dyS[j + 1];
+    // everything loaded before the branch (cell k stands in for a face off the grid)
+    const bool on = d == 0 ? xface_on(P, i, j) : yface_on(P, i, j);
+    const int kR = on ? (d == 0 ? k + 1 : k + P.W) : k;
+    const double divL = d == 0 ? P.geo.dx[j] : P.geo.dyN[j];
+    const double divR = d == 0 ? P.geo.dx[j] : P.geo.dyS[on ? j + 1 : j];
     const double M = w.s[d == 0 ? S_FXM : S_FYM][k];
     double lM = w.s[d == 0 ? S_LXM : S_LYM][k];
-    if (M > 0) lM += lout[k] / divL;
-    if (M < 0) lM -= lout[kR] / divR;
+    const double loutL = lout[k], loutR = lout[kR];
     const FaceSide L = face_side(w, b, k, d, 1.0), R = face_side(w, b, kR, d, -1.0);
     const int o = d == 0 ? S_LXNL : S_LYNL;
+    const double lNL = w.s[o][k], lNR = w.s[o + 1][k], lT = w.s[o + 2][k];
+    if (!on) continue;
+    if (M > 0) lM += loutL / divL;
+    if (M < 0) lM -= loutR / divR;
     // the solver sees (normal, tangential) = (u, v) on x-faces, (v, u) on y-faces
     const FaceIn f = face_rev(L.eta, L.b, d == 0 ? L.u : L.v, d == 0 ? L.v : L.u, R.eta, R.b, d == 0 ? R.u : R.v,
-                              d == 0 ? R.v : R.u, HC, lM, w.s[o][k], w.s[o + 1][k], w.s[o + 2][k]);
+                              d == 0 ? R.v : R.u, HC, lM, lNL, lNR, lT);
     double *q = r + 6 * d;
     q[0] = f.etaL; q[1] = d == 0 ? f.uL : f.vL; q[2] = d == 0 ? f.vL : f.uL;
     q[3] = f.etaR; q[4] = d == 0 ? f.uR : f.vR; q[5] = d == 0 ? f.vR : f.uR;
@@ -353,7 +397,7 @@ __host__ __device__ inline void adj_faces_rev_at(const Params &P, const double *
   for (int q = 0; q < 12; ++q) w.s[S_LXETAL + q][k] = r[q];
 }
 
-// pass 7: per cell, gather the face-state adjoints into eta, U, V and the
+// pass 6: per cell, gather the face-state adjoints into eta, U, V and the
 // slopes; the bed (and spherical geometric) term^T and Coriolis^T
 // (swe.mjs:570-628)
 __host__ __device__ inline void adj_cells_rev_at(const Params &P, const StateP &S, const double *b, const StateP &lR,
@@ -362,21 +406,24 @@ __host__ __device__ inline void adj_cells_rev_at(const Params &P, const StateP &
   double leta = 0, lU = 0, lV = 0, lsE[2] = {0, 0}, lsU[2] = {0, 0}, lsV[2] = {0, 0};
   for (int d = 0; d < 2; ++d) {
     const int base = d == 0 ? S_LXETAL : S_LYETAL;
-    const int km = d == 0 ? k - 1 : k - W;
     const bool has_m = d == 0 ? i >= 1 : j >= 1;
+    // loaded before the branch (cell k stands in off the grid)
+    const int km = has_m ? (d == 0 ? k - 1 : k - W) : k;
+    const double pe = w.s[base][k], pu = w.s[base + 1][k], pv = w.s[base + 2][k];
+    const double me = w.s[base + 3][km], mu = w.s[base + 4][km], mv = w.s[base + 5][km];
     // + side of this cell: the face it owns; - side: the face its lower neighbour owns
-    leta += w.s[base][k]; lU += w.s[base + 1][k]; lV += w.s[base + 2][k];
-    lsE[d] += 0.5 * w.s[base][k]; lsU[d] += 0.5 * w.s[base + 1][k]; lsV[d] += 0.5 * w.s[base + 2][k];
+    leta += pe; lU += pu; lV += pv;
+    lsE[d] += 0.5 * pe; lsU[d] += 0.5 * pu; lsV[d] += 0.5 * pv;
     if (has_m) {
-      leta += w.s[base + 3][km]; lU += w.s[base + 4][km]; lV += w.s[base + 5][km];
-      lsE[d] -= 0.5 * w.s[base + 3][km]; lsU[d] -= 0.5 * w.s[base + 4][km]; lsV[d] -= 0.5 * w.s[base + 5][km];
+      leta += me; lU += mu; lV += mv;
+      lsE[d] -= 0.5 * me; lsU[d] -= 0.5 * mu; lsV[d] -= 0.5 * mv;
     }
   }
   double lhuC = 0.0, lhvC = 0.0;
+  const double lrhu = lR.hu[k], lrhv = lR.hv[k];
+  const double eta = w.s[S_ETA][k], bk = b[k];
   if (i >= 2 && i <= W - 3 && j >= 2 && j <= P.H - 3) {
-    const double lrhu = lR.hu[k], lrhv = lR.hv[k];
     if (P.order >= 2 || P.sphere) {
-      const double eta = w.s[S_ETA][k], bk = b[k];
       for (int d = 0; d < 2; ++d) {
         const double sE = w.s[S_SEX + d][k], sB = w.s[S_SBX + d][k];
         const double aP = eta + 0.5 * sE - (bk + 0.5 * sB), aM = eta - 0.5 * sE - (bk - 0.5 * sB);
@@ -416,7 +463,7 @@ __host__ __device__ inline void adj_cells_rev_at(const Params &P, const StateP &
   w.s[S_LHUC][k] = lhuC; w.s[S_LHVC][k] = lhvC;
 }
 
-// pass 8: limiter^T gathered from the slopes of the cell and its two
+// pass 7: limiter^T gathered from the slopes of the cell and its two
 // neighbours along each axis, then vel^T: adds the adjoint of the stage input
 // state at (i, j) into lS
 __host__ __device__ inline void adj_slopes_rev_at(const Params &P, const StateP &S, const Work &w, const StateW &lS,
@@ -427,21 +474,33 @@ __host__ __device__ inline void adj_slopes_rev_at(const Params &P, const StateP 
     const double *eta = w.s[S_ETA], *U = w.s[S_U], *V = w.s[S_V];
     for (int d = 0; d < 2; ++d) {
       const int st = d == 0 ? 1 : W;
+      const int pos = d == 0 ? i : j, len = d == 0 ? W : P.H;
+      // every value the three slopes below can use, loaded before any branch (a
+      // cell off the grid reads cell k instead and is never used): the five cells
+      // k + m st, m = -2..2, and the slope adjoints of the middle three
+      double e5[5], u5[5], v5[5], h5[5], lse[3], lsu[3], lsv[3];
+      for (int m = -2; m <= 2; ++m) {
+        const int kc = (pos + m < 0 || pos + m >= len) ? k : k + m * st;
+        e5[m + 2] = eta[kc]; u5[m + 2] = U[kc]; v5[m + 2] = V[kc]; h5[m + 2] = S.h[kc];
+        if (m >= -1 && m <= 1) {
+          lse[m + 1] = w.s[S_LSEX + d][kc]; lsu[m + 1] = w.s[S_LSUX + d][kc]; lsv[m + 1] = w.s[S_LSVX + d][kc];
+        }
+      }
       for (int o = -1; o <= 1; ++o) {             // slope of cell c = k + o st
-        const int c = k + o * st;
         const int ci = d == 0 ? i + o : i, cj = d == 0 ? j : j + o;
         if (ci < 1 || ci > W - 2 || cj < 1 || cj > P.H - 2) continue;
-        if (S.h[c] <= P.min_depth || S.h[c + st] <= P.min_depth || S.h[c - st] <= P.min_depth) continue;
+        const int m = o + 2;                        // c, c - st, c + st are m, m - 1, m + 1
+        if (h5[m] <= P.min_depth || h5[m + 1] <= P.min_depth || h5[m - 1] <= P.min_depth) continue;
         // s_c = lim(q_c - q_{c-st}, q_{c+st} - q_c)
-        const LimD dE = limit_deriv(P.limiter, eta[c] - eta[c - st], eta[c + st] - eta[c]);
-        const LimD dU = limit_deriv(P.limiter, U[c] - U[c - st], U[c + st] - U[c]);
-        const LimD dV = limit_deriv(P.limiter, V[c] - V[c - st], V[c + st] - V[c]);
+        const LimD dE = limit_deriv(P.limiter, e5[m] - e5[m - 1], e5[m + 1] - e5[m]);
+        const LimD dU = limit_deriv(P.limiter, u5[m] - u5[m - 1], u5[m + 1] - u5[m]);
+        const LimD dV = limit_deriv(P.limiter, v5[m] - v5[m - 1], v5[m + 1] - v5[m]);
         const double fe = o == 0 ? dE.da - dE.db : (o == 1 ? -dE.da : dE.db);
         const double fu = o == 0 ? dU.da - dU.db : (o == 1 ? -dU.da : dU.db);
         const double fv = o == 0 ? dV.da - dV.db : (o == 1 ? -dV.da : dV.db);
-        leta += w.s[S_LSEX + d][c] * fe;
-        lU += w.s[S_LSUX + d][c] * fu;
-        lV += w.s[S_LSVX + d][c] * fv;
+        leta += lse[o + 1] * fe;
+        lU += lsu[o + 1] * fu;
+        lV += lsv[o + 1] * fv;
       }
     }
   }

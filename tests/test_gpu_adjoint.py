@@ -187,3 +187,46 @@ def test_adjoint_refusals():
         adj.backward()
     with pytest.raises(swe_amd.SweError, match="interior"):
         adj.set_gauges(np.array([0], np.int32), 1, np.zeros((10, 1)))
+
+
+HOST_CASES = [
+    dict(id="mc-wetdry", limiter="mc", bc=("reflect",) * 4, wetdry=True),
+    dict(id="minmod-mixed", limiter="minmod", bc=("outflow", "reflect", "periodic", "periodic")),
+]
+
+
+@pytest.mark.parametrize("case", HOST_CASES, ids=[c["id"] for c in HOST_CASES])
+def test_adjoint_step_equals_host_composition(case):
+    # Two steps with one gauge sample after the second (so that the first step's
+    # friction sees a momentum adjoint): the device's reverse (whose face flux
+    # terms and positivity factors come from the replayed stage sweeps) against
+    # the library's host reverse passes, which recompute everything, fed with the
+    # oracle's stage states.  Equal up to the order of the atomic adds of the
+    # boundary transpose.
+    from test_adjoint_cpu import step_rev_composed
+    nx, ny, dx = 40, 32, 25.0
+    s, adj, c = make(nx, ny, dx, limiter=case["limiter"], bc=case["bc"], nsteps=2, every=2,
+                     wetdry=case.get("wetdry", False))
+    adj.forward(c["h0"], c["nf"])
+    gh, gn = adj.backward()
+    dts = adj.dts()
+    kw = dict(order=2, limiter=case["limiter"], manning=0.0)
+    o = oracle.OracleSim(nx, ny, c["rows"], dx=dx, dy=dx, use_manning_field=True, **kw)
+    desc, keep = swe_amd.make_desc(nx, ny, c["rows"], **kw)
+    o.set_manning_field(c["nf"])
+    o.set_bc(case["bc"])
+    N = c["h0"].size
+    z = np.zeros(N)
+    o.load_state(c["b"], c["h0"], z, z)
+    o.step(float(dts[0]))
+    A1 = np.array([o.h.copy(), o.hu.copy(), o.hv.copy()])
+    o.step(float(dts[1]))
+    lam = np.zeros((3, N))
+    for g, ob in zip(c["gauges"], c["obs"][0]):
+        lam[0][g] += 2.0 * (c["b"][g] + o.h[g] - ob)
+    lam, lN2 = step_rev_composed(o, desc, case["bc"], c["b"], A1, float(dts[1]), c["nf"], lam)
+    lA, lN1 = step_rev_composed(o, desc, case["bc"], c["b"], np.array([c["h0"], z, z]), float(dts[0]), c["nf"], lam)
+    lN = lN2 + lN1
+    assert np.abs(lA[0]).max() > 0 and np.abs(lN).max() > 0
+    assert np.abs(gh - lA[0]).max() <= 1e-12 * np.abs(lA[0]).max()
+    assert np.abs(gn - lN).max() <= 1e-12 * np.abs(lN).max()

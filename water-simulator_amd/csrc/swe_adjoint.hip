@@ -7,19 +7,21 @@
 //   C  = friction(dryClean(C0), n)                                      (stage 2)
 // Its reverse, given the adjoint of C on the interior: friction^T and dryClean^T,
 // the RK combination^T, residual^T of stage 2, BC^T, then the same for stage 1.
-// residual^T runs as seven one-thread-per-cell passes over the grid with their
-// intermediates in HBM (swe_adjoint.hpp).
+// residual^T runs as one-thread-per-cell passes over the grid with their
+// intermediates in HBM (swe_adjoint.hpp): on the host all seven, on the device
+// five, since the replayed stage sweeps record the face flux terms and
+// positivity factors that the other two recompute.
 //
 // Time: the forward records dt per step on the device and keeps the state every
 // K steps (K ~ sqrt(nsteps)); the reverse sweep re-runs each segment from its
 // checkpoint with the forward's own stage kernels, writing each step's
 // BC-applied stage inputs A', B' and stage 2's dryCleaned, pre-friction state
-// straight into a segment store, then steps back through the segment with no
-// further recomputation.  The replay of segment s - 1 runs on a second stream
-// into the other of two stores while segment s is reversed.  Memory: nsteps/K
-// states of 3 W H doubles, two stores of K steps of 9 W H, plus 54 + 15 W H
-// doubles of pass intermediates and scratch (about 12 GB at 1024x1024 and 4000
-// steps).
+// straight into a segment store together with both stages' face flux terms and
+// positivity factors, then steps back through the segment with no further
+// recomputation.  The replay of segment s - 1 runs on a second stream into the
+// other of two stores while segment s is reversed.  Memory: nsteps/K states of
+// 3 W H doubles, two stores of K steps of 29 W H, plus 54 + 15 W H doubles of
+// pass intermediates and scratch (about 33 GB at 1024x1024 and 4000 steps).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -54,14 +56,6 @@ __device__ __forceinline__ bool cell_of(const Params &P, int &i, int &j) {
 __global__ void k_adj_cells(Params P, StateP S, const double *b, Work w) {
   int i, j;
   if (cell_of(P, i, j)) adj_cells_at(P, S, b, w, i, j);
-}
-__global__ void k_adj_faces(Params P, const double *b, Work w) {
-  int i, j;
-  if (cell_of(P, i, j)) adj_faces_at(P, b, w, i, j);
-}
-__global__ void k_adj_theta(Params P, StateP S, double dt, Work w) {
-  int i, j;
-  if (cell_of(P, i, j)) adj_theta_at(P, S, dt, w, i, j);
 }
 __global__ void k_adj_accum(Params P, StateP S, double dt, StateP lR, Work w) {
   int i, j;
@@ -225,8 +219,13 @@ int afail(swe_adj *a, int code, const std::string &m) {
 enum { T_R, T_CUR, T_LBP, T_L0, T_L1, T_COUNT };
 // what one step of a segment keeps: the BC-applied stage inputs (B' holds stage
 // 1's dryCleaned output in its interior) and stage 2's dryCleaned, pre-friction
-// state
+// state (3 fields each), then per stage the face flux terms and positivity
+// factors its sweep computed (REC_FIELDS each, in the order of the Work slots
+// S_FXM .. S_OUT)
 enum { G_AP, G_BP, G_DC2, G_COUNT };
+constexpr int REC_FIELDS = S_OUT - S_FXM + 1;
+constexpr int STEP_FIELDS = 3 * G_COUNT + 2 * REC_FIELDS;
+static_assert(REC_FIELDS == 10, "recorded face and positivity slots");
 StateW tslot(swe_adj *a, int q) { return st3(a->tmp + (size_t)q * 3 * a->N, a->N); }
 StateW cur_state(swe_ctx *c) { return {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]}; }
 
@@ -261,18 +260,22 @@ int replay_step(swe_adj *a, int64_t n) {
   return enqueue(a);
 }
 
-int residual_rev(swe_adj *a, const StateP &S, double dt, const StateP &lR, const StateW &lS, const double *tcs) {
+// residual^T at stage input S.  rec: the face flux terms and positivity factors
+// the replayed sweep recorded for this stage (REC_FIELDS fields N apart), which
+// stand in for the face and positivity recompute passes.
+int residual_rev(swe_adj *a, const StateP &S, double dt, const StateP &lR, const StateW &lS, const double *tcs,
+                 double *rec) {
   swe_ctx *c = a->ctx;
   const Params &P = c->P;
   hipStream_t st = c->stream;
   const dim3 g = grid_for(a->N), blk(256);
-  hipLaunchKernelGGL(k_adj_cells, g, blk, 0, st, P, S, a->bed, a->w);
-  hipLaunchKernelGGL(k_adj_faces, g, blk, 0, st, P, a->bed, a->w);
-  hipLaunchKernelGGL(k_adj_theta, g, blk, 0, st, P, S, dt, a->w);
-  hipLaunchKernelGGL(k_adj_accum, g, blk, 0, st, P, S, dt, lR, a->w);
-  hipLaunchKernelGGL(k_adj_faces_rev, g, blk, 0, st, P, a->bed, a->w);
-  hipLaunchKernelGGL(k_adj_cells_rev, g, blk, 0, st, P, S, a->bed, lR, a->w, tcs);
-  hipLaunchKernelGGL(k_adj_slopes_rev, g, blk, 0, st, P, S, a->w, lS);
+  Work w = a->w;
+  for (int q = 0; q < REC_FIELDS; ++q) w.s[S_FXM + q] = rec + (size_t)q * a->N;
+  hipLaunchKernelGGL(k_adj_cells, g, blk, 0, st, P, S, a->bed, w);
+  hipLaunchKernelGGL(k_adj_accum, g, blk, 0, st, P, S, dt, lR, w);
+  hipLaunchKernelGGL(k_adj_faces_rev, g, blk, 0, st, P, a->bed, w);
+  hipLaunchKernelGGL(k_adj_cells_rev, g, blk, 0, st, P, S, a->bed, lR, w, tcs);
+  hipLaunchKernelGGL(k_adj_slopes_rev, g, blk, 0, st, P, S, w, lS);
   ACHK(a, hipGetLastError());
   return SWE_OK;
 }
@@ -287,7 +290,10 @@ int bc_rev(swe_adj *a, const StateW &l) {
 }
 
 StateW gslot(swe_adj *a, int buf, int64_t q, int g) {
-  return st3(a->seg[buf] + ((size_t)q * G_COUNT + g) * 3 * a->N, a->N);
+  return st3(a->seg[buf] + ((size_t)q * STEP_FIELDS + 3 * g) * a->N, a->N);
+}
+double *recslot(swe_adj *a, int buf, int64_t q, int stage) {
+  return a->seg[buf] + ((size_t)q * STEP_FIELDS + 3 * G_COUNT + (size_t)(stage - 1) * REC_FIELDS) * a->N;
 }
 const double *tcs_of(swe_adj *a, int buf, int64_t q) { return a->tcs_seg[buf] + (size_t)q * 2 * MAX_TIDE * 2; }
 
@@ -313,6 +319,7 @@ int replay_segment(swe_adj *a, int64_t sgi, int64_t n0, int64_t n1, int buf) {
     SweepArgs s1 = ctx_base_args(c);
     s1.in = sp(Ap);
     s1.out = Bp;
+    s1.recf = recslot(a, buf, q, 1);
     ACHK(a, launch_sweep(MODE_STAGE1, s1, st));
     ACHK(a, launch_apply_bc(P, Bp, c->b, c->dctl, st));
     SweepArgs s2 = ctx_base_args(c);
@@ -320,6 +327,7 @@ int replay_segment(swe_adj *a, int64_t sgi, int64_t n0, int64_t n1, int buf) {
     s2.h0 = sp(Ap);
     s2.out = next;
     s2.rec = dc.h;
+    s2.recf = recslot(a, buf, q, 2);
     s2.mf_part = c->mf_part + np;
     ACHK(a, launch_sweep(MODE_STAGE2, s2, st));
   }
@@ -342,12 +350,12 @@ int step_rev(swe_adj *a, int buf, int64_t q, int64_t n) {
   int rc;
   // stage 2: friction^T, dryClean^T, C0 = 0.5 (A' + B' + dt R2)^T, residual^T, BC^T
   hipLaunchKernelGGL(k_adj_epilogue, g, blk, 0, st, P, sp(dc), sp(lam), c->mfield, dt, a->lN, 2, lAp, lBp, lR);
-  if ((rc = residual_rev(a, sp(Bp), dt, sp(lR), lBp, tcs + 2 * MAX_TIDE))) return rc;
+  if ((rc = residual_rev(a, sp(Bp), dt, sp(lR), lBp, tcs + 2 * MAX_TIDE, recslot(a, buf, q, 2)))) return rc;
   if ((rc = bc_rev(a, lBp))) return rc;
   // stage 1: dryClean^T (B's interior is stage 1's dryCleaned output), B0 = A' + dt R1 ^T,
   // residual^T, BC^T
   hipLaunchKernelGGL(k_adj_epilogue, g, blk, 0, st, P, sp(Bp), sp(lBp), c->mfield, dt, a->lN, 1, lAp, none, lR);
-  if ((rc = residual_rev(a, sp(Ap), dt, sp(lR), lAp, tcs))) return rc;
+  if ((rc = residual_rev(a, sp(Ap), dt, sp(lR), lAp, tcs, recslot(a, buf, q, 1)))) return rc;
   if ((rc = bc_rev(a, lAp))) return rc;
   std::swap(a->lam, a->lap);
   return SWE_OK;
@@ -496,7 +504,7 @@ int swe_adj_create(swe_ctx *c, int64_t nsteps, int ckpt, swe_adj **out) {
   for (int q = 0; q < S_COUNT; ++q) a->w.s[q] = a->work + (size_t)q * N;
   if ((e = hipMalloc(&a->ckpt, sizeof(double) * nck * N3)) != hipSuccess) return bail("checkpoints", e);
   for (int q = 0; q < 2; ++q) {
-    if ((e = hipMalloc(&a->seg[q], sizeof(double) * (size_t)a->K * G_COUNT * N3)) != hipSuccess)
+    if ((e = hipMalloc(&a->seg[q], sizeof(double) * (size_t)a->K * STEP_FIELDS * N)) != hipSuccess)
       return bail("segment store", e);
     if ((e = hipMalloc(&a->tcs_seg[q], sizeof(double) * (size_t)a->K * 2 * MAX_TIDE * 2)) != hipSuccess)
       return bail("tide factors", e);

@@ -283,16 +283,32 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
     // ---- positivity factor at row q = s-1 (swe.mjs:507-529) -----------------
     const int q = s - 1;
     const double hq = cM.h;
-    double th = 1.0;
+    double th = 1.0, out = 0.0;
     {
       const double eM = fxPrev.M, wM = shup(fxPrev.M);
       if (posOn && slopeCol && q >= 1 && q <= H - 2) {
-        double out = 0.0;
         if (eM > 0) out += div_r(eM, rowv(DX, q), rowv(RDX, q));
         if (wM < 0) out -= div_r(wM, rowv(DX, q), rowv(RDX, q));
         if (fyCur.M > 0) out += div_r(fyCur.M, rowv(DYN, q), rowv(RDYN, q));
         if (fyPrev.M < 0) out -= div_r(fyPrev.M, rowv(DYS, q), rowv(RDYS, q));
         if (out * dt > hq) th = hq > 0 ? hq / (out * dt) : 0.0;
+      }
+    }
+    if (A.recf && outLane) {
+      // each value once, by the chunk that owns its row: x-face (gi|gi+1) of row s,
+      // y-face (s-1|s) of its owner row s-1, theta and the outflow sum of row q
+      double *R = A.recf;
+      if (s >= y0 && s < y1) {
+        const int k = s * W + gi;
+        R[k] = fxCur.M; R[k + fs] = fxCur.NL; R[k + 2 * fs] = fxCur.NR; R[k + 3 * fs] = fxCur.T;
+      }
+      if (s - 1 >= y0 && s - 1 < y1) {
+        const int k = (s - 1) * W + gi;
+        R[k + 4 * fs] = fyCur.M; R[k + 5 * fs] = fyCur.NL; R[k + 6 * fs] = fyCur.NR; R[k + 7 * fs] = fyCur.T;
+      }
+      if (q >= y0 && q < y1) {
+        const int k = q * W + gi;
+        R[k + 8 * fs] = th; R[k + 9 * fs] = out;
       }
     }
     // ---- x-face contributions at row q (swe.mjs:530-540) ---------------------

@@ -91,6 +91,10 @@ struct SweepArgs {
   Ctl *ctl;
   double *mf_part;    // one partial per wave
   double *rec;        // stage 2, optional: the dryCleaned, pre-friction interior state (adjoint replay)
+  // optional (adjoint replay): per cell, fstride apart, the M, N+pL, N+pR, T flux
+  // terms of its east x-face, those of its north y-face, its positivity factor
+  // theta and outgoing flux sum (zero / theta = 1 where the stage has none)
+  double *recf;
   int rows_per_chunk;
 };
 

@@ -179,6 +179,30 @@ def inversion(nx=1024, ny=1024, dx=50.0, seed=4, ngauges=32):
     return b.ravel(), h.ravel(), params, np.array(gauges, dtype=np.int32)
 
 
+def inversion_directions(n, nx, ny, dx, wet, seed=46):
+    """n seeded random directions (dh, dn) of the inversion's control space, for
+    finite-difference checks of its gradient: dh is 1 cm times a sum of 4 Gaussian
+    humps of the surface (amplitude U(-1,1), sigma U(1,5) km, like the truth), dn is
+    1e-3 times a low Fourier mode of the Manning field (1 to 6 waves per side, random
+    phase); both vanish off the cells where wet (a bool array over W*H) is set."""
+    rng = SplitMix64(seed)
+    W, H = nx + 4, ny + 4
+    x = (np.arange(W) - 2 + 0.5) * dx
+    y = (np.arange(H) - 2 + 0.5) * dx
+    m = np.asarray(wet, dtype=bool).reshape(H, W)
+    out = []
+    for _ in range(n):
+        dh = np.zeros((H, W))
+        for _ in range(4):
+            cx, cy = rng.uniform(0, nx * dx), rng.uniform(0, ny * dx)
+            a, sg = rng.uniform(-1, 1), rng.uniform(1000, 5000)
+            dh += 0.01 * a * np.outer(np.exp(-((y - cy) / sg) ** 2), np.exp(-((x - cx) / sg) ** 2))
+        kx, ky, ph = rng.uniform(1, 6), rng.uniform(1, 6), rng.uniform(0, 2 * math.pi)
+        dn = 1e-3 * np.outer(np.cos(2 * math.pi * ky * y / (ny * dx)), np.sin(2 * math.pi * kx * x / (nx * dx) + ph))
+        out.append(((dh * m).ravel(), (dn * m).ravel()))
+    return out
+
+
 def sphere_tides(nx=2160, ny=1080, members=16, seed=5):
     """Global ocean on the 10-arc-minute lat-lon sphere (2160x1080, R = 6371 km,
     Omega = 7.292115e-5 s^-1, the globe's metric rows): bathymetry from spectral

@@ -264,6 +264,16 @@ def sphere_measure(steps, adj_steps, every=10):
                           "plain_forward_ms": pf_ms, "gradient_ms": b_ms, "gradient_over_forward": b_ms / pf_ms,
                           "grad_h0_max": float(np.abs(gh).max()), "grad_n_max": float(np.abs(gn).max())}
         log(f"[bench] sphere adjoint (member 0, {adj_steps} steps): gradient {b_ms:.1f} ms = {b_ms / pf_ms:.2f}x forward")
+        # the 24-hour misfit of one member: memory of one- and two-level checkpointing
+        n24 = int(math.ceil(86400.0 / out["dt_min"]))
+        ncell = (nx + 4) * (ny + 4)
+        k1 = max(1, round(math.sqrt(n24)))
+        k2 = max(1, round(n24 ** (1.0 / 3.0)))
+        out["adjoint"]["plan_24h"] = {
+            "steps": n24, "one_level": {"inner": k1, "outer": k1,
+                                        "GB": swe_amd.adjoint_plan_bytes(n24, ncell, k1, k1) / 1e9},
+            "two_levels": {"inner": k2, "outer": k2 * k2,
+                           "GB": swe_amd.adjoint_plan_bytes(n24, ncell, k2, k2 * k2) / 1e9}}
         adj.close()
     s.close()
     return out

@@ -193,8 +193,18 @@ int swe_batch_autotune(swe_batch *b, int steps, int *best_rows, int *best_waves)
  * context it is created on: its state after any swe_adj_* call is that of the
  * last forward step run, and a scalar Manning n becomes a per-cell field. */
 typedef struct swe_adj swe_adj;
-/* ckpt: steps between checkpoints (0 = round(sqrt(nsteps))) */
+/* ckpt: steps between checkpoints; 0 chooses round(sqrt(nsteps)) when that plan
+ * fits in 90% of the free device memory, else two levels (swe_adj_create_levels)
+ * with inner = round(cbrt(nsteps)) and outer = inner^2 */
 int swe_adj_create(swe_ctx *ctx, int64_t nsteps, int ckpt, swe_adj **out);
+/* Two-level checkpointing: the forward keeps the state every `outer` steps (a
+ * multiple of `inner`); the backward recomputes, per outer segment, the state
+ * every `inner` steps, then replays and reverses those inner segments.  Costs one
+ * more forward pass than one level (outer == inner) and needs far less memory. */
+int swe_adj_create_levels(swe_ctx *ctx, int64_t nsteps, int64_t inner, int64_t outer, swe_adj **out);
+/* Device bytes an adjoint of nsteps steps on a grid of ncells = W*H cells needs
+ * with the given intervals (nothing is allocated) */
+int swe_adj_plan(int64_t nsteps, int64_t ncells, int64_t inner, int64_t outer, double *bytes);
 void swe_adj_destroy(swe_adj *a);
 const char *swe_adj_last_error(const swe_adj *a);
 /* cells: k = j*W + i of interior cells; obs: nsteps/every samples of ng values */

@@ -89,3 +89,23 @@ def test_geometry_errors_match_reference():
         swe_amd.geometry_spherical(9, 10)            # odd nx reaching a pole
     with pytest.raises(ValueError):
         swe_amd.geometry_spherical(8, 10, lat0=10, lat1=5)
+
+
+def test_adjoint_memory_plan():
+    # swe_adj_plan is host arithmetic: 3 W H doubles per checkpoint, two replay
+    # stores of `inner` steps, the pass slots and scratch.  The 24-hour adjoint of
+    # one 2160x1080 sphere member (about 1.48e6 steps at dt = 0.0585 s) needs two
+    # levels to fit in one MI355X's 288 GB of HBM.
+    ncells = (2160 + 4) * (1080 + 4)
+    nsteps = int(round(86400 / 0.0585))
+    k1 = int(round(nsteps ** 0.5))
+    one = swe_amd.adjoint_plan_bytes(nsteps, ncells, k1, k1)
+    k2 = int(round(nsteps ** (1 / 3)))
+    two = swe_amd.adjoint_plan_bytes(nsteps, ncells, k2, k2 * k2)
+    assert one > 288e9 > two
+    # more checkpoints cost 3 W H doubles each
+    a = swe_amd.adjoint_plan_bytes(1000, 100, 10, 10)
+    b = swe_amd.adjoint_plan_bytes(1000, 100, 10, 20)
+    assert a - b == pytest.approx(8.0 * 100 * 3 * (100 - 50) - 8.0 * 100 * 3 * 2)
+    with pytest.raises(swe_amd.SweError):
+        swe_amd.adjoint_plan_bytes(1000, 100, 10, 15)      # outer not a multiple of inner

@@ -36,7 +36,7 @@ def rough_basin(nx, ny, dx, seed, *, wetdry=False):
 
 
 def make(nx=40, ny=32, dx=25.0, *, order=2, limiter="mc", bc=("reflect",) * 4, coriolis=0.0, nsteps=48,
-         every=4, ckpt=0, wetdry=False, seed=1):
+         every=4, ckpt=0, outer=0, wetdry=False, seed=1):
     rows = swe_amd.geometry_cartesian(nx, ny, dx, coriolis=coriolis)
     s = swe_amd.Solver(nx, ny, rows, order=order, limiter=limiter, manning=0.0, coriolis=coriolis, dx=dx, dy=dx)
     b, h0 = rough_basin(nx, ny, dx, seed, wetdry=wetdry)
@@ -48,7 +48,7 @@ def make(nx=40, ny=32, dx=25.0, *, order=2, limiter="mc", bc=("reflect",) * 4, c
     W = nx + 4
     gauges = np.array([(4 + 7 * q) * W + 3 + (11 * q) % nx for q in range(4)] + [(ny // 2 + 2) * W + nx // 2 + 2],
                       dtype=np.int32)
-    adj = swe_amd.Adjoint(s, nsteps, ckpt)
+    adj = swe_amd.Adjoint(s, nsteps, ckpt, outer)
     adj.set_gauges(gauges, every, np.zeros((nsteps // every, len(gauges))))
     adj.forward(h0, nf)
     ser = adj.series()
@@ -153,6 +153,19 @@ def test_adjoint_gradient_is_independent_of_checkpointing():
     grads = []
     for ckpt in (1, 5, 48):
         s, adj, c = make(order=2, limiter="mc", nsteps=48, ckpt=ckpt)
+        adj.forward(c["h0"], c["nf"])
+        grads.append(adj.backward())
+    for gh, gn in grads[1:]:
+        np.testing.assert_allclose(gh, grads[0][0], rtol=1e-12, atol=1e-12 * np.abs(grads[0][0]).max())
+        np.testing.assert_allclose(gn, grads[0][1], rtol=1e-12, atol=1e-12 * np.abs(grads[0][1]).max())
+
+
+def test_adjoint_two_levels_equal_one_level():
+    # outer checkpoints every 12 / 9 / 15 steps, inner segments recomputed from
+    # them (48 steps: whole, ragged and single-segment outer segments)
+    grads = []
+    for inner, outer in ((4, 0), (4, 12), (3, 9), (5, 15)):
+        s, adj, c = make(order=2, limiter="mc", nsteps=48, ckpt=inner, outer=outer, wetdry=True)
         adj.forward(c["h0"], c["nf"])
         grads.append(adj.backward())
     for gh, gn in grads[1:]:

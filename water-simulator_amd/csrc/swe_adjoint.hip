@@ -173,12 +173,14 @@ int host_params(const swe_desc *d, const int32_t bc[4], Params &P) {
 struct swe_adj {
   swe_ctx *ctx = nullptr;
   int64_t nsteps = 0;
-  int K = 1;                       // checkpoint interval
+  int K = 1;                       // steps of one replayed segment
+  int64_t KO = 1;                  // steps between the checkpoints kept for the whole run (a multiple of K)
   size_t N = 0;
   double *work = nullptr;          // S_COUNT * N
   Work w{};
-  double *ckpt = nullptr;          // ceil(nsteps / K) states of 3N
-  double *seg[2] = {nullptr, nullptr};      // two segment stores, K steps x 9N each
+  double *ckpt = nullptr;          // ceil(nsteps / KO) states of 3N
+  double *ickpt = nullptr;         // two levels: the KO / K segment start states of one outer segment
+  double *seg[2] = {nullptr, nullptr};      // two segment stores, K steps x STEP_FIELDS N each
   double *tcs_seg[2] = {nullptr, nullptr};  // K steps x 2 stages x MAX_TIDE x (cos, sin), per store
   double *bed = nullptr;           // the bed with its ghost ring, read by the reverse passes
   hipStream_t rs = nullptr;        // the replay stream
@@ -215,8 +217,8 @@ int afail(swe_adj *a, int code, const std::string &m) {
   return code;
 }
 
-// scratch slots
-enum { T_R, T_CUR, T_LBP, T_L0, T_L1, T_COUNT };
+// scratch slots (T_A0..T_A2: the states of the inner-checkpoint forward)
+enum { T_R, T_CUR, T_LBP, T_L0, T_L1, T_A0, T_A1, T_A2, T_COUNT };
 // what one step of a segment keeps: the BC-applied stage inputs (B' holds stage
 // 1's dryCleaned output in its interior) and stage 2's dryCleaned, pre-friction
 // state (3 fields each), then per stage the face flux terms and positivity
@@ -297,18 +299,18 @@ double *recslot(swe_adj *a, int buf, int64_t q, int stage) {
 }
 const double *tcs_of(swe_adj *a, int buf, int64_t q) { return a->tcs_seg[buf] + (size_t)q * 2 * MAX_TIDE * 2; }
 
-// Re-run steps [n0, n1) of checkpoint segment sgi into segment store `buf` on
+// Re-run steps [n0, n1) from state src into segment store `buf` on
 // the replay stream, with the forward's stage kernels at the recorded dt: step
 // q = n - n0 reads A' = slot q (BC applied in place), writes B' = its stage-1
 // output (BC applied in place), stage 2's dryCleaned pre-friction state, and its
 // output straight into slot q + 1.  The replay touches only the store, the
 // control block and the context's bed, never what the reverse sweep reads.
-int replay_segment(swe_adj *a, int64_t sgi, int64_t n0, int64_t n1, int buf) {
+int replay_segment(swe_adj *a, const double *src, int64_t n0, int64_t n1, int buf) {
   swe_ctx *c = a->ctx;
   const Params &P = c->P;
   hipStream_t st = a->rs;
   const int np = sweep_waves(P, c->chunk);
-  if (int rc = copy3_on(a, gslot(a, buf, 0, G_AP), sp3(a->ckpt + (size_t)sgi * 3 * a->N, a->N), st)) return rc;
+  if (int rc = copy3_on(a, gslot(a, buf, 0, G_AP), sp3(src, a->N), st)) return rc;
   for (int64_t n = n0; n < n1; ++n) {
     const int64_t q = n - n0;
     const StateW Ap = gslot(a, buf, q, G_AP), Bp = gslot(a, buf, q, G_BP), dc = gslot(a, buf, q, G_DC2);
@@ -330,6 +332,44 @@ int replay_segment(swe_adj *a, int64_t sgi, int64_t n0, int64_t n1, int buf) {
     s2.recf = recslot(a, buf, q, 2);
     s2.mf_part = c->mf_part + np;
     ACHK(a, launch_sweep(MODE_STAGE2, s2, st));
+  }
+  return SWE_OK;
+}
+
+// Two levels: the start states of the K-step segments of outer segment so, from
+// its checkpoint, by the stage kernels at the recorded dt on the replay stream
+// (the states of the forward run bit for bit).
+int inner_checkpoints(swe_adj *a, int64_t so) {
+  swe_ctx *c = a->ctx;
+  const Params &P = c->P;
+  hipStream_t st = a->rs;
+  const int np = sweep_waves(P, c->chunk);
+  const size_t N3 = 3 * a->N;
+  const int64_t o0 = so * a->KO, o1 = std::min<int64_t>(a->nsteps, o0 + a->KO);
+  const int64_t nin = (o1 - o0 + a->K - 1) / a->K;
+  int rc;
+  if ((rc = copy3_on(a, st3(a->ickpt, a->N), sp3(a->ckpt + (size_t)so * N3, a->N), st))) return rc;
+  if ((rc = copy3_on(a, tslot(a, T_A0), sp3(a->ckpt + (size_t)so * N3, a->N), st))) return rc;
+  int cur = 0;
+  for (int64_t m = 1; m < nin; ++m) {
+    for (int64_t n = o0 + (m - 1) * a->K; n < o0 + m * a->K; ++n) {
+      const StateW X = tslot(a, T_A0 + cur), Y = tslot(a, T_A0 + (cur + 1) % 3), Z = tslot(a, T_A0 + (cur + 2) % 3);
+      hipLaunchKernelGGL(k_ctl_dt, dim3(1), dim3(64), 0, st, c->dctl, a->dt_dev, a->t_dev, n, (double *)nullptr);
+      ACHK(a, launch_apply_bc(P, X, c->b, c->dctl, st));
+      SweepArgs s1 = ctx_base_args(c);
+      s1.in = sp(X);
+      s1.out = Y;
+      ACHK(a, launch_sweep(MODE_STAGE1, s1, st));
+      ACHK(a, launch_apply_bc(P, Y, c->b, c->dctl, st));
+      SweepArgs s2 = ctx_base_args(c);
+      s2.in = sp(Y);
+      s2.h0 = sp(X);
+      s2.out = Z;
+      s2.mf_part = c->mf_part + np;
+      ACHK(a, launch_sweep(MODE_STAGE2, s2, st));
+      cur = (cur + 2) % 3;
+    }
+    if ((rc = copy3_on(a, st3(a->ickpt + (size_t)m * N3, a->N), sp(tslot(a, T_A0 + cur)), st))) return rc;
   }
   return SWE_OK;
 }
@@ -395,8 +435,8 @@ int forward(swe_adj *a, bool record, double *loss) {
     ACHK(a, launch_finalize(c->dctl, c->mf_part, 0, c->P.cfl, 0, c->stream));
   }
   for (int64_t n = 0; n < a->nsteps; ++n) {
-    if (record && n % a->K == 0)
-      if ((rc = copy3(a, st3(a->ckpt + (size_t)(n / a->K) * N3, a->N), sp(cur_state(c))))) return rc;
+    if (record && n % a->KO == 0)
+      if ((rc = copy3(a, st3(a->ckpt + (size_t)(n / a->KO) * N3, a->N), sp(cur_state(c))))) return rc;
     if (record) {
       if ((rc = enqueue(a))) return rc;
       hipLaunchKernelGGL(k_ctl_record, dim3(1), dim3(64), 0, c->stream, c->dctl, a->dt_dev, n);
@@ -465,7 +505,7 @@ void swe_adj_destroy(swe_adj *a) {
   // touches nothing of the context, which may already be gone; hipFree waits
   // for the device work that uses the buffers
   if (a->rs) (void)hipStreamSynchronize(a->rs);
-  (void)hipFree(a->work); (void)hipFree(a->ckpt); (void)hipFree(a->tmp);
+  (void)hipFree(a->work); (void)hipFree(a->ckpt); (void)hipFree(a->ickpt); (void)hipFree(a->tmp);
   for (int q = 0; q < 2; ++q) {
     (void)hipFree(a->seg[q]); (void)hipFree(a->tcs_seg[q]);
     if (a->replayed[q]) (void)hipEventDestroy(a->replayed[q]);
@@ -478,8 +518,34 @@ void swe_adj_destroy(swe_adj *a) {
   delete a;
 }
 
+int swe_adj_plan(int64_t nsteps, int64_t ncells, int64_t inner, int64_t outer, double *bytes) {
+  if (nsteps < 1 || ncells < 1 || inner < 1 || outer < inner || outer % inner || !bytes) return SWE_EINVAL;
+  const double nck = (double)((nsteps + outer - 1) / outer);
+  const double nin = outer > inner ? (double)((outer + inner - 1) / inner) : 0.0;
+  const double fields = 3.0 * (nck + nin) + 2.0 * (double)inner * STEP_FIELDS + S_COUNT + 3.0 * T_COUNT + 2.0;
+  *bytes = 8.0 * (double)ncells * fields + 16.0 * (double)nsteps + 2.0 * 8.0 * (double)inner * 2 * MAX_TIDE * 2;
+  return SWE_OK;
+}
+
 int swe_adj_create(swe_ctx *c, int64_t nsteps, int ckpt, swe_adj **out) {
   if (!c || !out || nsteps < 1 || ckpt < 0) return ctx_fail(c, SWE_EINVAL, "bad adjoint arguments");
+  if (ckpt > 0) return swe_adj_create_levels(c, nsteps, ckpt, ckpt, out);
+  // sqrt(nsteps) checkpointing when it fits in 90% of the free device memory,
+  // else two levels with segments of cbrt(nsteps) steps and checkpoints every
+  // cbrt(nsteps)^2 steps
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return ctx_fail(c, SWE_EHIP, "hipMemGetInfo failed");
+  const int64_t k1 = std::max<int64_t>(1, std::llround(std::sqrt((double)nsteps)));
+  double bytes = 0.0;
+  swe_adj_plan(nsteps, (int64_t)c->N, k1, k1, &bytes);
+  if (bytes <= 0.9 * (double)free_b) return swe_adj_create_levels(c, nsteps, k1, k1, out);
+  const int64_t k2 = std::max<int64_t>(1, std::llround(std::cbrt((double)nsteps)));
+  return swe_adj_create_levels(c, nsteps, k2, std::min<int64_t>(nsteps, k2 * k2) / k2 * k2, out);
+}
+
+int swe_adj_create_levels(swe_ctx *c, int64_t nsteps, int64_t inner, int64_t outer, swe_adj **out) {
+  if (!c || !out || nsteps < 1 || inner < 1 || inner > (1 << 30) || outer < inner || outer % inner)
+    return ctx_fail(c, SWE_EINVAL, "bad adjoint arguments (outer must be a multiple of inner)");
   *out = nullptr;
   for (int s = 0; s < 4; ++s)
     if (c->P.bc[s] != BC_REFLECT && c->P.bc[s] != BC_OUTFLOW && c->P.bc[s] != BC_PERIODIC)
@@ -488,12 +554,13 @@ int swe_adj_create(swe_ctx *c, int64_t nsteps, int ckpt, swe_adj **out) {
   swe_adj *a = new swe_adj();
   a->ctx = c;
   a->nsteps = nsteps;
-  a->K = ckpt > 0 ? ckpt : std::max(1, (int)std::lround(std::sqrt((double)nsteps)));
+  a->K = (int)inner;
+  a->KO = outer;
   a->N = c->N;
   a->lam = T_L0;
   a->lap = T_L1;
   const size_t N = a->N, N3 = 3 * N;
-  const size_t nck = (size_t)((nsteps + a->K - 1) / a->K);
+  const size_t nck = (size_t)((nsteps + a->KO - 1) / a->KO);
   auto bail = [&](const char *what, hipError_t e) {
     const std::string m = std::string(what) + ": " + hipGetErrorString(e);
     swe_adj_destroy(a);
@@ -503,6 +570,10 @@ int swe_adj_create(swe_ctx *c, int64_t nsteps, int ckpt, swe_adj **out) {
   if ((e = hipMalloc(&a->work, sizeof(double) * S_COUNT * N)) != hipSuccess) return bail("adjoint passes", e);
   for (int q = 0; q < S_COUNT; ++q) a->w.s[q] = a->work + (size_t)q * N;
   if ((e = hipMalloc(&a->ckpt, sizeof(double) * nck * N3)) != hipSuccess) return bail("checkpoints", e);
+  if (a->KO > a->K) {
+    const size_t nin = (size_t)((a->KO + a->K - 1) / a->K);
+    if ((e = hipMalloc(&a->ickpt, sizeof(double) * nin * N3)) != hipSuccess) return bail("inner checkpoints", e);
+  }
   for (int q = 0; q < 2; ++q) {
     if ((e = hipMalloc(&a->seg[q], sizeof(double) * (size_t)a->K * STEP_FIELDS * N)) != hipSuccess)
       return bail("segment store", e);
@@ -611,33 +682,41 @@ int swe_adj_backward(swe_adj *a, double *grad_h0, double *grad_n) {
   // the reverse passes read a private copy of the bed (ghost ring as the forward
   // leaves it); the replay stream keeps writing the context's own
   ACHK(a, hipMemcpyAsync(a->bed, c->b, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
-  // Segments are reversed last to first.  While the reverse passes of segment s
-  // (HBM-bound gathers, main stream) run, the replay of segment s - 1 (VALU-bound
-  // stage sweeps, replay stream) fills the other store.
-  const int64_t nseg = (a->nsteps + a->K - 1) / a->K;
-  auto bounds = [&](int64_t sgi, int64_t &n0, int64_t &n1) {
-    n0 = sgi * a->K;
-    n1 = std::min<int64_t>(a->nsteps, n0 + a->K);
+  // Segments of K steps are reversed last to first.  While the reverse passes of
+  // one segment (HBM-bound gathers, main stream) run, the replay of the segment
+  // before it (VALU-bound stage sweeps, replay stream) fills the other store; with
+  // two levels the replay stream first recomputes the start states of an outer
+  // segment's K-step segments, when it reaches that outer segment.
+  struct Seg { int64_t so, m, n0, n1; };
+  std::vector<Seg> segs;
+  const bool two = a->KO > a->K;
+  const int64_t nout = (a->nsteps + a->KO - 1) / a->KO;
+  for (int64_t so = nout - 1; so >= 0; --so) {
+    const int64_t o0 = so * a->KO, o1 = std::min<int64_t>(a->nsteps, o0 + a->KO);
+    for (int64_t m = (o1 - o0 + a->K - 1) / a->K - 1; m >= 0; --m)
+      segs.push_back({so, m, o0 + m * a->K, std::min<int64_t>(o1, o0 + (m + 1) * a->K)});
+  }
+  auto issue_replay = [&](size_t i) -> int {
+    const Seg &g = segs[i];
+    if (two && (i == 0 || segs[i - 1].so != g.so))
+      if (int r = inner_checkpoints(a, g.so)) return r;
+    const double *src = two ? a->ickpt + (size_t)g.m * N3 : a->ckpt + (size_t)g.so * N3;
+    return replay_segment(a, src, g.n0, g.n1, (int)(i % 2));
   };
   for (int q = 0; q < 2; ++q) ACHK(a, hipEventRecord(a->reversed[q], st));
   ACHK(a, hipStreamWaitEvent(a->rs, a->reversed[0], 0));
-  int64_t n0, n1;
-  bounds(nseg - 1, n0, n1);
-  if ((rc = replay_segment(a, nseg - 1, n0, n1, (int)((nseg - 1) % 2)))) return rc;
-  ACHK(a, hipEventRecord(a->replayed[(nseg - 1) % 2], a->rs));
-  for (int64_t sgi = nseg - 1; sgi >= 0; --sgi) {
-    const int buf = (int)(sgi % 2);
+  if ((rc = issue_replay(0))) return rc;
+  ACHK(a, hipEventRecord(a->replayed[0], a->rs));
+  for (size_t i = 0; i < segs.size(); ++i) {
+    const int buf = (int)(i % 2);
     ACHK(a, hipStreamWaitEvent(st, a->replayed[buf], 0));
-    if (sgi > 0) {
-      int64_t m0, m1;
-      bounds(sgi - 1, m0, m1);
+    if (i + 1 < segs.size()) {
       ACHK(a, hipStreamWaitEvent(a->rs, a->reversed[1 - buf], 0));
-      if ((rc = replay_segment(a, sgi - 1, m0, m1, 1 - buf))) return rc;
+      if ((rc = issue_replay(i + 1))) return rc;
       ACHK(a, hipEventRecord(a->replayed[1 - buf], a->rs));
     }
-    bounds(sgi, n0, n1);
-    for (int64_t n = n1 - 1; n >= n0; --n) {
-      if ((rc = step_rev(a, buf, n - n0, n))) return rc;
+    for (int64_t n = segs[i].n1 - 1; n >= segs[i].n0; --n) {
+      if ((rc = step_rev(a, buf, n - segs[i].n0, n))) return rc;
       if ((rc = inject(a, n))) return rc;
     }
     ACHK(a, hipEventRecord(a->reversed[buf], st));

@@ -580,6 +580,8 @@ def _bind_adjoint(L):
     D = ctypes.POINTER(Desc)
     sig = {
         "swe_adj_create": (i, [vp, i64, i, ctypes.POINTER(vp)]),
+        "swe_adj_create_levels": (i, [vp, i64, i64, i64, ctypes.POINTER(vp)]),
+        "swe_adj_plan": (i, [i64, i64, i64, i64, ctypes.POINTER(ctypes.c_double)]),
         "swe_adj_destroy": (None, [vp]),
         "swe_adj_last_error": (ctypes.c_char_p, [vp]),
         "swe_adj_set_gauges": (i, [vp, i, ip, i, dp]),
@@ -668,19 +670,37 @@ def epilogue_rev_host(desc, stage, pre, nfield, dt, lout):
     return out[0], out[1], out[2], lN
 
 
+def adjoint_plan_bytes(nsteps, ncells, inner, outer):
+    """Device bytes of an adjoint of nsteps steps over ncells = W*H cells with
+    replayed segments of `inner` steps and checkpoints every `outer` steps."""
+    L = lib()
+    _bind_adjoint(L)
+    out = ctypes.c_double()
+    rc = L.swe_adj_plan(int(nsteps), int(ncells), int(inner), int(outer), ctypes.byref(out))
+    if rc != 0:
+        raise SweError("swe_adj_plan: bad arguments")
+    return out.value
+
+
 class Adjoint:
     """Gradients of a gauge misfit through `nsteps` self-timed steps of a Solver
     (swe_adj_* of the C ABI): J = sum_s sum_g (eta_g(t_s) - obs[s, g])^2, eta
     sampled every `every` steps; dJ/dh0 (initial depth with zero momentum) and
     dJ/dn (per-cell Manning n).  The dt sequence is that of the last forward()."""
 
-    def __init__(self, solver, nsteps, ckpt=0):
+    def __init__(self, solver, nsteps, ckpt=0, outer=0):
+        """ckpt: steps per replayed segment (0: chosen by the library); outer: steps
+        between the checkpoints kept for the whole run, a multiple of ckpt (0: one
+        level, outer = ckpt)."""
         L = lib()
         _bind_adjoint(L)
         self.solver = solver
         self.nsteps = int(nsteps)
         h = ctypes.c_void_p()
-        rc = L.swe_adj_create(solver._h, self.nsteps, int(ckpt), ctypes.byref(h))
+        if outer:
+            rc = L.swe_adj_create_levels(solver._h, self.nsteps, int(ckpt), int(outer), ctypes.byref(h))
+        else:
+            rc = L.swe_adj_create(solver._h, self.nsteps, int(ckpt), ctypes.byref(h))
         if rc != 0:
             raise SweError(f"swe_adj_create: {L.swe_last_error(solver._h).decode()}")
         self._h = h

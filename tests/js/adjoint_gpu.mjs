@@ -70,5 +70,32 @@ function gradientCheck(label, sim, nsteps, gauges, every, mkField) {
   gradientCheck('sphere + tide', sim, 40, gauges, 4, (N) => Float64Array.from({ length: N }, () => 0.02));
 }
 
+{
+  // two-level checkpointing (outer checkpoints every 12 steps, segments of 4
+  // recomputed from them) gives the one-level gradient
+  const mk = () => {
+    const r = xorshift(6);                      // the same bed for both runs
+    return new ShallowWater({ nx: 32, ny: 24, dx: 25, manning: 0.025,
+      bed: () => -4 + 0.1 * r(), eta0: (x, y) => 0.2 * Math.exp(-((x - 300) ** 2 + (y - 250) ** 2) / 150 ** 2) });
+  };
+  const grads = [];
+  for (const opts of [{ checkpointEvery: 4 }, { checkpointEvery: 4, outerEvery: 12 }]) {
+    const sim = mk();
+    const W = sim.W;
+    const gauges = [5 * W + 7, 12 * W + 20, 20 * W + 30];
+    const adj = new Adjoint(sim, 48, opts);
+    const h0 = Float64Array.from(sim.h);
+    adj.setGauges(gauges, 4, new Float64Array(12 * gauges.length).fill(0.01));
+    adj.forward(h0, null);
+    grads.push(adj.backward());
+  }
+  let d = 0, m = 0;
+  for (let k = 0; k < grads[0].dh0.length; k++) {
+    d = Math.max(d, Math.abs(grads[0].dh0[k] - grads[1].dh0[k]));
+    m = Math.max(m, Math.abs(grads[0].dh0[k]));
+  }
+  report('two-level checkpointing gives the one-level dJ/dh0', m > 0 && d <= 1e-12 * m, `max diff ${d} of ${m}`);
+}
+
 console.log(`\n${fails === 0 ? 'ALL PASS' : 'FAILURES'} ${checks - fails}/${checks}`);
 process.exit(fails === 0 ? 0 : 1);

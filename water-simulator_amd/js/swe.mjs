@@ -363,13 +363,16 @@ export class ShallowWater {
  * solver's device state, so `sim.h` etc. are re-uploaded by its next call.
  */
 export class Adjoint {
-  constructor(sim, nsteps, { checkpointEvery = 0 } = {}) {
+  // checkpointEvery: steps per replayed segment (0: chosen by the library);
+  // outerEvery: steps between the checkpoints kept for the whole run, a multiple of
+  // checkpointEvery (0: one level)
+  constructor(sim, nsteps, { checkpointEvery = 0, outerEvery = 0 } = {}) {
     this.sim = sim;
     this.nsteps = nsteps;
     this.ng = 0;
     this.every = 1;
     this._prepare();
-    this._adj = addon.adjCreate(sim._dev, nsteps, checkpointEvery);
+    this._adj = addon.adjCreate(sim._dev, nsteps, checkpointEvery, outerEvery);
   }
 
   _prepare() {

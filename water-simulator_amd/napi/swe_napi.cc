@@ -342,15 +342,21 @@ napi_value throw_adj(napi_env env, swe_adj *a, const char *what) {
 
 void finalize_adj(napi_env, void *data, void *) { swe_adj_destroy(static_cast<swe_adj *>(data)); }
 
-// adjCreate(h, nsteps, checkpointEvery) -> adjoint handle
+// adjCreate(h, nsteps, checkpointEvery, outerEvery = 0) -> adjoint handle; a
+// nonzero outerEvery (a multiple of checkpointEvery) selects two-level checkpointing
 napi_value AdjCreate(napi_env env, napi_callback_info info) {
-  napi_value argv[3];
-  if (!get_args(env, info, 3, argv)) return nullptr;
+  napi_value argv[4];
+  if (!get_args(env, info, 4, argv)) return nullptr;
   swe_ctx *c = get_ctx(env, argv[0]);
   if (!c) return nullptr;
+  napi_valuetype t4;
+  NAPI_OK(env, napi_typeof(env, argv[3], &t4));
+  const int64_t outer = t4 == napi_number ? (int64_t)get_double(env, argv[3]) : 0;
+  const int64_t nsteps = (int64_t)get_double(env, argv[1]);
+  const int inner = get_int(env, argv[2]);
   swe_adj *a = nullptr;
-  if (swe_adj_create(c, (int64_t)get_double(env, argv[1]), get_int(env, argv[2]), &a) != SWE_OK)
-    return throw_swe(env, c, "swe_adj_create");
+  const int rc = outer > 0 ? swe_adj_create_levels(c, nsteps, inner, outer, &a) : swe_adj_create(c, nsteps, inner, &a);
+  if (rc != SWE_OK) return throw_swe(env, c, "swe_adj_create");
   napi_value ext;
   NAPI_OK(env, napi_create_external(env, a, finalize_adj, nullptr, &ext));
   return ext;

@@ -609,14 +609,8 @@ __device__ __forceinline__ double harm_eval(int n, double c0, const double *amp,
 // wavefront per basin).
 //   phase 0: after k_maxdt -- dt for the step about to run
 //   phase 1: after a step -- massFloored, t, steps, dt_next, next dt and forcing
-__device__ __forceinline__ void finalize_body(Ctl *ctl, const double *mf_part, int nparts, double cfl, int phase) {
-  const int lane = threadIdx.x & 63;
-  double s1 = 0.0, s2 = 0.0;
-  if (phase == 1) {
-    for (int i = lane; i < nparts; i += 64) { s1 += mf_part[i]; s2 += mf_part[nparts + i]; }
-    for (int off = 32; off >= 1; off >>= 1) { s1 += __shfl_down(s1, off, 64); s2 += __shfl_down(s2, off, 64); }
-  }
-  if (lane != 0) return;
+// s1, s2: the two stages' massFloored sums (phase 1); the body runs on one lane.
+__device__ __forceinline__ void finalize_ctl(Ctl *ctl, double s1, double s2, double cfl, int phase) {
   const unsigned long long bits = ctl->cfl_bits;
   const double inv = __longlong_as_double((long long)bits);
   const double dtn = bits ? cfl / inv : __builtin_inf();
@@ -646,8 +640,38 @@ __device__ __forceinline__ void finalize_body(Ctl *ctl, const double *mf_part, i
   }
 }
 
-__global__ void k_finalize(Ctl *ctl, const double *mf_part, int nparts, double cfl, int phase) {
-  finalize_body(ctl, mf_part, nparts, cfl, phase);
+// massFloored partials of both stages summed in a fixed order by one wavefront
+// (batched basins: few partials each)
+__device__ __forceinline__ void finalize_body(Ctl *ctl, const double *mf_part, int nparts, double cfl, int phase) {
+  const int lane = threadIdx.x & 63;
+  double s1 = 0.0, s2 = 0.0;
+  if (phase == 1) {
+    for (int i = lane; i < nparts; i += 64) { s1 += mf_part[i]; s2 += mf_part[nparts + i]; }
+    for (int off = 32; off >= 1; off >>= 1) { s1 += __shfl_down(s1, off, 64); s2 += __shfl_down(s2, off, 64); }
+  }
+  if (lane == 0) finalize_ctl(ctl, s1, s2, cfl, phase);
+}
+
+// One grid's partials (thousands per stage) by FIN_THREADS threads: strided
+// per-thread sums, a shuffle tree per wavefront, then the wavefronts' sums in
+// order, so the result is the same on every run.
+constexpr int FIN_THREADS = 1024;
+__global__ __launch_bounds__(FIN_THREADS) void k_finalize(Ctl *ctl, const double *mf_part, int nparts, double cfl,
+                                                          int phase) {
+  __shared__ double w1[FIN_THREADS / 64], w2[FIN_THREADS / 64];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  double s1 = 0.0, s2 = 0.0;
+  if (phase == 1) {
+    for (int i = t; i < nparts; i += FIN_THREADS) { s1 += mf_part[i]; s2 += mf_part[nparts + i]; }
+    for (int off = 32; off >= 1; off >>= 1) { s1 += __shfl_down(s1, off, 64); s2 += __shfl_down(s2, off, 64); }
+    if (lane == 0) { w1[wv] = s1; w2[wv] = s2; }
+    __syncthreads();
+    if (t == 0) {
+      s1 = 0.0; s2 = 0.0;
+      for (int q = 0; q < FIN_THREADS / 64; ++q) { s1 += w1[q]; s2 += w2[q]; }
+    }
+  }
+  if (t == 0) finalize_ctl(ctl, s1, s2, cfl, phase);
 }
 
 // ---- batched ensembles: one launch per phase for all basins -----------------
@@ -732,7 +756,7 @@ hipError_t launch_maxdt(const Params &P, StateP S, Ctl *ctl, hipStream_t st) {
 }
 
 hipError_t launch_finalize(Ctl *ctl, const double *mf_part, int nparts, double cfl, int phase, hipStream_t st) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, st, ctl, mf_part, nparts, cfl, phase);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, st, ctl, mf_part, nparts, cfl, phase);
   return hipGetLastError();
 }
 

@@ -20,6 +20,8 @@ sys.path.insert(0, os.path.join(ROOT, "water-simulator_amd"))
 
 METRIC = "FP64 cell-updates/s + % HBM peak; batched vs plain; adjoint/forward time"
 HBM_PEAK_GBS = 8000.0
+# FP64 VALU issue: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per 4 cycles
+VALU_PEAK_WAVE_INSTR_S = 256 * 4 * 2.4e9 / 4
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -51,18 +53,24 @@ def cpu_baseline(seconds=12.0):
                 sample=f"oracle C restatement, 256x256 sample of the 4096x4096 recipe, {n} self-timed steps in {el:.1f} s")
 
 
-def committed_traffic(kernel):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary (FETCH_SIZE and
-    WRITE_SIZE, each scaled by a plain-copy calibration), or None."""
+def committed_figure(name, kernel):
+    """A per-launch figure of the committed rocprofv3 summary of the 4096x4096
+    workload at 94 rows per chunk, or None: traffic_bytes_per_launch (FETCH_SIZE and
+    WRITE_SIZE, each scaled by a plain-copy calibration) or
+    valu_wave_instructions_per_launch (SQ_INSTS_VALU)."""
     path = os.path.join(ROOT, "profiles", "k_sweep_large_grid_4096.md")
     if not os.path.exists(path):
         return None
-    key = f"traffic_bytes_per_launch {kernel}:"
+    key = f"{name} {kernel}:"
     with open(path) as f:
         for line in f:
             if line.startswith(key):
                 return float(line.split(":", 1)[1])
     return None
+
+
+def committed_traffic(kernel):
+    return committed_figure("traffic_bytes_per_launch", kernel)
 
 
 def p_dx():
@@ -332,6 +340,14 @@ def main():
     bytes_s2 = 80.0 * W * H            # reads A(h,hu,hv) + B(h,hu,hv) + b, writes C(h,hu,hv)
     achieved = bytes_s2 / (ms[1] * 1e-3) / 1e9
     traffic = committed_traffic("k_sweep<2>")
+    valu = None
+    i1, i2 = committed_figure("valu_wave_instructions_per_launch", "k_sweep<1>"), \
+        committed_figure("valu_wave_instructions_per_launch", "k_sweep<2>")
+    if i1 and i2 and sim.chunk_rows() == 94:
+        rate = (i1 + i2) / ((ms[0] + ms[1]) * 1e-3)
+        valu = {"kernels": "k_sweep<stage1> + k_sweep<stage2>", "achieved": rate, "peak": VALU_PEAK_WAVE_INSTR_S,
+                "unit": "wave64 FP64-issue instr/s", "frac": rate / VALU_PEAK_WAVE_INSTR_S,
+                "instructions_per_step": i1 + i2}
     finite = sim.get_scalar(swe_amd.S_NONFINITE) == 0
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": n,
@@ -344,6 +360,9 @@ def main():
         "kernel_ms": {"stage1": ms[0], "stage2": ms[1], "finalize": ms[2]},
         "roofline": {"bound": "hbm", "kernel": "k_sweep<stage2>", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        # what actually bounds the stage kernels: FP64 VALU issue (instructions per
+        # launch from the committed profile, which holds for its 94-row chunks)
+        "valu": valu,
         "finite": finite,
     }
     sim.close()

@@ -16,6 +16,7 @@
  */
 #ifndef SWE_AMD_H
 #define SWE_AMD_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -82,6 +83,13 @@ void swe_hllc(double out[3], double hL, double huL, double hvL, double hR, doubl
               double dry, double g);
 /* version string and the gfx target the library was built for */
 const char *swe_build_info(void);
+
+/* Page-locked host buffers (no reference counterpart: the reference's arrays are
+ * ordinary Float64Arrays).  Fields kept in them move between host and device at
+ * full DMA speed in swe_set_field / swe_get_field; the Node drop-in backs the
+ * public arrays of ShallowWater with them.  Contents are not cleared. */
+int swe_host_alloc(size_t bytes, void **out);
+void swe_host_free(void *p);
 
 /* Host <-> device copies of whole fields (the JS object's Float64Array mirrors).
  * SWE_F_MANNING: setManningField (swe.mjs:713-731); NULL returns to the scalar.

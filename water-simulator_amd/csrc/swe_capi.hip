@@ -74,12 +74,14 @@ static int upload_row(swe_ctx *c, int row, const double *host) {
 static int default_chunk(const Params &P) {
   const int strips = (P.W + SWEEP_OUT - 1) / SWEEP_OUT;
   // about two rounds of the 256 CU x 4 SIMD x 3 resident wavefronts the
-  // stage kernels' registers allow; swe_autotune refines this on the device
+  // stage kernels' registers allow, and the shortest chunks on grids too small
+  // to fill them (each wave marches its rows one after another, so a step of a
+  // small grid takes time in proportion to the chunk height); swe_autotune
+  // refines this on the device
   const int target_waves = 6144;
   int chunks = (target_waves + strips - 1) / strips;
   if (chunks < 1) chunks = 1;
   int rows = (P.H + chunks - 1) / chunks;
-  if (rows < 24) rows = 24;
   if (rows > P.H) rows = P.H;
   if (rows < MIN_CHUNK) rows = MIN_CHUNK;
   return rows;
@@ -105,6 +107,22 @@ void swe_hllc(double out[3], double hL, double huL, double hvL, double hR, doubl
 }
 
 const char *swe_build_info(void) { return "water-simulator_amd 0.1 gfx950 fp64"; }
+
+int swe_host_alloc(size_t bytes, void **out) {
+  if (!out) return fail(nullptr, SWE_EINVAL, "null output pointer");
+  *out = nullptr;
+  const hipError_t e = hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *out = nullptr;
+    return fail(nullptr, e == hipErrorOutOfMemory ? SWE_ENOMEM : SWE_EHIP,
+                std::string("hipHostMalloc: ") + hipGetErrorString(e));
+  }
+  return SWE_OK;
+}
+
+void swe_host_free(void *p) {
+  if (p) (void)hipHostFree(p);
+}
 
 const char *swe_last_error(const swe_ctx *ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
 
@@ -588,7 +606,7 @@ int swe_autotune(swe_ctx *c, int reps, int *best_rows) {
   HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_stage1_ptr(), SWEEP_LANES, 0));
   const int slots = cus * (per_cu > 0 ? per_cu : 4);
   const int strips = (c->P.W + SWEEP_OUT - 1) / SWEEP_OUT;
-  std::vector<int> cand = {16, 24, 32, 40, 48, 64, 96};
+  std::vector<int> cand = {8, 12, 16, 24, 32, 40, 48, 64, 96};
   for (int rounds = 1; rounds <= 4; ++rounds) {
     const int chunks = (rounds * slots + strips - 1) / strips;
     if (chunks >= 1) cand.push_back((c->P.H + chunks - 1) / chunks);

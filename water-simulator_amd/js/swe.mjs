@@ -55,12 +55,16 @@ export class ShallowWater {
     this.W = nx + 2 * this.ng;
     this.H = ny + 2 * this.ng;
     const N = this.W * this.H;
-    this.b = new Float64Array(N);
-    this.h = new Float64Array(N);
-    this.hu = new Float64Array(N);
-    this.hv = new Float64Array(N);
-    this.rh = new Float64Array(N); this.rhu = new Float64Array(N); this.rhv = new Float64Array(N);
-    this.theta = new Float64Array(N).fill(1);
+    // the public fields are ordinary Float64Arrays over page-locked host memory, so
+    // the per-call uploads and downloads of strict mode run at full DMA speed (an
+    // array a caller substitutes for one of them works too, at pageable speed)
+    const f64 = (n) => addon.allocPinned(n);
+    this.b = f64(N);
+    this.h = f64(N);
+    this.hu = f64(N);
+    this.hv = f64(N);
+    this.rh = f64(N); this.rhu = f64(N); this.rhv = f64(N);
+    this.theta = f64(N).fill(1);
     for (let j = 0; j < this.H; j++) {
       for (let i = 0; i < this.W; i++) {
         const k = j * this.W + i;

@@ -342,6 +342,39 @@ napi_value throw_adj(napi_env env, swe_adj *a, const char *what) {
 
 void finalize_adj(napi_env, void *data, void *) { swe_adj_destroy(static_cast<swe_adj *>(data)); }
 
+void finalize_pinned(napi_env env, void *data, void *hint) {
+  int64_t change = 0;
+  napi_adjust_external_memory(env, -(int64_t)(reinterpret_cast<uintptr_t>(hint)), &change);
+  swe_host_free(data);
+}
+
+// allocPinned(n) -> Float64Array of n zeros in page-locked host memory (freed when
+// the array is collected; V8 is told of the external bytes)
+napi_value AllocPinned(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  const double nd = get_double(env, argv[0]);
+  if (!(nd >= 0) || nd > 9.0e15) {
+    napi_throw_range_error(env, nullptr, "swe_napi: bad array length");
+    return nullptr;
+  }
+  const size_t n = (size_t)nd, bytes = n * sizeof(double);
+  void *p = nullptr;
+  if (swe_host_alloc(bytes, &p) != SWE_OK) return throw_swe(env, nullptr, "swe_host_alloc");
+  std::memset(p, 0, bytes);
+  napi_value ab, ta;
+  if (napi_create_external_arraybuffer(env, p, bytes, finalize_pinned, reinterpret_cast<void *>((uintptr_t)bytes),
+                                       &ab) != napi_ok) {
+    swe_host_free(p);
+    napi_throw_error(env, nullptr, "swe_napi: napi_create_external_arraybuffer failed");
+    return nullptr;
+  }
+  int64_t change = 0;
+  napi_adjust_external_memory(env, (int64_t)bytes, &change);
+  NAPI_OK(env, napi_create_typedarray(env, napi_float64_array, n, ab, 0, &ta));
+  return ta;
+}
+
 // adjCreate(h, nsteps, checkpointEvery, outerEvery = 0) -> adjoint handle; a
 // nonzero outerEvery (a multiple of checkpointEvery) selects two-level checkpointing
 napi_value AdjCreate(napi_env env, napi_callback_info info) {
@@ -458,6 +491,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"jsPow", JsPow}, {"buildInfo", BuildInfo}, {"setTide", SetTide},
       {"adjCreate", AdjCreate}, {"adjSetGauges", AdjSetGauges}, {"adjForward", AdjForward}, {"adjLoss", AdjLoss},
       {"adjBackward", AdjBackward}, {"adjSeries", AdjSeries}, {"adjTiming", AdjTiming},
+      {"allocPinned", AllocPinned},
   };
   for (auto &f : fns) {
     napi_value fn;

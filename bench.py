@@ -320,7 +320,7 @@ def main():
     if args.chunk > 0:
         sim.set_chunk_rows(args.chunk)
     elif args.chunk < 0:
-        log(f"[bench] autotuned chunk rows: {sim.autotune()}")
+        log(f"[bench] autotuned chunk rows: {sim.autotune(reps=10)}")
     sim.use_graph(True)
     log(f"[bench] warmup {args.warmup} steps")
     sim.advance(args.warmup)

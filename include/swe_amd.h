@@ -203,7 +203,8 @@ int swe_batch_autotune(swe_batch *b, int steps, int *best_rows, int *best_waves)
 typedef struct swe_adj swe_adj;
 /* ckpt: steps between checkpoints; 0 chooses round(sqrt(nsteps)) when that plan
  * fits in 90% of the free device memory, else two levels (swe_adj_create_levels)
- * with inner = round(cbrt(nsteps)) and outer = inner^2 */
+ * with inner up to round(cbrt(nsteps)) (smaller until the plan fits) and
+ * outer = inner^2 */
 int swe_adj_create(swe_ctx *ctx, int64_t nsteps, int ckpt, swe_adj **out);
 /* Two-level checkpointing: the forward keeps the state every `outer` steps (a
  * multiple of `inner`); the backward recomputes, per outer segment, the state

@@ -9,8 +9,8 @@
 // the RK combination^T, residual^T of stage 2, BC^T, then the same for stage 1.
 // residual^T runs as one-thread-per-cell passes over the grid with their
 // intermediates in HBM (swe_adjoint.hpp): on the host all seven, on the device
-// five, since the replayed stage sweeps record the face flux terms and
-// positivity factors that the other two recompute.
+// four, since the replayed stage sweeps record the velocities, surface, slopes,
+// face flux terms and positivity factors that the other three recompute.
 //
 // Time: the forward records dt per step on the device and keeps the state every
 // K steps (K ~ sqrt(nsteps)); the reverse sweep re-runs each segment from its
@@ -20,8 +20,8 @@
 // positivity factors, then steps back through the segment with no further
 // recomputation.  The replay of segment s - 1 runs on a second stream into the
 // other of two stores while segment s is reversed.  Memory: nsteps/K states of
-// 3 W H doubles, two stores of K steps of 29 W H, plus 54 + 15 W H doubles of
-// pass intermediates and scratch (about 33 GB at 1024x1024 and 4000 steps).
+// 3 W H doubles, two stores of K steps of 51 W H, plus 54 + 24 W H doubles of
+// pass intermediates and scratch (about 56 GB at 1024x1024 and 4000 steps).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -53,10 +53,6 @@ __device__ __forceinline__ bool cell_of(const Params &P, int &i, int &j) {
   return j < P.H;
 }
 
-__global__ void k_adj_cells(Params P, StateP S, const double *b, Work w) {
-  int i, j;
-  if (cell_of(P, i, j)) adj_cells_at(P, S, b, w, i, j);
-}
 __global__ void k_adj_accum(Params P, StateP S, double dt, StateP lR, Work w) {
   int i, j;
   if (cell_of(P, i, j)) adj_accum_at(P, S, dt, lR, w, i, j);
@@ -221,13 +217,12 @@ int afail(swe_adj *a, int code, const std::string &m) {
 enum { T_R, T_CUR, T_LBP, T_L0, T_L1, T_A0, T_A1, T_A2, T_COUNT };
 // what one step of a segment keeps: the BC-applied stage inputs (B' holds stage
 // 1's dryCleaned output in its interior) and stage 2's dryCleaned, pre-friction
-// state (3 fields each), then per stage the face flux terms and positivity
-// factors its sweep computed (REC_FIELDS each, in the order of the Work slots
-// S_FXM .. S_OUT)
+// state (3 fields each), then per stage the cell fields, face flux terms and
+// positivity factors its sweep computed (REC_FIELDS each, in the order of the
+// Work slots S_U .. S_OUT)
 enum { G_AP, G_BP, G_DC2, G_COUNT };
-constexpr int REC_FIELDS = S_OUT - S_FXM + 1;
 constexpr int STEP_FIELDS = 3 * G_COUNT + 2 * REC_FIELDS;
-static_assert(REC_FIELDS == 10, "recorded face and positivity slots");
+static_assert(REC_FIELDS == S_OUT - S_U + 1 && S_U == 0, "recorded slots are Work slots S_U .. S_OUT");
 StateW tslot(swe_adj *a, int q) { return st3(a->tmp + (size_t)q * 3 * a->N, a->N); }
 StateW cur_state(swe_ctx *c) { return {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]}; }
 
@@ -262,9 +257,9 @@ int replay_step(swe_adj *a, int64_t n) {
   return enqueue(a);
 }
 
-// residual^T at stage input S.  rec: the face flux terms and positivity factors
-// the replayed sweep recorded for this stage (REC_FIELDS fields N apart), which
-// stand in for the face and positivity recompute passes.
+// residual^T at stage input S.  rec: the cell fields, face flux terms and
+// positivity factors the replayed sweep recorded for this stage (REC_FIELDS
+// fields N apart), which stand in for the three forward recompute passes.
 int residual_rev(swe_adj *a, const StateP &S, double dt, const StateP &lR, const StateW &lS, const double *tcs,
                  double *rec) {
   swe_ctx *c = a->ctx;
@@ -272,8 +267,7 @@ int residual_rev(swe_adj *a, const StateP &S, double dt, const StateP &lR, const
   hipStream_t st = c->stream;
   const dim3 g = grid_for(a->N), blk(256);
   Work w = a->w;
-  for (int q = 0; q < REC_FIELDS; ++q) w.s[S_FXM + q] = rec + (size_t)q * a->N;
-  hipLaunchKernelGGL(k_adj_cells, g, blk, 0, st, P, S, a->bed, w);
+  for (int q = 0; q < REC_FIELDS; ++q) w.s[S_U + q] = rec + (size_t)q * a->N;
   hipLaunchKernelGGL(k_adj_accum, g, blk, 0, st, P, S, dt, lR, w);
   hipLaunchKernelGGL(k_adj_faces_rev, g, blk, 0, st, P, a->bed, w);
   hipLaunchKernelGGL(k_adj_cells_rev, g, blk, 0, st, P, S, a->bed, lR, w, tcs);
@@ -531,16 +525,23 @@ int swe_adj_create(swe_ctx *c, int64_t nsteps, int ckpt, swe_adj **out) {
   if (!c || !out || nsteps < 1 || ckpt < 0) return ctx_fail(c, SWE_EINVAL, "bad adjoint arguments");
   if (ckpt > 0) return swe_adj_create_levels(c, nsteps, ckpt, ckpt, out);
   // sqrt(nsteps) checkpointing when it fits in 90% of the free device memory,
-  // else two levels with segments of cbrt(nsteps) steps and checkpoints every
-  // cbrt(nsteps)^2 steps
+  // else two levels with segments of up to cbrt(nsteps) steps and checkpoints
+  // every segment length squared
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return ctx_fail(c, SWE_EHIP, "hipMemGetInfo failed");
   const int64_t k1 = std::max<int64_t>(1, std::llround(std::sqrt((double)nsteps)));
   double bytes = 0.0;
   swe_adj_plan(nsteps, (int64_t)c->N, k1, k1, &bytes);
   if (bytes <= 0.9 * (double)free_b) return swe_adj_create_levels(c, nsteps, k1, k1, out);
-  const int64_t k2 = std::max<int64_t>(1, std::llround(std::cbrt((double)nsteps)));
-  return swe_adj_create_levels(c, nsteps, k2, std::min<int64_t>(nsteps, k2 * k2) / k2 * k2, out);
+  // two levels: the largest inner interval up to cbrt(nsteps) whose plan fits
+  int64_t k2 = std::max<int64_t>(1, std::llround(std::cbrt((double)nsteps)));
+  auto outer_of = [&](int64_t k) { return std::max<int64_t>(k, std::min<int64_t>(nsteps, k * k) / k * k); };
+  while (k2 > 1) {
+    swe_adj_plan(nsteps, (int64_t)c->N, k2, outer_of(k2), &bytes);
+    if (bytes <= 0.9 * (double)free_b) break;
+    k2 = k2 * 3 / 4;
+  }
+  return swe_adj_create_levels(c, nsteps, k2, outer_of(k2), out);
 }
 
 int swe_adj_create_levels(swe_ctx *c, int64_t nsteps, int64_t inner, int64_t outer, swe_adj **out) {

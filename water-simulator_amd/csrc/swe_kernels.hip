@@ -157,7 +157,8 @@ __device__ __forceinline__ FluxF face_flux(const Face4 &L, const Face4 &R, const
 // One wavefront's share of a stage: strip `strip` (58 output columns) and rows
 // [chunk*rows_per_chunk, +rows_per_chunk) of the basin described by A.  wave_id
 // indexes the massFloored partial this wavefront owns.
-template <int MODE>
+// REC: compile in the optional recording for the adjoint replay (A.recf)
+template <int MODE, bool REC>
 __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int chunk, int wave_id) {
   const Params &P = A.P;
   const int W = P.W, H = P.H;
@@ -254,6 +255,14 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
         sVy = limit(P.limiter, c0.V - cM.V, cP.V - c0.V);
       }
     }
+    if (REC && A.recf && outLane && s >= y0 && s < y1) {
+      // the cell fields of (gi, s), recorded for the adjoint replay
+      double *R = A.recf;
+      const int k = s * W + gi;
+      R[k] = c0.U; R[k + fs] = c0.V; R[k + 2 * fs] = c0.eta;
+      R[k + 3 * fs] = sEx; R[k + 4 * fs] = sEy; R[k + 5 * fs] = sUx; R[k + 6 * fs] = sUy;
+      R[k + 7 * fs] = sVx; R[k + 8 * fs] = sVy; R[k + 9 * fs] = sBx; R[k + 10 * fs] = sBy;
+    }
     // reconstructed face states of cell (gi,s) (swe.mjs:418-427)
     const Face4 fE = {c0.eta + 0.5 * sEx, c0.b + 0.5 * sBx, c0.U + 0.5 * sUx, c0.V + 0.5 * sVx};
     const Face4 fWst = {c0.eta - 0.5 * sEx, c0.b - 0.5 * sBx, c0.U - 0.5 * sUx, c0.V - 0.5 * sVx};
@@ -294,21 +303,22 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
         if (out * dt > hq) th = hq > 0 ? hq / (out * dt) : 0.0;
       }
     }
-    if (A.recf && outLane) {
-      // each value once, by the chunk that owns its row: x-face (gi|gi+1) of row s,
-      // y-face (s-1|s) of its owner row s-1, theta and the outflow sum of row q
+    if (REC && A.recf && outLane) {
+      // each value once, by the chunk that owns its row: the x-face (gi|gi+1) of
+      // row s, the y-face (s-1|s) of its owner row s-1, theta and the outflow sum
+      // of row q (the cell fields of row s went out with the slopes)
       double *R = A.recf;
       if (s >= y0 && s < y1) {
         const int k = s * W + gi;
-        R[k] = fxCur.M; R[k + fs] = fxCur.NL; R[k + 2 * fs] = fxCur.NR; R[k + 3 * fs] = fxCur.T;
+        R[k + 11 * fs] = fxCur.M; R[k + 12 * fs] = fxCur.NL; R[k + 13 * fs] = fxCur.NR; R[k + 14 * fs] = fxCur.T;
       }
       if (s - 1 >= y0 && s - 1 < y1) {
         const int k = (s - 1) * W + gi;
-        R[k + 4 * fs] = fyCur.M; R[k + 5 * fs] = fyCur.NL; R[k + 6 * fs] = fyCur.NR; R[k + 7 * fs] = fyCur.T;
+        R[k + 15 * fs] = fyCur.M; R[k + 16 * fs] = fyCur.NL; R[k + 17 * fs] = fyCur.NR; R[k + 18 * fs] = fyCur.T;
       }
       if (q >= y0 && q < y1) {
         const int k = q * W + gi;
-        R[k + 8 * fs] = th; R[k + 9 * fs] = out;
+        R[k + 19 * fs] = th; R[k + 20 * fs] = out;
       }
     }
     // ---- x-face contributions at row q (swe.mjs:530-540) ---------------------
@@ -461,7 +471,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
 
 template <int MODE>
 __global__ __launch_bounds__(SWEEP_LANES) void k_sweep(SweepArgs A) {
-  sweep_body<MODE>(A, blockIdx.x, blockIdx.y, blockIdx.y * gridDim.x + blockIdx.x);
+  sweep_body<MODE, true>(A, blockIdx.x, blockIdx.y, blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 // Batched ensembles: a persistent grid of wavefronts pulls (basin, strip, chunk)
@@ -481,7 +491,7 @@ __global__ __launch_bounds__(SWEEP_LANES, 3) void k_sweep_batch(const BasinDev *
     const int strip = __builtin_amdgcn_readfirstlane(tasks[t].strip);
     const int chunk = __builtin_amdgcn_readfirstlane(tasks[t].chunk);
     const int wave = __builtin_amdgcn_readfirstlane(tasks[t].wave);
-    sweep_body<MODE>(basins[basin].sw[parity][MODE - 1], strip, chunk, wave);
+    sweep_body<MODE, false>(basins[basin].sw[parity][MODE - 1], strip, chunk, wave);
   }
 }
 

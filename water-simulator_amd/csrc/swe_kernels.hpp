@@ -91,9 +91,11 @@ struct SweepArgs {
   Ctl *ctl;
   double *mf_part;    // one partial per wave
   double *rec;        // stage 2, optional: the dryCleaned, pre-friction interior state (adjoint replay)
-  // optional (adjoint replay): per cell, fstride apart, the M, N+pL, N+pR, T flux
-  // terms of its east x-face, those of its north y-face, its positivity factor
-  // theta and outgoing flux sum (zero / theta = 1 where the stage has none)
+  // optional (adjoint replay): per cell, fstride apart, its velocities U, V and
+  // surface, its limited slopes (eta, U, V, b; x then y each: sEx sEy sUx sUy
+  // sVx sVy sBx sBy), the M, N+pL, N+pR, T flux terms of its east x-face, those
+  // of its north y-face, its positivity factor theta and outgoing flux sum (zero /
+  // theta = 1 where the stage has none): REC_FIELDS fields
   double *recf;
   int rows_per_chunk;
 };
@@ -125,6 +127,7 @@ __device__ __forceinline__ void tide_times(Ctl *ctl) {
   }
 }
 
+constexpr int REC_FIELDS = 21;      // fields of SweepArgs::recf
 constexpr int SWEEP_LANES = 64;
 constexpr int SWEEP_HALO = 3;
 constexpr int SWEEP_OUT = SWEEP_LANES - 2 * SWEEP_HALO;   // 58 output columns per wave

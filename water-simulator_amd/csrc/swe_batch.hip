@@ -514,7 +514,7 @@ int swe_batch_autotune(swe_batch *b, int steps, int *best_rows, int *best_waves)
   const int old_rows = b->chunk;
   int best_r = old_rows, best_w = base_waves;
   double best = 1e300;
-  for (int rows : {16, 24, 32, 48, 64}) {
+  for (int rows : {8, 12, 16, 24, 32, 48, 64}) {
     for (int wmul : {1, 2}) {
       if ((rc = swe_batch_set_chunk_rows(b, rows, base_waves * wmul))) continue;
       // restore the starting state for every trial

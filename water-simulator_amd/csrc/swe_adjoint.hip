@@ -20,7 +20,7 @@
 // positivity factors, then steps back through the segment with no further
 // recomputation.  The replay of segment s - 1 runs on a second stream into the
 // other of two stores while segment s is reversed.  Memory: nsteps/K states of
-// 3 W H doubles, two stores of K steps of 51 W H, plus 54 + 24 W H doubles of
+// 3 W H doubles, two stores of K steps of 51 W H, plus 33 + 24 W H doubles of
 // pass intermediates and scratch (about 56 GB at 1024x1024 and 4000 steps).
 #include <hip/hip_runtime.h>
 
@@ -172,7 +172,7 @@ struct swe_adj {
   int K = 1;                       // steps of one replayed segment
   int64_t KO = 1;                  // steps between the checkpoints kept for the whole run (a multiple of K)
   size_t N = 0;
-  double *work = nullptr;          // S_COUNT * N
+  double *work = nullptr;          // the reverse passes' own slots, (S_COUNT - REC_FIELDS) * N
   Work w{};
   double *ckpt = nullptr;          // ceil(nsteps / KO) states of 3N
   double *ickpt = nullptr;         // two levels: the KO / K segment start states of one outer segment
@@ -516,7 +516,8 @@ int swe_adj_plan(int64_t nsteps, int64_t ncells, int64_t inner, int64_t outer, d
   if (nsteps < 1 || ncells < 1 || inner < 1 || outer < inner || outer % inner || !bytes) return SWE_EINVAL;
   const double nck = (double)((nsteps + outer - 1) / outer);
   const double nin = outer > inner ? (double)((outer + inner - 1) / inner) : 0.0;
-  const double fields = 3.0 * (nck + nin) + 2.0 * (double)inner * STEP_FIELDS + S_COUNT + 3.0 * T_COUNT + 2.0;
+  const double fields =
+      3.0 * (nck + nin) + 2.0 * (double)inner * STEP_FIELDS + (S_COUNT - REC_FIELDS) + 3.0 * T_COUNT + 2.0;
   *bytes = 8.0 * (double)ncells * fields + 16.0 * (double)nsteps + 2.0 * 8.0 * (double)inner * 2 * MAX_TIDE * 2;
   return SWE_OK;
 }
@@ -568,8 +569,11 @@ int swe_adj_create_levels(swe_ctx *c, int64_t nsteps, int64_t inner, int64_t out
     return ctx_fail(c, e == hipErrorOutOfMemory ? SWE_ENOMEM : SWE_EHIP, m);
   };
   hipError_t e;
-  if ((e = hipMalloc(&a->work, sizeof(double) * S_COUNT * N)) != hipSuccess) return bail("adjoint passes", e);
-  for (int q = 0; q < S_COUNT; ++q) a->w.s[q] = a->work + (size_t)q * N;
+  // slots S_U .. S_OUT are the recorded stage fields (pointed into the segment
+  // store per stage); only the reverse passes' own slots are allocated here
+  const size_t nslots = S_COUNT - REC_FIELDS;
+  if ((e = hipMalloc(&a->work, sizeof(double) * nslots * N)) != hipSuccess) return bail("adjoint passes", e);
+  for (int q = 0; q < S_COUNT; ++q) a->w.s[q] = q < REC_FIELDS ? nullptr : a->work + (size_t)(q - REC_FIELDS) * N;
   if ((e = hipMalloc(&a->ckpt, sizeof(double) * nck * N3)) != hipSuccess) return bail("checkpoints", e);
   if (a->KO > a->K) {
     const size_t nin = (size_t)((a->KO + a->K - 1) / a->K);
@@ -589,7 +593,7 @@ int swe_adj_create_levels(swe_ctx *c, int64_t nsteps, int64_t inner, int64_t out
   if ((e = hipMalloc(&a->lN, sizeof(double) * N)) != hipSuccess) return bail("Manning adjoint", e);
   if ((e = hipMalloc(&a->dt_dev, sizeof(double) * (size_t)nsteps)) != hipSuccess) return bail("dt record", e);
   if ((e = hipMalloc(&a->t_dev, sizeof(double) * (size_t)nsteps)) != hipSuccess) return bail("t record", e);
-  hipLaunchKernelGGL(k_fill, grid_for(S_COUNT * N), dim3(256), 0, c->stream, S_COUNT * N, a->work, 0.0);
+  hipLaunchKernelGGL(k_fill, grid_for(nslots * N), dim3(256), 0, c->stream, nslots * N, a->work, 0.0);
   hipLaunchKernelGGL(k_fill, grid_for(T_COUNT * N3), dim3(256), 0, c->stream, T_COUNT * N3, a->tmp, 0.0);
   if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail("clear", e);
   // the differentiated forward always reads a per-cell Manning field

@@ -1,8 +1,16 @@
-"""Writes a sample of the 4096x4096 large-grid bed (the benchmark's seeded
-recipe at a smaller size) and times the reference JS solver on it with
-time_reference.mjs.  Build-container only: the reference does not travel.
+"""Times the reference JS solver on the benchmark's workloads with
+time_reference.mjs: writes the workload's bed and initial surface (the same
+seeded recipes the benchmark uses, at the stated size) and a spec, and runs
+Node on them.  Build-container only: the reference does not travel.
 
-    python oracle/time_reference.py [--n 512] [--steps 5] [--reps 3]
+    python oracle/time_reference.py [--workload large|inversion|sphere] [--n 512] [--steps 5] [--reps 3]
+
+large: the 4096x4096 recipe at n x n (dx = 10 m, west Flather M2 tide of 1 m,
+Manning 0.025, f = 1e-4); inversion: the 1024x1024 shelf at its full size
+(outflow west, walls elsewhere, Manning 0.025, f = 1e-4); sphere: member 0 of
+the 2160x1080 global ocean at its full size (periodic in longitude, walls at
+the poles, its Manning n; the reference has no equilibrium-tide forcing, so the
+step is timed without it).
 """
 import argparse
 import json
@@ -11,28 +19,58 @@ import subprocess
 import sys
 import tempfile
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(os.path.dirname(HERE), "water-simulator_amd"))
 
 
+def workload(name, n):
+    import workloads
+    if name == "large":
+        (b, h, hu, hv), p = workloads.large_grid(nx=n, ny=n, dx=10.0)
+        spec = dict(nx=n, ny=n, dx=10.0, sphere=None, manning=0.025, coriolis=1e-4, cfl=0.45,
+                    bc=["flather", "reflect", "reflect", "reflect"], flather=dict(amp=1.0, period=44712.0))
+        return b, np.zeros_like(b), spec
+    if name == "inversion":
+        b, h, p, gauges = workloads.inversion()
+        spec = dict(nx=p["nx"], ny=p["ny"], dx=p["dx"], sphere=None, manning=p["manning"],
+                    coriolis=p["coriolis"], cfl=p["cfl"], bc=list(p["bc"]), flather=None)
+        return b, np.where(h > 0, b + h, b), spec
+    if name == "sphere":
+        b, h, p, manning = workloads.sphere_tides()
+        spec = dict(nx=p["nx"], ny=p["ny"], dx=None,
+                    sphere=dict(R=p["R"], lat0=p["lat0"], lat1=p["lat1"], omega=p["omega"]),
+                    manning=manning[0], coriolis=0.0, cfl=p["cfl"],
+                    bc=["periodic", "periodic", "reflect", "reflect"], flather=None)
+        return b, np.zeros_like(b), spec
+    raise SystemExit(f"unknown workload {name}")
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--workload", default="large", choices=["large", "inversion", "sphere"])
+    ap.add_argument("--n", type=int, default=512, help="grid size of the large workload")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--reps", type=int, default=3)
     a = ap.parse_args()
-    import workloads
-    (b, h, hu, hv), p = workloads.large_grid(nx=a.n, ny=a.n, dx=10.0)
-    with tempfile.NamedTemporaryFile(suffix=".f64", delete=False) as f:
-        f.write(b.astype("<f8").tobytes())
-        bed = f.name
+    b, eta, spec = workload(a.workload, a.n)
+    files = []
     try:
-        r = subprocess.run(["node", os.path.join(HERE, "time_reference.mjs"), bed, str(a.n), str(a.n),
-                            str(a.steps), str(a.reps)], capture_output=True, text=True, check=True)
+        for arr in (b, eta):
+            with tempfile.NamedTemporaryFile(suffix=".f64", delete=False) as f:
+                f.write(np.ascontiguousarray(arr, dtype="<f8").tobytes())
+                files.append(f.name)
+        spec.update(workload=a.workload, bed=files[0], eta=files[1], steps=a.steps, reps=a.reps)
+        with tempfile.NamedTemporaryFile("w", suffix=".json", delete=False) as f:
+            json.dump(spec, f)
+            files.append(f.name)
+        r = subprocess.run(["node", os.path.join(HERE, "time_reference.mjs"), files[2]],
+                           capture_output=True, text=True, check=True)
     finally:
-        os.unlink(bed)
-    out = json.loads(r.stdout.strip().splitlines()[-1])
-    print(json.dumps(out))
+        for name in files:
+            os.unlink(name)
+    print(r.stdout.strip().splitlines()[-1])
 
 
 if __name__ == "__main__":

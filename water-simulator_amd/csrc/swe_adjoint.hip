@@ -9,8 +9,9 @@
 // the RK combination^T, residual^T of stage 2, BC^T, then the same for stage 1.
 // residual^T runs as one-thread-per-cell passes over the grid with their
 // intermediates in HBM (swe_adjoint.hpp): on the host all seven, on the device
-// four, since the replayed stage sweeps record the velocities, surface, slopes,
-// face flux terms and positivity factors that the other three recompute.
+// three (accumulation + positivity^T, face^T, gather + limiter^T), since the
+// replayed stage sweeps record the velocities, surface, slopes, face flux terms
+// and positivity factors that the other passes recompute.
 //
 // Time: the forward records dt per step on the device and keeps the state every
 // K steps (K ~ sqrt(nsteps)); the reverse sweep re-runs each segment from its
@@ -20,7 +21,7 @@
 // positivity factors, then steps back through the segment with no further
 // recomputation.  The replay of segment s - 1 runs on a second stream into the
 // other of two stores while segment s is reversed.  Memory: nsteps/K states of
-// 3 W H doubles, two stores of K steps of 51 W H, plus 33 + 24 W H doubles of
+// 3 W H doubles, two stores of K steps of 51 W H, plus 22 + 24 W H doubles of
 // pass intermediates and scratch (about 56 GB at 1024x1024 and 4000 steps).
 #include <hip/hip_runtime.h>
 
@@ -61,13 +62,10 @@ __global__ void k_adj_faces_rev(Params P, const double *b, Work w) {
   int i, j;
   if (cell_of(P, i, j)) adj_faces_rev_at(P, b, w, i, j);
 }
-__global__ void k_adj_cells_rev(Params P, StateP S, const double *b, StateP lR, Work w, const double *tcs) {
+__global__ void k_adj_cells_slopes_rev(Params P, StateP S, const double *b, StateP lR, Work w, const double *tcs,
+                                       StateW lS) {
   int i, j;
-  if (cell_of(P, i, j)) adj_cells_rev_at(P, S, b, lR, w, tcs, i, j);
-}
-__global__ void k_adj_slopes_rev(Params P, StateP S, Work w, StateW lS) {
-  int i, j;
-  if (cell_of(P, i, j)) adj_slopes_rev_at(P, S, w, lS, i, j);
+  if (cell_of(P, i, j)) adj_cells_slopes_rev_at(P, S, b, lR, w, tcs, lS, i, j);
 }
 __global__ void k_adj_bc(Params P, StateW l, int pass) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -172,7 +170,7 @@ struct swe_adj {
   int K = 1;                       // steps of one replayed segment
   int64_t KO = 1;                  // steps between the checkpoints kept for the whole run (a multiple of K)
   size_t N = 0;
-  double *work = nullptr;          // the reverse passes' own slots, (S_COUNT - REC_FIELDS) * N
+  double *work = nullptr;          // the device reverse passes' own slots, DEV_SLOTS * N
   Work w{};
   double *ckpt = nullptr;          // ceil(nsteps / KO) states of 3N
   double *ickpt = nullptr;         // two levels: the KO / K segment start states of one outer segment
@@ -222,6 +220,8 @@ enum { T_R, T_CUR, T_LBP, T_L0, T_L1, T_A0, T_A1, T_A2, T_COUNT };
 // Work slots S_U .. S_OUT)
 enum { G_AP, G_BP, G_DC2, G_COUNT };
 constexpr int STEP_FIELDS = 3 * G_COUNT + 2 * REC_FIELDS;
+// Work slots the device passes keep in HBM: S_LXM .. S_LYVR
+constexpr int DEV_SLOTS = S_LETA - REC_FIELDS;
 static_assert(REC_FIELDS == S_OUT - S_U + 1 && S_U == 0, "recorded slots are Work slots S_U .. S_OUT");
 StateW tslot(swe_adj *a, int q) { return st3(a->tmp + (size_t)q * 3 * a->N, a->N); }
 StateW cur_state(swe_ctx *c) { return {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]}; }
@@ -270,8 +270,7 @@ int residual_rev(swe_adj *a, const StateP &S, double dt, const StateP &lR, const
   for (int q = 0; q < REC_FIELDS; ++q) w.s[S_U + q] = rec + (size_t)q * a->N;
   hipLaunchKernelGGL(k_adj_accum, g, blk, 0, st, P, S, dt, lR, w);
   hipLaunchKernelGGL(k_adj_faces_rev, g, blk, 0, st, P, a->bed, w);
-  hipLaunchKernelGGL(k_adj_cells_rev, g, blk, 0, st, P, S, a->bed, lR, w, tcs);
-  hipLaunchKernelGGL(k_adj_slopes_rev, g, blk, 0, st, P, S, w, lS);
+  hipLaunchKernelGGL(k_adj_cells_slopes_rev, g, blk, 0, st, P, S, a->bed, lR, w, tcs, lS);
   ACHK(a, hipGetLastError());
   return SWE_OK;
 }
@@ -516,8 +515,7 @@ int swe_adj_plan(int64_t nsteps, int64_t ncells, int64_t inner, int64_t outer, d
   if (nsteps < 1 || ncells < 1 || inner < 1 || outer < inner || outer % inner || !bytes) return SWE_EINVAL;
   const double nck = (double)((nsteps + outer - 1) / outer);
   const double nin = outer > inner ? (double)((outer + inner - 1) / inner) : 0.0;
-  const double fields =
-      3.0 * (nck + nin) + 2.0 * (double)inner * STEP_FIELDS + (S_COUNT - REC_FIELDS) + 3.0 * T_COUNT + 2.0;
+  const double fields = 3.0 * (nck + nin) + 2.0 * (double)inner * STEP_FIELDS + DEV_SLOTS + 3.0 * T_COUNT + 2.0;
   *bytes = 8.0 * (double)ncells * fields + 16.0 * (double)nsteps + 2.0 * 8.0 * (double)inner * 2 * MAX_TIDE * 2;
   return SWE_OK;
 }
@@ -570,10 +568,12 @@ int swe_adj_create_levels(swe_ctx *c, int64_t nsteps, int64_t inner, int64_t out
   };
   hipError_t e;
   // slots S_U .. S_OUT are the recorded stage fields (pointed into the segment
-  // store per stage); only the reverse passes' own slots are allocated here
-  const size_t nslots = S_COUNT - REC_FIELDS;
+  // store per stage) and the fused gather + limiter^T pass keeps S_LETA .. in
+  // registers: only the slots in between are allocated here
+  const size_t nslots = DEV_SLOTS;
   if ((e = hipMalloc(&a->work, sizeof(double) * nslots * N)) != hipSuccess) return bail("adjoint passes", e);
-  for (int q = 0; q < S_COUNT; ++q) a->w.s[q] = q < REC_FIELDS ? nullptr : a->work + (size_t)(q - REC_FIELDS) * N;
+  for (int q = 0; q < S_COUNT; ++q)
+    a->w.s[q] = (q < REC_FIELDS || q >= S_LETA) ? nullptr : a->work + (size_t)(q - REC_FIELDS) * N;
   if ((e = hipMalloc(&a->ckpt, sizeof(double) * nck * N3)) != hipSuccess) return bail("checkpoints", e);
   if (a->KO > a->K) {
     const size_t nin = (size_t)((a->KO + a->K - 1) / a->K);

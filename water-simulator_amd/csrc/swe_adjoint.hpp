@@ -400,8 +400,9 @@ __host__ __device__ inline void adj_faces_rev_at(const Params &P, const double *
 // pass 6: per cell, gather the face-state adjoints into eta, U, V and the
 // slopes; the bed (and spherical geometric) term^T and Coriolis^T
 // (swe.mjs:570-628)
-__host__ __device__ inline void adj_cells_rev_at(const Params &P, const StateP &S, const double *b, const StateP &lR,
-                                                 const Work &w, const double *tcs, int i, int j) {
+struct CellRev { double leta, lU, lV, lsE[2], lsU[2], lsV[2], lhuC, lhvC; };
+__host__ __device__ inline CellRev cells_rev(const Params &P, const StateP &S, const double *b, const StateP &lR,
+                                             const Work &w, const double *tcs, int i, int j) {
   const int W = P.W, k = j * W + i;
   double leta = 0, lU = 0, lV = 0, lsE[2] = {0, 0}, lsU[2] = {0, 0}, lsV[2] = {0, 0};
   for (int d = 0; d < 2; ++d) {
@@ -456,20 +457,67 @@ __host__ __device__ inline void adj_cells_rev_at(const Params &P, const StateP &
       leta += lrhu * T[j] * (ts * cl + tc * sl) + lrhv * T[P.H + j] * (tc * cl - ts * sl);
     }
   }
-  w.s[S_LETA][k] = leta; w.s[S_LU][k] = lU; w.s[S_LV][k] = lV;
-  w.s[S_LSEX][k] = lsE[0]; w.s[S_LSEY][k] = lsE[1];
-  w.s[S_LSUX][k] = lsU[0]; w.s[S_LSUY][k] = lsU[1];
-  w.s[S_LSVX][k] = lsV[0]; w.s[S_LSVY][k] = lsV[1];
-  w.s[S_LHUC][k] = lhuC; w.s[S_LHVC][k] = lhvC;
+  CellRev r;
+  r.leta = leta; r.lU = lU; r.lV = lV;
+  for (int d = 0; d < 2; ++d) { r.lsE[d] = lsE[d]; r.lsU[d] = lsU[d]; r.lsV[d] = lsV[d]; }
+  r.lhuC = lhuC; r.lhvC = lhvC;
+  return r;
+}
+
+__host__ __device__ inline void adj_cells_rev_at(const Params &P, const StateP &S, const double *b, const StateP &lR,
+                                                 const Work &w, const double *tcs, int i, int j) {
+  const int k = j * P.W + i;
+  const CellRev c = cells_rev(P, S, b, lR, w, tcs, i, j);
+  w.s[S_LETA][k] = c.leta; w.s[S_LU][k] = c.lU; w.s[S_LV][k] = c.lV;
+  w.s[S_LSEX][k] = c.lsE[0]; w.s[S_LSEY][k] = c.lsE[1];
+  w.s[S_LSUX][k] = c.lsU[0]; w.s[S_LSUY][k] = c.lsU[1];
+  w.s[S_LSVX][k] = c.lsV[0]; w.s[S_LSVY][k] = c.lsV[1];
+  w.s[S_LHUC][k] = c.lhuC; w.s[S_LHVC][k] = c.lhvC;
+}
+
+// The slope adjoints (eta, U, V) along axis d of cell (i, j) alone, in the same
+// operations and order as cells_rev, so bit-identical to its lsE[d], lsU[d],
+// lsV[d]: the fused device pass below evaluates them for a cell's neighbours
+// instead of reading them back from memory.
+struct SlopeAdj { double e, u, v; };
+__host__ __device__ inline SlopeAdj slope_adj(const Params &P, const double *b, const StateP &lR, const Work &w,
+                                              int i, int j, int d) {
+  const int W = P.W, k = j * W + i;
+  const int base = d == 0 ? S_LXETAL : S_LYETAL;
+  const bool has_m = d == 0 ? i >= 1 : j >= 1;
+  const int km = has_m ? (d == 0 ? k - 1 : k - W) : k;
+  const double pe = w.s[base][k], pu = w.s[base + 1][k], pv = w.s[base + 2][k];
+  const double me = w.s[base + 3][km], mu = w.s[base + 4][km], mv = w.s[base + 5][km];
+  SlopeAdj r = {0.0, 0.0, 0.0};
+  r.e += 0.5 * pe; r.u += 0.5 * pu; r.v += 0.5 * pv;
+  if (has_m) { r.e -= 0.5 * me; r.u -= 0.5 * mu; r.v -= 0.5 * mv; }
+  if (i >= 2 && i <= W - 3 && j >= 2 && j <= P.H - 3 && (P.order >= 2 || P.sphere)) {
+    const double eta = w.s[S_ETA][k], bk = b[k];
+    const double sE = w.s[S_SEX + d][k], sB = w.s[S_SBX + d][k];
+    const double aP = eta + 0.5 * sE - (bk + 0.5 * sB), aM = eta - 0.5 * sE - (bk - 0.5 * sB);
+    const double div = d == 0 ? P.geo.dx[j] : P.geo.bedPhi[j];
+    const double lrhv = lR.hv[k];
+    const double lt = d == 0 ? lR.hu[k] : lrhv;
+    double lhP = lt * P.neg_half_g * sB / div, lhM = lhP;
+    if (d == 1 && P.sphere) {
+      lhP += lrhv * P.geo.geo[j] * P.g * 2.0 * jmax0(aP);
+      lhM += lrhv * P.geo.geo[j] * P.g * 2.0 * jmax0(aM);
+    }
+    if (aP > 0) r.e += 0.5 * lhP;
+    if (aM > 0) r.e -= 0.5 * lhM;
+  }
+  return r;
 }
 
 // pass 7: limiter^T gathered from the slopes of the cell and its two
 // neighbours along each axis, then vel^T: adds the adjoint of the stage input
-// state at (i, j) into lS
-__host__ __device__ inline void adj_slopes_rev_at(const Params &P, const StateP &S, const Work &w, const StateW &lS,
-                                                  int i, int j) {
+// state at (i, j) into lS.  slope_of(d, o) gives the slope adjoints of cell
+// k + o st along d (o = -1, 0, 1; asked only for cells whose slope exists).
+template <class F>
+__host__ __device__ inline void slopes_rev_core(const Params &P, const StateP &S, const Work &w, const StateW &lS,
+                                                int i, int j, double leta, double lU, double lV, double lhth,
+                                                double lhuC, double lhvC, const F &slope_of) {
   const int W = P.W, k = j * W + i;
-  double leta = w.s[S_LETA][k], lU = w.s[S_LU][k], lV = w.s[S_LV][k];
   if (P.order >= 2) {
     const double *eta = w.s[S_ETA], *U = w.s[S_U], *V = w.s[S_V];
     for (int d = 0; d < 2; ++d) {
@@ -477,14 +525,11 @@ __host__ __device__ inline void adj_slopes_rev_at(const Params &P, const StateP 
       const int pos = d == 0 ? i : j, len = d == 0 ? W : P.H;
       // every value the three slopes below can use, loaded before any branch (a
       // cell off the grid reads cell k instead and is never used): the five cells
-      // k + m st, m = -2..2, and the slope adjoints of the middle three
-      double e5[5], u5[5], v5[5], h5[5], lse[3], lsu[3], lsv[3];
+      // k + m st, m = -2..2
+      double e5[5], u5[5], v5[5], h5[5];
       for (int m = -2; m <= 2; ++m) {
         const int kc = (pos + m < 0 || pos + m >= len) ? k : k + m * st;
         e5[m + 2] = eta[kc]; u5[m + 2] = U[kc]; v5[m + 2] = V[kc]; h5[m + 2] = S.h[kc];
-        if (m >= -1 && m <= 1) {
-          lse[m + 1] = w.s[S_LSEX + d][kc]; lsu[m + 1] = w.s[S_LSUX + d][kc]; lsv[m + 1] = w.s[S_LSVX + d][kc];
-        }
       }
       for (int o = -1; o <= 1; ++o) {             // slope of cell c = k + o st
         const int ci = d == 0 ? i + o : i, cj = d == 0 ? j : j + o;
@@ -498,17 +543,47 @@ __host__ __device__ inline void adj_slopes_rev_at(const Params &P, const StateP 
         const double fe = o == 0 ? dE.da - dE.db : (o == 1 ? -dE.da : dE.db);
         const double fu = o == 0 ? dU.da - dU.db : (o == 1 ? -dU.da : dU.db);
         const double fv = o == 0 ? dV.da - dV.db : (o == 1 ? -dV.da : dV.db);
-        leta += lse[o + 1] * fe;
-        lU += lsu[o + 1] * fu;
-        lV += lsv[o + 1] * fv;
+        const SlopeAdj ls = slope_of(d, o);
+        leta += ls.e * fe;
+        lU += ls.u * fu;
+        lV += ls.v * fv;
       }
     }
   }
   const double h = S.h[k];
   const VelD du = vel_deriv(S.hu[k], h, P.eps4), dv = vel_deriv(S.hv[k], h, P.eps4);
-  lS.h[k] += leta + lU * du.dh + lV * dv.dh + w.s[S_LHTH][k];
-  lS.hu[k] += lU * du.dhq + w.s[S_LHUC][k];
-  lS.hv[k] += lV * dv.dhq + w.s[S_LHVC][k];
+  lS.h[k] += leta + lU * du.dh + lV * dv.dh + lhth;
+  lS.hu[k] += lU * du.dhq + lhuC;
+  lS.hv[k] += lV * dv.dhq + lhvC;
+}
+
+// pass 7 reading pass 6's slots (host)
+__host__ __device__ inline void adj_slopes_rev_at(const Params &P, const StateP &S, const Work &w, const StateW &lS,
+                                                  int i, int j) {
+  const int k = j * P.W + i;
+  const int W = P.W;
+  slopes_rev_core(P, S, w, lS, i, j, w.s[S_LETA][k], w.s[S_LU][k], w.s[S_LV][k], w.s[S_LHTH][k], w.s[S_LHUC][k],
+                  w.s[S_LHVC][k], [&](int d, int o) {
+                    const int c = k + o * (d == 0 ? 1 : W);
+                    SlopeAdj r = {w.s[S_LSEX + d][c], w.s[S_LSUX + d][c], w.s[S_LSVX + d][c]};
+                    return r;
+                  });
+}
+
+// passes 6 and 7 in one (device): the cell's own gather, its neighbours' slope
+// adjoints recomputed (slope_adj), no slots written between them
+__host__ __device__ inline void adj_cells_slopes_rev_at(const Params &P, const StateP &S, const double *b,
+                                                        const StateP &lR, const Work &w, const double *tcs,
+                                                        const StateW &lS, int i, int j) {
+  const int k = j * P.W + i;
+  const CellRev c = cells_rev(P, S, b, lR, w, tcs, i, j);
+  slopes_rev_core(P, S, w, lS, i, j, c.leta, c.lU, c.lV, w.s[S_LHTH][k], c.lhuC, c.lhvC, [&](int d, int o) {
+    if (o == 0) {
+      SlopeAdj r = {c.lsE[d], c.lsU[d], c.lsV[d]};
+      return r;
+    }
+    return slope_adj(P, b, lR, w, d == 0 ? i + o : i, d == 0 ? j : j + o, d);
+  });
 }
 
 // ---- boundary conditions^T (swe.mjs:289-303, :1026-1045, :1119-1124) ---------

@@ -1,0 +1,173 @@
+"""Size-independent properties at the benchmark configurations' full sizes, where
+the oracle would take minutes per step:
+
+* lake at rest on a 4096x4096 grid over the fully wet kinds of bed the
+  reference's own still-water check uses (tools/verify.mjs:112-146: a smooth
+  bump, a sharp step, a random-rough bed, in bands, with its 0.15 m cross-ripple),
+  with Coriolis and Manning on: the surface stays flat and still to that check's
+  bars (1e-12 m, 1e-12 m/s) and the volume to 1e-13;
+* closed-basin mass on the 4096x4096 benchmark bed (beach, 64 bumps, noise;
+  wet/dry fronts) with a surface hump: volume conserved to 1e-13 relative,
+  floored mass counted (swe.mjs:734-745);
+* the 4096x4096 benchmark workload itself (Flather tide, friction, Coriolis):
+  bit-identical after 20 steps whatever the row-chunk height, and finite;
+* the 512-basin ensemble at its full size (65.6 M cells): sampled basins of the
+  batched run bit-identical to their plain runs.
+
+Left out of the still-lake grid: the check's surface-piercing island.  Tiled in
+2-D its shoreline is balanced only to ~1e-5 m in the scheme itself -- the oracle
+(the reference's step restated line by line and pinned to its golden runs)
+gives the same imbalance on a 200x200 copy of that bed, bit for bit with the
+GPU.  In shoreline cells a few mm deep the clipped face depths max(0, eta - b)
+no longer cancel the centred bed term; the benchmark bed's 0.2 m cell noise does
+the same (5e-6 m after 200 steps).  Mass is conserved there regardless (the
+second test)."""
+import math
+
+import numpy as np
+import pytest
+
+import swe_amd
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+NX = NY = 4096
+
+
+def interior(W, H):
+    m = np.zeros((H, W), bool)
+    m[2:-2, 2:-2] = True
+    return m.ravel()
+
+
+def test_lake_at_rest_4096():
+    dx = 5.0
+    W, H = NX + 4, NY + 4
+    x = (np.arange(W) - 1.5) * dx
+    y = (np.arange(H) - 1.5) * dx
+    X, Y = np.meshgrid(x, y)
+    xm = np.mod(X, 1000.0)                    # the reference's 1 km test bed, tiled
+    b = np.where(Y < NY * dx / 3, 0.8 * np.exp(-((xm - 500) / 120) ** 2),
+        np.where(Y < 2 * NY * dx / 3, np.where((xm > 480) & (xm < 620), 1.2, 0.1),
+                 0.6 * (np.sin(X * 0.031) + np.sin(X * 0.0173) * np.cos(X * 0.0077)))) + 0.15 * np.sin(Y * 0.05)
+    b = b.ravel()
+    ETA = 2.0
+    h = np.maximum(0.0, ETA - b)
+    z = np.zeros_like(h)
+    rows = swe_amd.geometry_cartesian(NX, NY, dx, coriolis=1e-4)
+    s = swe_amd.Solver(NX, NY, rows, manning=0.025, coriolis=1e-4, dx=dx, dy=dx)
+    s.load_state(b, h, z, z)
+    s.set_bc(("reflect", "reflect", "periodic", "periodic"))
+    inner = interior(W, H)
+    v0 = math.fsum(h[inner])
+    assert s.advance(200) == 200
+    _, h1, hu1, hv1 = s.state()
+    wet = inner & (h1 > 1e-3)
+    dev = np.abs((b + h1)[wet] - ETA).max()
+    spd = np.hypot(hu1[wet] / h1[wet], hv1[wet] / h1[wet]).max()
+    dv = abs(math.fsum(h1[inner]) - v0) / v0
+    print(f"  4096^2 lake at rest, 200 steps: surface {dev:.2e} m, speed {spd:.2e} m/s, volume {dv:.2e}")
+    assert dev < 1e-12 and spd < 1e-12 and dv < 1e-13
+
+
+def test_closed_basin_mass_4096_benchmark_bed():
+    (b, h, hu, hv), p = workloads.large_grid()
+    dx = p["dx"]
+    W, H = NX + 4, NY + 4
+    X, Y = np.meshgrid((np.arange(W) - 1.5) * dx, (np.arange(H) - 1.5) * dx)
+    e0 = (0.5 * np.exp(-((X - 0.4 * NX * dx) ** 2 + (Y - 0.5 * NY * dx) ** 2) / 2000.0 ** 2)).ravel()
+    h0 = np.maximum(0.0, e0 - b)
+    rows = swe_amd.geometry_cartesian(NX, NY, dx, coriolis=p["coriolis"])
+    s = swe_amd.Solver(NX, NY, rows, manning=p["manning"], coriolis=p["coriolis"], cfl=p["cfl"], dx=dx, dy=dx)
+    s.load_state(b, h0, hu, hv)
+    s.set_bc(("reflect",) * 4)
+    inner = interior(W, H)
+    v0 = math.fsum(h0[inner])
+    floored = 0.0
+    for _ in range(100):
+        assert s.step() > 0
+        floored += s.mass_floored
+    _, h1, _, _ = s.state()
+    drift = (math.fsum(h1[inner]) - v0 - floored) / v0
+    print(f"  4096^2 closed basin, 100 steps: relative volume drift {drift:.2e} (floored {floored:.2e})")
+    assert abs(drift) < 1e-13
+
+
+def test_benchmark_workload_4096_chunk_independent():
+    (b, h, hu, hv), p = workloads.large_grid()
+    rows = swe_amd.geometry_cartesian(NX, NY, p["dx"], coriolis=p["coriolis"])
+    out = []
+    for chunk in (94, 48):
+        s = swe_amd.Solver(NX, NY, rows, manning=p["manning"], coriolis=p["coriolis"], cfl=p["cfl"],
+                           dx=p["dx"], dy=p["dx"])
+        s.load_state(b, h, hu, hv)
+        s.set_bc(p["bc"])
+        s.set_harmonic(0, amp=[p["tide_amp"]], omega=[p["tide_omega"]], phase=[0.0])
+        s.set_chunk_rows(chunk)
+        s.use_graph(True)
+        assert s.advance(20) == 20
+        out.append((s.state()[1:], s.t))
+        s.close()
+    (a, ta), (c, tc) = out
+    assert ta == tc and ta > 0
+    for x, y in zip(a, c):
+        assert np.all(np.isfinite(x)) and np.array_equal(x, y)
+
+
+def test_ensemble_512_sampled_basins_equal_plain_runs():
+    specs = workloads.ensemble(nb=512)
+    B = swe_amd.Batch([dict(nx=s["nx"], ny=s["ny"], manning=s["manning"], cfl=s["cfl"],
+                            rows=swe_amd.geometry_cartesian(s["nx"], s["ny"], s["dx"])) for s in specs])
+    for i, s in enumerate(specs):
+        B.load_state(i, *s["state"])
+        B.set_bc(i, s["bc"])
+        B.set_harmonic(i, 0, amp=[s["amp"]], omega=[s["omega"]], phase=[0.0],
+                       uamp=[s["uamp"]], uomega=[s["omega"]], uphase=[0.0])
+    B.use_graph(True)
+    B.advance(20)
+    sizes = [s["nx"] * s["ny"] for s in specs]
+    for i in sorted({0, int(np.argmax(sizes)), int(np.argmin(sizes)), 257}):
+        s = specs[i]
+        g = swe_amd.Solver(s["nx"], s["ny"], swe_amd.geometry_cartesian(s["nx"], s["ny"], s["dx"]),
+                           manning=s["manning"], cfl=s["cfl"])
+        g.load_state(*s["state"])
+        g.set_bc(s["bc"])
+        g.set_harmonic(0, amp=[s["amp"]], omega=[s["omega"]], phase=[0.0],
+                       uamp=[s["uamp"]], uomega=[s["omega"]], uphase=[0.0])
+        g.advance(20)
+        _, bh, bhu, bhv = B.state(i)
+        _, gh, ghu, ghv = g.state()
+        assert B.get_scalar(i, swe_amd.S_T) == g.t
+        assert np.array_equal(bh, gh) and np.array_equal(bhu, ghu) and np.array_equal(bhv, ghv), i
+        g.close()
+    B.close()
+
+
+def test_island_shoreline_bed_gpu_equals_oracle():
+    # the surface-piercing island bed left out above, at 200x200: the GPU agrees
+    # with the oracle bit for bit, imbalance at the 2-D shoreline included
+    import oracle
+    nx = ny = 200
+    dx = 5.0
+    W, H = nx + 4, ny + 4
+    X, Y = np.meshgrid((np.arange(W) - 1.5) * dx, (np.arange(H) - 1.5) * dx)
+    b = (2.6 * np.exp(-((np.mod(X, 1000.0) - 500) / 90) ** 2) + 0.15 * np.sin(Y * 0.05)).ravel()
+    h = np.maximum(0.0, 2.0 - b)
+    z = np.zeros_like(h)
+    rows = swe_amd.geometry_cartesian(nx, ny, dx, coriolis=1e-4)
+    o = oracle.OracleSim(nx, ny, rows, dx=dx, dy=dx, manning=0.025, coriolis=1e-4)
+    o.load_state(b, h, z, z)
+    o.set_bc(("reflect", "reflect", "periodic", "periodic"))
+    s = swe_amd.Solver(nx, ny, rows, manning=0.025, coriolis=1e-4, dx=dx, dy=dx)
+    s.load_state(b, h, z, z)
+    s.set_bc(("reflect", "reflect", "periodic", "periodic"))
+    for _ in range(100):
+        assert o.step() == s.step()
+    _, gh, ghu, ghv = s.state()
+    assert np.array_equal(gh, o.h) and np.array_equal(ghu, o.hu) and np.array_equal(ghv, o.hv)
+    inner = interior(W, H)
+    wet = inner & (o.h > 1e-3)
+    dev = np.abs(b + o.h - 2.0)[wet].max()
+    print(f"  island bed 200x200, 100 steps: surface deviation {dev:.2e} m in the oracle and on the GPU")
+    assert np.any(inner & (o.h == 0.0))

@@ -1,5 +1,7 @@
-"""Size-independent properties at the benchmark configurations' full sizes, where
-the oracle would take minutes per step:
+"""The benchmark configurations at their full sizes: the GPU against the oracle
+bit for bit over the few steps the oracle affords there (4096x4096: about 5 s a
+step; the 2160x1080 sphere: about 1 s), and size-independent properties over
+longer runs:
 
 * lake at rest on a 4096x4096 grid over the fully wet kinds of bed the
   reference's own still-water check uses (tools/verify.mjs:112-146: a smooth
@@ -171,3 +173,80 @@ def test_island_shoreline_bed_gpu_equals_oracle():
     dev = np.abs(b + o.h - 2.0)[wet].max()
     print(f"  island bed 200x200, 100 steps: surface deviation {dev:.2e} m in the oracle and on the GPU")
     assert np.any(inner & (o.h == 0.0))
+
+
+def test_large_grid_4096_gpu_equals_oracle():
+    # the benchmark workload at its full size: two steps on the GPU and in the
+    # oracle (about 5 s each there), every field bit for bit, ghosts included
+    import oracle
+    (b, h, hu, hv), p = workloads.large_grid()
+    rows = swe_amd.geometry_cartesian(NX, NY, p["dx"], coriolis=p["coriolis"])
+    kw = dict(manning=p["manning"], coriolis=p["coriolis"], cfl=p["cfl"], dx=p["dx"], dy=p["dx"])
+    o = oracle.OracleSim(NX, NY, rows, **kw)
+    o.load_state(b, h, hu, hv)
+    o.set_bc(p["bc"])
+    s = swe_amd.Solver(NX, NY, rows, **kw)
+    s.load_state(b, h, hu, hv)
+    s.set_bc(p["bc"])
+    t = 0.0
+    for _ in range(2):
+        ext = [p["tide_amp"] * math.cos(p["tide_omega"] * t), 0.0] + [0.0] * 6
+        dt_o, dt_g = o.step(ext=ext), s.step(ext=ext)
+        assert dt_o == dt_g and dt_g > 0
+        t += dt_g
+    _, gh, ghu, ghv = s.state()
+    assert np.array_equal(gh, o.h) and np.array_equal(ghu, o.hu) and np.array_equal(ghv, o.hv)
+
+
+def test_sphere_2160x1080_gpu_equals_oracle():
+    # config 5's member 0 (without the tide, which the reference has no form of):
+    # three steps, every field bit for bit, ghosts and polar rows included
+    import oracle
+    b, h, p, manning = workloads.sphere_tides()
+    nx, ny = p["nx"], p["ny"]
+    rows = swe_amd.geometry_spherical(nx, ny, R=p["R"], lat0=p["lat0"], lat1=p["lat1"], omega=p["omega"])
+    z = np.zeros_like(h)
+    o = oracle.OracleSim(nx, ny, rows, sphere=True, R=p["R"], manning=manning[0], cfl=p["cfl"])
+    o.load_state(b, h, z, z)
+    o.set_bc(("periodic", "periodic", "reflect", "reflect"))
+    s = swe_amd.Solver(nx, ny, rows, sphere=True, R=p["R"], manning=manning[0], cfl=p["cfl"])
+    s.load_state(b, h, z, z)
+    for _ in range(3):
+        assert o.step() == s.step()
+    _, gh, ghu, ghv = s.state()
+    assert np.array_equal(gh, o.h) and np.array_equal(ghu, o.hu) and np.array_equal(ghv, o.hv)
+
+
+def test_sphere_2160x1080_mass_under_tide_and_resting_ocean():
+    b, h, p, manning = workloads.sphere_tides()
+    nx, ny = p["nx"], p["ny"]
+    W, H = nx + 4, ny + 4
+    rows = swe_amd.geometry_spherical(nx, ny, R=p["R"], lat0=p["lat0"], lat1=p["lat1"], omega=p["omega"])
+    lat, lon = swe_amd.sphere_lonlat(nx, ny, p["lat0"], p["lat1"])
+    area = np.repeat(np.asarray(rows["area"]), W)
+    inner = interior(W, H)
+    z = np.zeros_like(h)
+    # the equilibrium tide forces momentum only: the volume (h x cell area) stays put
+    s = swe_amd.Solver(nx, ny, rows, sphere=True, R=p["R"], manning=manning[0], cfl=p["cfl"])
+    s.load_state(b, h, z, z)
+    s.set_tide(p["tides"], lat, lon)
+    v0 = math.fsum((h * area)[inner])
+    floored = 0.0
+    for _ in range(200):
+        assert s.step() > 0
+        floored += s.mass_floored
+    _, h1, _, _ = s.state()
+    drift = (math.fsum((h1 * area)[inner]) - v0) / v0
+    print(f"  sphere 2160x1080 + M2/S2/K1, 200 steps: volume drift {drift:.2e} (floored {floored:.2e})")
+    assert floored == 0.0 and abs(drift) < 1e-13
+    s.close()
+    # without the tide the same ocean, coasts and poles included, stays at rest
+    s = swe_amd.Solver(nx, ny, rows, sphere=True, R=p["R"], manning=manning[0], cfl=p["cfl"])
+    s.load_state(b, h, z, z)
+    assert s.advance(100) == 100
+    _, h2, hu2, hv2 = s.state()
+    wet = inner & (h2 > 1e-3)
+    dev = np.abs((b + h2)[wet]).max()
+    spd = np.hypot(hu2[wet] / h2[wet], hv2[wet] / h2[wet]).max()
+    print(f"  sphere resting ocean, 100 steps: surface {dev:.2e} m, speed {spd:.2e} m/s")
+    assert dev < 16 * np.spacing(6000.0) and spd < 1e-11

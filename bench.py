@@ -191,6 +191,64 @@ def adjoint_measure(nsteps, every=10, fd_dirs=0):
     return out
 
 
+def inversion_measure(iters, nsteps=4000, every=10):
+    """Config 4 as an inversion: scipy's L-BFGS-B on the initial surface of the wet
+    cells, from a flat sea plus 1 cm of noise, driven by the adjoint's forward/backward pair (one
+    recorded forward for the misfit and one backward for dJ/d eta0 per
+    evaluation).  Reports the misfit and the distance to the hidden truth."""
+    import numpy as np
+    from scipy.optimize import minimize
+    import swe_amd
+    import workloads
+    b, ht, p, gauges = workloads.inversion()
+    nx, ny, dx = p["nx"], p["ny"], p["dx"]
+    W, H = nx + 4, ny + 4
+    rows = swe_amd.geometry_cartesian(nx, ny, dx, coriolis=p["coriolis"])
+    s = swe_amd.Solver(nx, ny, rows, manning=p["manning"], coriolis=p["coriolis"], cfl=p["cfl"], dx=dx, dy=dx)
+    z = np.zeros_like(ht)
+    s.load_state(b, ht, z, z)
+    s.set_bc(p["bc"])
+    s.autotune()
+    n0 = np.full(W * H, 0.025)
+    adj = swe_amd.Adjoint(s, nsteps)
+    adj.set_gauges(gauges, every, np.zeros((nsteps // every, len(gauges))))
+    adj.forward(ht, n0)
+    adj.set_gauges(gauges, every, adj.series())
+    inner = np.zeros((H, W), bool)
+    inner[2:-2, 2:-2] = True
+    wet = inner.ravel() & (b < -1.0)
+    eta_truth = (b + ht)[wet]
+
+    def h_of(e):
+        h = np.maximum(0.0, -b)
+        h[wet] = np.maximum(0.0, e - b[wet])
+        return h
+
+    hist = []
+    t0 = time.perf_counter()
+
+    def f(e):
+        J = adj.forward(h_of(e), n0)
+        gh, _ = adj.backward()
+        hist.append(J)
+        log(f"[bench] inversion evaluation {len(hist)}: J = {J:.4e} ({time.perf_counter() - t0:.0f} s)")
+        return J, gh[wet]
+
+    # a flat sea puts every limiter argument exactly on its kink (all surface
+    # differences zero), where the subgradient is no descent direction: start
+    # 1 cm of seeded noise away from it
+    e0 = 0.01 * workloads.SplitMix64(47).normals(int(wet.sum()))
+    r = minimize(f, e0, jac=True, method="L-BFGS-B", options=dict(maxiter=iters))
+    el = time.perf_counter() - t0
+    out = {"optimizer": "L-BFGS-B", "iterations": int(r.nit), "evaluations": len(hist), "steps": nsteps,
+           "misfit_start": hist[0], "misfit_end": float(r.fun),
+           "truth_error_start": float(np.linalg.norm(e0 - eta_truth)),
+           "truth_error_end": float(np.linalg.norm(r.x - eta_truth)), "seconds": el}
+    adj.close()
+    s.close()
+    return out
+
+
 def sphere_measure(steps, adj_steps, every=10):
     """Config 5: 16-member global-ocean ensemble on the 2160x1080 sphere with the
     M2/S2/K1 equilibrium tide, one batched launch per phase; throughput over a
@@ -298,6 +356,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=-1, help="rows per wavefront chunk; -1 autotune, 0 default")
     ap.add_argument("--adjoint-steps", type=int, default=4000, help="0 skips the adjoint measurement")
     ap.add_argument("--adjoint-fd", type=int, default=0, help="finite-difference directions checked on the adjoint")
+    ap.add_argument("--inversion-iters", type=int, default=0, help="L-BFGS iterations of the config 4 inversion")
     ap.add_argument("--sphere-steps", type=int, default=100, help="0 skips the spherical ensemble")
     ap.add_argument("--sphere-adjoint-steps", type=int, default=100)
     args = ap.parse_args()
@@ -370,6 +429,8 @@ def main():
         out["batched"] = batched_measure(args.batched_steps, 5, 20)
     if args.adjoint_steps > 0:
         out["adjoint"] = adjoint_measure(args.adjoint_steps, fd_dirs=args.adjoint_fd)
+    if args.inversion_iters > 0:
+        out["inversion"] = inversion_measure(args.inversion_iters)
     if args.sphere_steps > 0:
         out["sphere"] = sphere_measure(args.sphere_steps, args.sphere_adjoint_steps)
     if not args.no_cpu:

@@ -64,3 +64,48 @@ def test_lbfgs_on_the_adjoint_recovers_the_source():
           f"|eta - truth| {err0:.4e} -> {err:.4e}")
     assert r.fun < 0.1 * J0
     assert err < err0
+
+
+def test_lbfgs_calibrates_manning():
+    # friction calibration: a hump sloshing in a 3 m deep closed basin, gauges
+    # every 10 steps over 600 steps; the hidden Manning field varies in space and
+    # L-BFGS-B (n kept within [0.005, 0.08]) starts from a uniform 0.03
+    nx, ny, dx = 64, 64, 20.0
+    W, H = nx + 4, ny + 4
+    rows = swe_amd.geometry_cartesian(nx, ny, dx)
+    X, Y = np.meshgrid((np.arange(W) - 1.5) * dx, (np.arange(H) - 1.5) * dx)
+    b = (-3.0 + 0.3 * np.sin(X / 150.0) * np.cos(Y / 170.0)).ravel()
+    eta = (0.4 * np.exp(-((X - 400) ** 2 + (Y - 600) ** 2) / 150.0 ** 2)).ravel()
+    h0 = np.maximum(0.0, eta - b)
+    s = swe_amd.Solver(nx, ny, rows, manning=0.03, dx=dx, dy=dx)
+    z = np.zeros_like(h0)
+    s.load_state(b, h0, z, z)
+    s.set_bc(("reflect",) * 4)
+    nsteps, every = 600, 10
+    gauges = np.array([(6 + 16 * (q // 4)) * W + 6 + 16 * (q % 4) for q in range(16)], dtype=np.int32)
+    n_truth = (0.03 + 0.015 * np.sin(X / 200.0) * np.cos(Y / 260.0)).ravel()
+    adj = swe_amd.Adjoint(s, nsteps)
+    adj.set_gauges(gauges, every, np.zeros((nsteps // every, len(gauges))))
+    adj.forward(h0, n_truth)
+    adj.set_gauges(gauges, every, adj.series())
+    inner = np.zeros((H, W), bool)
+    inner[2:-2, 2:-2] = True
+    inner = inner.ravel()
+    calls = []
+
+    def f(n_in):
+        n = np.full(W * H, 0.03)
+        n[inner] = n_in
+        J = adj.forward(h0, n)
+        _, gn = adj.backward()
+        calls.append(J)
+        return J, gn[inner]
+
+    x0 = np.full(int(inner.sum()), 0.03)
+    J0 = f(x0)[0]
+    r = minimize(f, x0, jac=True, method="L-BFGS-B", bounds=[(0.005, 0.08)] * x0.size, options=dict(maxiter=20))
+    err0 = np.linalg.norm(x0 - n_truth[inner])
+    err = np.linalg.norm(r.x - n_truth[inner])
+    print(f"  {len(calls)} forward/backward pairs: misfit {J0:.4e} -> {r.fun:.4e}; |n - truth| {err0:.4e} -> {err:.4e}")
+    assert r.fun < 0.1 * J0
+    assert err < err0

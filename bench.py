@@ -345,6 +345,40 @@ def sphere_measure(steps, adj_steps, every=10):
     return out
 
 
+def diagnostics_measure(sim, swe_amd, nx, ny, reps=20):
+    """Device diagnostics of the 4096^2 state (kernel time from HIP events, 32 B per
+    interior cell read) against a host copy of b, h, hu, hv; and the gauge recorder's
+    cost per step (1024 gauges sampled every step) against unrecorded steps."""
+    log("[bench] device diagnostics and gauge recorder")
+    for _ in range(3):
+        sim.diagnostics(0.0)
+    kern, t0 = [], time.perf_counter()
+    for _ in range(reps):
+        d = sim.diagnostics(0.0)
+        kern.append(sim.get_scalar(swe_amd.S_DIAG_MS))
+    wall = (time.perf_counter() - t0) / reps
+    k_ms = sum(kern) / reps
+    gbs = 32.0 * nx * ny / (k_ms * 1e-3) / 1e9
+    t0 = time.perf_counter()
+    sim.state()
+    copy_s = time.perf_counter() - t0
+    steps = 100
+    t = sim.t
+    sim.advance(steps)
+    plain = sim.last_advance_ms() / steps
+    W = nx + 4
+    cells = [(2 + (q // 32) * (ny // 32)) * W + 2 + (q % 32) * (nx // 32) for q in range(1024)]
+    sim.set_recorder(cells, every=1, capacity=steps)
+    sim.advance(steps)
+    recorded = sim.last_advance_ms() / steps
+    _, eta = sim.record()
+    sim.set_recorder([], every=1)
+    return {"kernel_us": 1e3 * k_ms, "call_us": 1e6 * wall, "bytes_per_cell": 32, "achieved_GBs": gbs,
+            "frac_hbm": gbs / HBM_PEAK_GBS, "host_copy_ms": 1e3 * copy_s, "values": d,
+            "recorder": {"gauges": len(cells), "every": 1, "ms_per_step_plain": plain,
+                         "ms_per_step_recorded": recorded, "samples": int(eta.shape[0]), "t_from": t}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -408,6 +442,7 @@ def main():
                 "unit": "wave64 FP64-issue instr/s", "frac": rate / VALU_PEAK_WAVE_INSTR_S,
                 "instructions_per_step": i1 + i2}
     finite = sim.get_scalar(swe_amd.S_NONFINITE) == 0
+    diag = diagnostics_measure(sim, swe_amd, nx, ny)
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": n,
         "warmup": args.warmup, "ms_per_step": 1e3 * el / n, "higher_is_better": True, "scaling": "weak",
@@ -423,6 +458,7 @@ def main():
         # launch from the committed profile, which holds for its 94-row chunks)
         "valu": valu,
         "finite": finite,
+        "diagnostics": diag,
     }
     sim.close()
     if args.batched_steps > 0:

@@ -46,9 +46,14 @@ enum swe_row { SWE_ROW_DX = 0, SWE_ROW_DYN = 1, SWE_ROW_DYS = 2, SWE_ROW_DYCFL =
 /* scalars settable after creation */
 enum swe_scalar { SWE_S_MANNING = 0, SWE_S_CORIOLIS = 1, SWE_S_CFL = 2, SWE_S_T = 3,
                   SWE_S_SPONGE_REF = 4, SWE_S_STEPS = 5, SWE_S_MASS_FLOORED = 6,
-                  SWE_S_DT_NEXT = 7, SWE_S_NONFINITE = 8, SWE_S_CHUNK_ROWS = 9 };
-/* dt policy of swe_advance */
-enum swe_dt_mode { SWE_DT_SELF = 0, SWE_DT_FIXED = 1, SWE_DT_UNTIL = 2 };
+                  SWE_S_DT_NEXT = 7, SWE_S_NONFINITE = 8, SWE_S_CHUNK_ROWS = 9,
+                  SWE_S_DIAG_MS = 10 };
+/* dt policy of swe_advance; SAMPLED: the sample-time schedule of the recorder
+ * (swe_set_recorder with every = 0) up to t_end */
+enum swe_dt_mode { SWE_DT_SELF = 0, SWE_DT_FIXED = 1, SWE_DT_UNTIL = 2, SWE_DT_SAMPLED = 3 };
+/* results of swe_diagnostics */
+enum swe_diag { SWE_D_VOLUME = 0, SWE_D_ENERGY = 1, SWE_D_MAX_SURFACE_DEVIATION = 2, SWE_D_MAX_SPEED = 3,
+                SWE_D_FINITE = 4, SWE_D_COUNT = 5 };
 
 typedef struct swe_desc {
   int32_t nx, ny;           /* interior cells */
@@ -162,6 +167,29 @@ int swe_sync(swe_ctx *ctx);
 int swe_device_ptr(swe_ctx *ctx, int field, double **dptr);
 /* elapsed ms of the last swe_advance measured with HIP events on the context's stream */
 int swe_last_advance_ms(swe_ctx *ctx, double *ms);
+/* The reference's diagnostics of the current device state, computed on the
+ * device with no copy of the fields (volume swe.mjs:819-837, energy :864-879,
+ * maxSurfaceDeviation(eta_ref) :882-894, maxSpeed :897-909, finite :912-917;
+ * out[SWE_D_FINITE] is 1 or 0).  Energy, the spherical volume, both maxima and
+ * finite equal the reference's bit for bit (the same per-row and row orders);
+ * the Cartesian volume, one running sum over all cells in the reference, is the
+ * exactly rounded sum here.  SWE_S_DIAG_MS gives the kernels' duration. */
+int swe_diagnostics(swe_ctx *ctx, double eta_ref, double out[SWE_D_COUNT]);
+/* Gauge recorder of swe_advance (no reference counterpart: tools/waves.mjs:211-232
+ * records sim.eta(i, j) = b + h at chosen cells from JS after each step).  Arms
+ * the recorder with ng cells (indices into W*H) and room for `capacity` samples:
+ * every > 0 samples after every every-th step of later swe_advance calls; every
+ * = 0 samples at the times t0, t0 + dt_rec, ... as waves.mjs does, each sample
+ * taken after the first step that ends at or after its time, and
+ * swe_advance(SWE_DT_SAMPLED, t_end) shortens steps to land on those times
+ * (min(maxDt, max(1e-6, min(t_end, next) - t)), waves.mjs:222-223).  ng = 0
+ * disarms it.  Clears earlier samples. */
+int swe_set_recorder(swe_ctx *ctx, int ng, const int32_t *cells, int every, double t0, double dt_rec,
+                     int64_t capacity);
+/* The samples so far: eta (min(count, capacity) rows of ng values) and their
+ * times t; either may be NULL.  SWE_ESTATE when more samples were due than fit
+ * (the first `capacity` are returned). */
+int swe_get_record(swe_ctx *ctx, double *eta, double *times, int64_t *count);
 
 /* ---- batched ensembles (no reference counterpart: the reference runs one
  * ShallowWater per JS object; these advance many uneven basins in one launch

@@ -51,3 +51,9 @@ def test_js_restated_verification_suites_on_gpu():
 @pytest.mark.gpu
 def test_js_adjoint_and_tide_on_gpu():
     assert run_node("tests/js/adjoint_gpu.mjs") == 0
+
+
+@needs_node
+@pytest.mark.gpu
+def test_js_device_diagnostics_and_recorded_runs_on_gpu():
+    assert run_node("tests/js/record_gpu.mjs") == 0

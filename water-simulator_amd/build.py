@@ -10,7 +10,8 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = ["csrc/swe_kernels.hip", "csrc/swe_capi.hip", "csrc/swe_batch.hip", "csrc/swe_adjoint.hip"]
+SOURCES = ["csrc/swe_kernels.hip", "csrc/swe_capi.hip", "csrc/swe_batch.hip", "csrc/swe_adjoint.hip",
+           "csrc/swe_diag.hip"]
 OUT = os.path.join(HERE, "libswe_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17",

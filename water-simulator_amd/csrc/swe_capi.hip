@@ -204,6 +204,8 @@ void swe_destroy(swe_ctx *c) {
   (void)hipFree(c->mfield); (void)hipFree(c->sponge); (void)hipFree(c->tide);
   (void)hipFree(c->rh); (void)hipFree(c->theta);
   (void)hipFree(c->dctl); (void)hipFree(c->mf_part);
+  (void)hipFree(c->rec_cells); (void)hipFree(c->rec_series); (void)hipFree(c->rec_times);
+  (void)hipFree(c->diag);
   if (c->hctl) (void)hipHostFree(c->hctl);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -336,6 +338,7 @@ int swe_get_scalar(swe_ctx *c, int which, double *v) {
     case SWE_S_DT_NEXT: *v = c->hctl->dt_next; break;
     case SWE_S_NONFINITE: *v = c->hctl->nonfinite; break;
     case SWE_S_CHUNK_ROWS: *v = c->chunk; break;
+    case SWE_S_DIAG_MS: *v = c->diag_ms; break;
     default: return fail(c, SWE_EINVAL, "unknown scalar");
   }
   return SWE_OK;
@@ -432,7 +435,8 @@ int swe_max_dt(swe_ctx *c, double *out) {
 // one step on the device, stream-ordered; the caller handles the ctl block.
 // applyBC before each stage (swe.mjs:784,798) writes the ghost ring of that
 // stage's input; a refused step still gets its applyBC, as in the reference.
-static int enqueue_step(swe_ctx *c, int want_theta, int want_rh) {
+// rec: sample the gauge recorder after the step (device-resident runs)
+static int enqueue_step(swe_ctx *c, int want_theta, int want_rh, bool rec = false) {
   const int np = sweep_waves(c->P, c->chunk);
   if (np > c->max_parts) return fail(c, SWE_EINVAL, "chunk too small");
   StateW SA = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
@@ -456,7 +460,12 @@ static int enqueue_step(swe_ctx *c, int want_theta, int want_rh) {
     b2.theta = c->theta; b2.rh = c->rh; b2.rhu = c->rhu; b2.rhv = c->rhv;
   }
   HIPCHK(c, launch_sweep(MODE_STAGE2, b2, c->stream));
-  HIPCHK(c, launch_finalize(c->dctl, c->mf_part, np, c->P.cfl, 1, c->stream));
+  if (rec && c->rec_n > 0) {
+    const RecArgs r = {c->st[c->C][0], c->b, c->rec_cells, c->rec_series, c->rec_times, c->rec_n, c->rec_cap};
+    HIPCHK(c, launch_finalize(c->dctl, c->mf_part, np, c->P.cfl, 1, c->stream, &r));
+  } else {
+    HIPCHK(c, launch_finalize(c->dctl, c->mf_part, np, c->P.cfl, 1, c->stream));
+  }
   std::swap(c->A, c->C);
   return SWE_OK;
 }
@@ -637,7 +646,7 @@ static int build_graph(swe_ctx *c, int which) {
   // which = parity of the buffer rotation (A index 0 or 2)
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   const int saveA = c->A, saveC = c->C;
-  int rc = enqueue_step(c, 0, 0);
+  int rc = enqueue_step(c, 0, 0, true);
   c->A = saveA; c->C = saveC;
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(c->stream, &g);
@@ -650,14 +659,19 @@ static int build_graph(swe_ctx *c, int which) {
 
 int swe_advance(swe_ctx *c, int64_t nsteps, int dt_mode, double dt_value, double t_end, int64_t *taken) {
   if (!c) return fail(nullptr, SWE_EINVAL, "null context");
-  if (nsteps < 0 || dt_mode < SWE_DT_SELF || dt_mode > SWE_DT_UNTIL) return fail(c, SWE_EINVAL, "bad advance args");
+  if (nsteps < 0 || dt_mode < SWE_DT_SELF || dt_mode > SWE_DT_SAMPLED) return fail(c, SWE_EINVAL, "bad advance args");
   for (int s = 0; s < 4; ++s)
     if (c->P.bc[s] == BC_STORED) return fail(c, SWE_EINVAL, "swe_advance needs device boundary kinds on every side");
   int rc = pull_ctl(c);
   if (rc) return rc;
   Ctl *h = c->hctl;
+  if (dt_mode == SWE_DT_SAMPLED && (c->rec_n == 0 || h->rec_every != 0))
+    return fail(c, SWE_EINVAL, "SWE_DT_SAMPLED needs a recorder armed with sample times (every = 0)");
   const long long steps0 = h->steps;
-  h->dt_mode = dt_mode == SWE_DT_SELF ? DT_SELF : dt_mode == SWE_DT_FIXED ? DT_FIXED : DT_UNTIL;
+  h->dt_mode = dt_mode == SWE_DT_SELF    ? DT_SELF
+               : dt_mode == SWE_DT_FIXED ? DT_FIXED
+               : dt_mode == SWE_DT_UNTIL ? DT_UNTIL
+                                         : DT_SAMPLED;
   h->dt_fixed = dt_value;
   h->t_end = t_end;
   h->cfl_bits = 0;
@@ -667,14 +681,20 @@ int swe_advance(swe_ctx *c, int64_t nsteps, int dt_mode, double dt_value, double
   HIPCHK(c, launch_finalize(c->dctl, c->mf_part, 0, c->P.cfl, 0, c->stream));
   if (c->use_graph && c->graph_chunk != c->chunk) { drop_graphs(c); c->graph_chunk = c->chunk; }
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  // a sampled run ends at t_end: its length is checked every SAMPLED_BATCH steps
+  constexpr int64_t SAMPLED_BATCH = 256;
   for (int64_t n = 0; n < nsteps; ++n) {
+    if (dt_mode == SWE_DT_SAMPLED && n > 0 && n % SAMPLED_BATCH == 0) {
+      if ((rc = pull_ctl(c))) return rc;
+      if (!(h->t < t_end)) break;
+    }
     if (c->use_graph) {
       const int which = c->A == 0 ? 0 : 1;
       if (!c->gexec[which]) { if ((rc = build_graph(c, which))) return rc; }
       HIPCHK(c, hipGraphLaunch(c->gexec[which], c->stream));
       std::swap(c->A, c->C);
     } else {
-      if ((rc = enqueue_step(c, 0, 0))) return rc;
+      if ((rc = enqueue_step(c, 0, 0, true))) return rc;
     }
   }
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -753,6 +773,61 @@ int swe_last_advance_ms(swe_ctx *c, double *ms) {
   if (!c || !ms) return fail(c, SWE_EINVAL, "null argument");
   *ms = c->last_ms;
   return SWE_OK;
+}
+
+int swe_set_recorder(swe_ctx *c, int ng, const int32_t *cells, int every, double t0, double dt_rec,
+                     int64_t capacity) {
+  if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  if (ng < 0 || (ng > 0 && (!cells || capacity < 1 || every < 0)))
+    return fail(c, SWE_EINVAL, "recorder: bad arguments");
+  if (ng > 0 && every == 0 && !(dt_rec > 0 && std::isfinite(dt_rec) && std::isfinite(t0)))
+    return fail(c, SWE_EINVAL, "recorder: sample times need a finite t0 and dt_rec > 0");
+  for (int g = 0; g < ng; ++g)
+    if (cells[g] < 0 || (size_t)cells[g] >= c->N) return fail(c, SWE_EINVAL, "recorder: gauge cell out of range");
+  if ((double)ng * (double)capacity * sizeof(double) > 64.0 * (1ull << 30))
+    return fail(c, SWE_EINVAL, "recorder: more than 64 GiB of samples");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);     // graphs bake the recorder's buffers in
+  (void)hipFree(c->rec_cells); (void)hipFree(c->rec_series); (void)hipFree(c->rec_times);
+  c->rec_cells = nullptr; c->rec_series = nullptr; c->rec_times = nullptr;
+  c->rec_n = 0; c->rec_cap = 0;
+  int rc = pull_ctl(c);
+  if (rc) return rc;
+  Ctl *h = c->hctl;
+  h->rec_every = every; h->rec_flag = 0;
+  h->rec_count = 0; h->rec_row = 0; h->rec_since = 0;
+  h->rec_t0 = t0; h->rec_next = t0; h->rec_dt = dt_rec;
+  if (ng > 0) {
+    hipError_t e;
+    if ((e = hipMalloc(&c->rec_cells, sizeof(int) * ng)) != hipSuccess ||
+        (e = hipMalloc(&c->rec_series, sizeof(double) * (size_t)ng * capacity)) != hipSuccess ||
+        (e = hipMalloc(&c->rec_times, sizeof(double) * capacity)) != hipSuccess) {
+      (void)hipFree(c->rec_cells); (void)hipFree(c->rec_series); (void)hipFree(c->rec_times);
+      c->rec_cells = nullptr; c->rec_series = nullptr; c->rec_times = nullptr;
+      return fail(c, e == hipErrorOutOfMemory ? SWE_ENOMEM : SWE_EHIP, std::string("recorder: ") + hipGetErrorString(e));
+    }
+    HIPCHK(c, hipMemcpyAsync(c->rec_cells, cells, sizeof(int) * ng, hipMemcpyHostToDevice, c->stream));
+    c->rec_n = ng;
+    c->rec_cap = capacity;
+  }
+  if ((rc = push_ctl(c))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SWE_OK;
+}
+
+int swe_get_record(swe_ctx *c, double *eta, double *times, int64_t *count) {
+  if (!c || !count) return fail(c, SWE_EINVAL, "null argument");
+  int rc = pull_ctl(c);
+  if (rc) return rc;
+  const long long n = c->hctl->rec_count;
+  *count = n;
+  const long long rows = n < c->rec_cap ? n : c->rec_cap;
+  if (rows > 0 && eta)
+    HIPCHK(c, hipMemcpyAsync(eta, c->rec_series, sizeof(double) * (size_t)rows * c->rec_n, hipMemcpyDeviceToHost, c->stream));
+  if (rows > 0 && times)
+    HIPCHK(c, hipMemcpyAsync(times, c->rec_times, sizeof(double) * (size_t)rows, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return n > c->rec_cap ? fail(c, SWE_ESTATE, "recorder: more samples than its capacity (the first ones kept)") : SWE_OK;
 }
 
 }  // extern "C"

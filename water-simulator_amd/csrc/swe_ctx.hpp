@@ -15,7 +15,9 @@ int sweep_waves(const Params &P, int rows_per_chunk);
 const void *k_sweep_stage1_ptr();
 hipError_t launch_apply_bc(const Params &P, StateW S, double *b, const Ctl *ctl, hipStream_t st);
 hipError_t launch_maxdt(const Params &P, StateP S, Ctl *ctl, hipStream_t st);
-hipError_t launch_finalize(Ctl *ctl, const double *mf_part, int nparts, double cfl, int phase, hipStream_t st);
+// rec: the gauge recorder of a device-resident step (nullptr: none)
+hipError_t launch_finalize(Ctl *ctl, const double *mf_part, int nparts, double cfl, int phase, hipStream_t st,
+                           const RecArgs *rec = nullptr);
 hipError_t launch_friction(const Params &P, StateW S, const double *mfield, double dt, hipStream_t st);
 hipError_t launch_dry_clean(const Params &P, StateW S, double *part, int blocks, hipStream_t st);
 hipError_t launch_tide_times(Ctl *ctl, hipStream_t st);
@@ -57,6 +59,15 @@ struct swe_ctx {
   int graph_chunk = -1;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
+  // gauge recorder of swe_advance: cells, rec_cap sample rows of rec_n values, times
+  int *rec_cells = nullptr;
+  double *rec_series = nullptr, *rec_times = nullptr;
+  int rec_n = 0;
+  long long rec_cap = 0;
+  // device diagnostics: per-row sums, per-block partials, maxima, result
+  double *diag = nullptr;
+  int diag_blocks = 0;
+  double diag_ms = 0.0;       // duration of the last swe_diagnostics' kernels
   std::string err;
 };
 

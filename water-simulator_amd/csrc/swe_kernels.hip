@@ -512,6 +512,9 @@ __device__ __forceinline__ void apply_bc_cell(const Params &P, StateW S, double 
 __global__ void k_apply_bc(Params P, StateW S, double *b, const Ctl *ctl) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 4 * P.W + 4 * (P.H - 4)) return;
+  // the steps a sampled run launches past its end are not steps of the run:
+  // they leave even the ghost ring alone
+  if (ctl->dt_mode == DT_SAMPLED && ctl->idle) return;
   apply_bc_cell(P, S, b, ctl->ext, t);
 }
 
@@ -620,7 +623,8 @@ __device__ __forceinline__ double harm_eval(int n, double c0, const double *amp,
 //   phase 0: after k_maxdt -- dt for the step about to run
 //   phase 1: after a step -- massFloored, t, steps, dt_next, next dt and forcing
 // s1, s2: the two stages' massFloored sums (phase 1); the body runs on one lane.
-__device__ __forceinline__ void finalize_ctl(Ctl *ctl, double s1, double s2, double cfl, int phase) {
+// rec: the step's gauge recorder is armed (device-resident runs only).
+__device__ __forceinline__ void finalize_ctl(Ctl *ctl, double s1, double s2, double cfl, int phase, bool rec = false) {
   const unsigned long long bits = ctl->cfl_bits;
   const double inv = __longlong_as_double((long long)bits);
   const double dtn = bits ? cfl / inv : __builtin_inf();
@@ -628,11 +632,24 @@ __device__ __forceinline__ void finalize_ctl(Ctl *ctl, double s1, double s2, dou
   ctl->dt_next = dtn;
   if (phase == 1) {
     const double dt = ctl->dt;
+    ctl->rec_flag = 0;
     if ((dt > 0) && !__builtin_isinf(dt)) {
       ctl->mf = s1 + s2;
       ctl->t = ctl->t + dt;
       ctl->steps = ctl->steps + 1;
       ctl->dt_last = dt;
+      if (rec) {
+        bool take = false;
+        if (ctl->rec_every > 0) {
+          if (++ctl->rec_since >= ctl->rec_every) { ctl->rec_since = 0; take = true; }
+        } else if (ctl->t >= ctl->rec_next) {
+          // tools/waves.mjs:224-227: a sample once t reaches the next sample
+          // time (from the analysis start on), and the next time moves on
+          take = ctl->t >= ctl->rec_t0;
+          ctl->rec_next = ctl->rec_next + ctl->rec_dt;
+        }
+        if (take) { ctl->rec_row = ctl->rec_count; ctl->rec_count = ctl->rec_count + 1; ctl->rec_flag = 1; }
+      }
     } else {
       ctl->dt_last = 0.0;
     }
@@ -640,6 +657,11 @@ __device__ __forceinline__ void finalize_ctl(Ctl *ctl, double s1, double s2, dou
   if (ctl->dt_mode == DT_SELF) ctl->dt = dtn;
   else if (ctl->dt_mode == DT_FIXED) ctl->dt = ctl->dt_fixed;
   else if (ctl->dt_mode == DT_UNTIL) ctl->dt = fmin(dtn, ctl->t_end - ctl->t);
+  else if (ctl->dt_mode == DT_SAMPLED)
+    // tools/waves.mjs:222-223: while (t < T) step(min(maxDt, max(1e-6, min(T, next) - t)));
+    // at t >= T the step is refused (dt 0), which leaves the state as it is
+    ctl->dt = ctl->t < ctl->t_end ? jmin(dtn, jmax(1e-6, jmin(ctl->t_end, ctl->rec_next) - ctl->t)) : 0.0;
+  ctl->idle = ctl->dt_mode == DT_SAMPLED && !(ctl->t < ctl->t_end);
   tide_times(ctl);
   if (ctl->harm_on) {
     for (int side = 0; side < 4; ++side) {
@@ -666,10 +688,15 @@ __device__ __forceinline__ void finalize_body(Ctl *ctl, const double *mf_part, i
 // per-thread sums, a shuffle tree per wavefront, then the wavefronts' sums in
 // order, so the result is the same on every run.
 constexpr int FIN_THREADS = 1024;
+// With a recorder (r.cells set, phase 1), the step's gauge sample is written by
+// the whole block once the control lane has decided on it.
 __global__ __launch_bounds__(FIN_THREADS) void k_finalize(Ctl *ctl, const double *mf_part, int nparts, double cfl,
-                                                          int phase) {
+                                                          int phase, RecArgs r) {
   __shared__ double w1[FIN_THREADS / 64], w2[FIN_THREADS / 64];
+  __shared__ long long rrow;
+  __shared__ double rt;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const bool rec = r.cells != nullptr && phase == 1;
   double s1 = 0.0, s2 = 0.0;
   if (phase == 1) {
     for (int i = t; i < nparts; i += FIN_THREADS) { s1 += mf_part[i]; s2 += mf_part[nparts + i]; }
@@ -681,7 +708,21 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize(Ctl *ctl, const double
       for (int q = 0; q < FIN_THREADS / 64; ++q) { s1 += w1[q]; s2 += w2[q]; }
     }
   }
-  if (t == 0) finalize_ctl(ctl, s1, s2, cfl, phase);
+  if (t == 0) {
+    finalize_ctl(ctl, s1, s2, cfl, phase, rec);
+    rrow = (rec && ctl->rec_flag) ? ctl->rec_row : -1;
+    rt = ctl->t;
+  }
+  if (!rec) return;
+  __syncthreads();
+  const long long row = rrow;
+  if (row < 0 || row >= r.cap) return;
+  // tools/waves.mjs:225: sim.eta(i, j) = b + h
+  for (int g = t; g < r.n; g += FIN_THREADS) {
+    const int k = r.cells[g];
+    r.series[row * r.n + g] = r.b[k] + r.h[k];
+  }
+  if (t == 0) r.times[row] = rt;
 }
 
 // ---- batched ensembles: one launch per phase for all basins -----------------
@@ -765,8 +806,11 @@ hipError_t launch_maxdt(const Params &P, StateP S, Ctl *ctl, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_finalize(Ctl *ctl, const double *mf_part, int nparts, double cfl, int phase, hipStream_t st) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, st, ctl, mf_part, nparts, cfl, phase);
+hipError_t launch_finalize(Ctl *ctl, const double *mf_part, int nparts, double cfl, int phase, hipStream_t st,
+                           const RecArgs *rec) {
+  RecArgs r{};
+  if (rec) r = *rec;
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, st, ctl, mf_part, nparts, cfl, phase, r);
   return hipGetLastError();
 }
 

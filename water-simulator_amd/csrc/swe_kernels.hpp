@@ -16,7 +16,7 @@ namespace swe {
 
 enum BcKind { BC_REFLECT = 0, BC_OUTFLOW = 1, BC_PERIODIC = 2, BC_FLATHER = 3, BC_STORED = 4 };
 enum Mode { MODE_STAGE1 = 1, MODE_STAGE2 = 2, MODE_PROBE = 3 };
-enum DtMode { DT_SELF = 0, DT_FIXED = 1, DT_UNTIL = 2, DT_HOST = 3 };
+enum DtMode { DT_SELF = 0, DT_FIXED = 1, DT_UNTIL = 2, DT_HOST = 3, DT_SAMPLED = 4 };
 enum { MAX_HARM = 4, MAX_TIDE = 4 };
 
 struct Geo {
@@ -43,7 +43,8 @@ struct Ctl {
   double dt_fixed, t_end;
   unsigned long long cfl_bits;   // max-reduction accumulator (bit pattern of a non-negative double)
   long long steps;
-  int dt_mode, nonfinite, harm_on, pad;
+  int dt_mode, nonfinite, harm_on;
+  int idle;            // DT_SAMPLED run past t_end: the remaining launched steps do nothing
   double dt_last;      // dt the last step ran with (0 if it was refused)
   Harm harm[4];
   // equilibrium-tide constituents: angular frequency and phase, and
@@ -52,6 +53,23 @@ struct Ctl {
   int tide_n, tide_pad;
   double tide_om[MAX_TIDE], tide_ph[MAX_TIDE];
   double tide_cs[2][MAX_TIDE][2];
+  // gauge recorder of device-resident runs: after a step that ran, eta = b + h at
+  // the recorder's cells goes to sample row rec_row when rec_flag is set; samples
+  // are taken every rec_every-th step, or (rec_every == 0) at the first step
+  // ending at or after each sample time rec_next = rec_t0, rec_t0 + rec_dt, ...
+  int rec_every, rec_flag;
+  long long rec_count, rec_row, rec_since;
+  double rec_next, rec_t0, rec_dt;
+};
+
+// Gauge recorder arguments of k_finalize (cells == nullptr: no recorder).
+struct RecArgs {
+  const double *h, *b;   // the state after the step
+  const int *cells;
+  double *series;        // rec_cap rows of n values
+  double *times;         // rec_cap sample times
+  int n;
+  long long cap;
 };
 
 struct Params {

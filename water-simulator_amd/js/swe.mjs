@@ -279,6 +279,53 @@ export class ShallowWater {
     return this;
   }
 
+  /**
+   * tools/waves.mjs's recording loop (waves.mjs:211-232) run on the device, with
+   * no counterpart in the reference: from sim.t until `until`, each step is
+   * min(maxDt, max(1e-6, min(until, next) - t)), and eta = b + h at `cells`
+   * (indices into W*H) is sampled after the first step at or past each sample
+   * time t0, t0 + dtRec, ...  Returns { times, eta } (eta: samples x cells) and
+   * leaves the state as the host loop would.
+   */
+  recordRun(cells, { t0 = this.t, dtRec, until, capacity = 0 }) {
+    this._boundaries();
+    this._syncParams();
+    if (this.forcing) throw new Error('recordRun runs without a host forcing hook');
+    const c = Int32Array.from(cells);
+    const cap = capacity > 0 ? capacity : Math.ceil((until - t0) / dtRec) + 2;
+    addon.setScalar(this._dev, S.T, this.t);
+    this._upload(['B', 'H', 'HU', 'HV']);
+    addon.setRecorder(this._dev, c, 0, t0, dtRec, cap);
+    try {
+      const taken = addon.advance(this._dev, Number.MAX_SAFE_INTEGER, 3, 0, until);
+      const n = addon.getRecord(this._dev, null, null);
+      const eta = new Float64Array(n * c.length), times = new Float64Array(n);
+      addon.getRecord(this._dev, eta, times);
+      this._download(['B', 'H', 'HU', 'HV']);
+      this.t = addon.getScalar(this._dev, S.T);
+      this.steps += taken;
+      return { times, eta };
+    } finally {
+      addon.setRecorder(this._dev, new Int32Array(0), 0, 0, 0, 0);
+    }
+  }
+
+  /**
+   * volume, energy, maxSurfaceDeviation(ref), maxSpeed and finite of the device
+   * state, computed on the device (no counterpart in the reference, whose
+   * methods below loop over the arrays).  The device state is the arrays'
+   * state after step(), advance() and recordRun(); writes to sim.h etc. reach
+   * the device with the next step.  Energy, the maxima, finite and the
+   * spherical volume equal the host methods bit for bit; the Cartesian volume
+   * is the exactly rounded sum (the host method's running sum rounds at each
+   * cell).
+   */
+  deviceDiagnostics(ref = 0) {
+    const out = new Float64Array(5);
+    addon.diagnostics(this._dev, ref, out);
+    return { volume: out[0], energy: out[1], maxSurfaceDeviation: out[2], maxSpeed: out[3], finite: out[4] === 1 };
+  }
+
   // ---- diagnostics (host, over the coherent mirror) ---------------------------
   volume() {
     const { W, ng, nx, ny } = this;

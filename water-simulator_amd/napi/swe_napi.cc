@@ -275,6 +275,59 @@ napi_value Advance(napi_env env, napi_callback_info info) {
   return num(env, (double)taken);
 }
 
+// diagnostics(h, ref, Float64Array out[5]): volume, energy, maxSurfaceDeviation, maxSpeed, finite
+napi_value Diagnostics(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  swe_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  double *out = nullptr;
+  size_t n = 0;
+  if (!get_f64(env, argv[2], &out, &n) || !out || n < SWE_D_COUNT) {
+    napi_throw_type_error(env, nullptr, "diagnostics: Float64Array(5)");
+    return nullptr;
+  }
+  if (swe_diagnostics(c, get_double(env, argv[1]), out) != SWE_OK) return throw_swe(env, c, "swe_diagnostics");
+  return nullptr;
+}
+
+bool get_i32(napi_env env, napi_value v, int32_t **data, size_t *len);
+
+// setRecorder(h, Int32Array cells, every, t0, dtRec, capacity)
+napi_value SetRecorder(napi_env env, napi_callback_info info) {
+  napi_value argv[6];
+  if (!get_args(env, info, 6, argv)) return nullptr;
+  swe_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  int32_t *cells = nullptr;
+  size_t n = 0;
+  if (!get_i32(env, argv[1], &cells, &n)) {
+    napi_throw_type_error(env, nullptr, "setRecorder: Int32Array cells");
+    return nullptr;
+  }
+  if (swe_set_recorder(c, (int)n, cells, get_int(env, argv[2]), get_double(env, argv[3]), get_double(env, argv[4]),
+                       (int64_t)get_double(env, argv[5])) != SWE_OK)
+    return throw_swe(env, c, "swe_set_recorder");
+  return nullptr;
+}
+
+// getRecord(h, Float64Array eta|null, Float64Array times|null) -> samples recorded
+napi_value GetRecord(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  swe_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  double *eta = nullptr, *times = nullptr;
+  size_t n1 = 0, n2 = 0;
+  if (!get_f64(env, argv[1], &eta, &n1) || !get_f64(env, argv[2], &times, &n2)) {
+    napi_throw_type_error(env, nullptr, "getRecord: Float64Array or null");
+    return nullptr;
+  }
+  int64_t count = 0;
+  if (swe_get_record(c, eta, times, &count) != SWE_OK) return throw_swe(env, c, "swe_get_record");
+  return num(env, (double)count);
+}
+
 // hllc(hL, huL, hvL, hR, huR, hvR, dry, g) -> [m, n, t]
 napi_value Hllc(napi_env env, napi_callback_info info) {
   napi_value argv[8];
@@ -491,7 +544,8 @@ napi_value Init(napi_env env, napi_value exports) {
       {"jsPow", JsPow}, {"buildInfo", BuildInfo}, {"setTide", SetTide},
       {"adjCreate", AdjCreate}, {"adjSetGauges", AdjSetGauges}, {"adjForward", AdjForward}, {"adjLoss", AdjLoss},
       {"adjBackward", AdjBackward}, {"adjSeries", AdjSeries}, {"adjTiming", AdjTiming},
-      {"allocPinned", AllocPinned},
+      {"allocPinned", AllocPinned}, {"diagnostics", Diagnostics}, {"setRecorder", SetRecorder},
+      {"getRecord", GetRecord},
   };
   for (auto &f : fns) {
     napi_value fn;

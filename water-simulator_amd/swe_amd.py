@@ -25,8 +25,9 @@ F_B, F_H, F_HU, F_HV, F_MANNING, F_RH, F_RHU, F_RHV, F_THETA, F_SPONGE = range(1
 BC = {"reflect": 0, "outflow": 1, "periodic": 2, "flather": 3, "stored": 4}
 LIMITER = {"minmod": 0, "mc": 1, "central": 2}
 (S_MANNING, S_CORIOLIS, S_CFL, S_T, S_SPONGE_REF, S_STEPS, S_MASS_FLOORED, S_DT_NEXT, S_NONFINITE,
- S_CHUNK_ROWS) = range(10)
-DT_SELF, DT_FIXED, DT_UNTIL = 0, 1, 2
+ S_CHUNK_ROWS, S_DIAG_MS) = range(11)
+DT_SELF, DT_FIXED, DT_UNTIL, DT_SAMPLED = 0, 1, 2, 3
+DIAGNOSTICS = ("volume", "energy", "maxSurfaceDeviation", "maxSpeed", "finite")
 
 
 class SweError(RuntimeError):
@@ -79,6 +80,9 @@ def lib():
             "swe_profile_step": (i, [vp, i, dp]),
             "swe_autotune": (i, [vp, i, ctypes.POINTER(i)]),
             "swe_set_tide": (i, [vp, i, ctypes.POINTER(ctypes.c_int32), dp, dp, dp, dp, dp]),
+            "swe_diagnostics": (i, [vp, d, dp]),
+            "swe_set_recorder": (i, [vp, i, ctypes.POINTER(ctypes.c_int32), i, d, d, i64]),
+            "swe_get_record": (i, [vp, dp, dp, ctypes.POINTER(i64)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -133,6 +137,7 @@ class Solver:
         self._h = h
         self._mf = 0.0
         self._probe_rh = False
+        self._rec_n = 0
 
     def _ck(self, rc, what):
         if rc != 0:
@@ -254,6 +259,35 @@ class Solver:
         v = ctypes.c_double()
         self._ck(lib().swe_last_advance_ms(self._h, ctypes.byref(v)), "swe_last_advance_ms")
         return v.value
+
+    # ---- device-resident diagnostics and gauges ----------------------------------
+    def diagnostics(self, eta_ref=0.0):
+        """volume, energy, maxSurfaceDeviation(eta_ref), maxSpeed and finite of the
+        device state (swe.mjs:819-917) without copying it to the host."""
+        out = np.zeros(5)
+        self._ck(lib().swe_diagnostics(self._h, float(eta_ref), _dp(out)), "swe_diagnostics")
+        r = dict(zip(DIAGNOSTICS, out.tolist()))
+        r["finite"] = bool(r["finite"])
+        return r
+
+    def set_recorder(self, cells, every=0, t0=0.0, dt_rec=0.0, capacity=1024):
+        """Arm the gauge recorder of advance(): eta = b + h at `cells` after every
+        `every`-th step, or (every=0) at the sample times t0, t0 + dt_rec, ...
+        (tools/waves.mjs:211-232; advance(n, DT_SAMPLED, t_end=T) lands on them).
+        An empty `cells` disarms it."""
+        c = np.ascontiguousarray(cells, dtype=np.int32)
+        self._rec_n = c.size
+        self._ck(lib().swe_set_recorder(self._h, c.size, c.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                        int(every), float(t0), float(dt_rec), int(capacity)), "swe_set_recorder")
+
+    def record(self):
+        """(times, eta): the samples so far, eta of shape (samples, gauges)."""
+        n = ctypes.c_int64()
+        self._ck(lib().swe_get_record(self._h, None, None, ctypes.byref(n)), "swe_get_record")
+        eta = np.zeros((n.value, self._rec_n))
+        times = np.zeros(n.value)
+        self._ck(lib().swe_get_record(self._h, _dp(eta), _dp(times), ctypes.byref(n)), "swe_get_record")
+        return times, eta
 
     def set_harmonic(self, side, amp=(), omega=(), phase=(), msl=0.0, uamp=(), uomega=(), uphase=(), u0=0.0):
         def arr(x):

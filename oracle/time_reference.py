@@ -3,14 +3,15 @@ time_reference.mjs: writes the workload's bed and initial surface (the same
 seeded recipes the benchmark uses, at the stated size) and a spec, and runs
 Node on them.  Build-container only: the reference does not travel.
 
-    python oracle/time_reference.py [--workload large|inversion|sphere] [--n 512] [--steps 5] [--reps 3]
+    python oracle/time_reference.py [--workload large|inversion|sphere|gridlands] [--n 512] [--steps 5] [--reps 3]
 
 large: the 4096x4096 recipe at n x n (dx = 10 m, west Flather M2 tide of 1 m,
 Manning 0.025, f = 1e-4); inversion: the 1024x1024 shelf at its full size
 (outflow west, walls elsewhere, Manning 0.025, f = 1e-4); sphere: member 0 of
 the 2160x1080 global ocean at its full size (periodic in longitude, walls at
 the poles, its Manning n; the reference has no equilibrium-tide forcing, so the
-step is timed without it).
+step is timed without it); gridlands: the settled synthetic gridlands-terrain
+export of the import tests (120 x 90 at 50 m, closed box, no friction).
 """
 import argparse
 import json
@@ -44,12 +45,23 @@ def workload(name, n):
                     manning=manning[0], coriolis=0.0, cfl=p["cfl"],
                     bc=["periodic", "periodic", "reflect", "reflect"], flather=None)
         return b, np.zeros_like(b), spec
+    if name == "gridlands":
+        import gzip
+        import json
+        import gridlands
+        with gzip.open(os.path.join(os.path.dirname(HERE), "tests", "golden", "gridlands_synth.terrain.json.gz"),
+                       "rt") as f:
+            t = gridlands.gridlands_terrain(json.load(f), settle=True)
+        b, h = gridlands.sample_state(t)
+        spec = dict(nx=t["W"], ny=t["H"], dx=t["cellMeters"], sphere=None, manning=0.0, coriolis=0.0, cfl=0.45,
+                    bc=["reflect"] * 4, flather=None)
+        return b, b + h, spec
     raise SystemExit(f"unknown workload {name}")
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="large", choices=["large", "inversion", "sphere"])
+    ap.add_argument("--workload", default="large", choices=["large", "inversion", "sphere", "gridlands"])
     ap.add_argument("--n", type=int, default=512, help="grid size of the large workload")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--reps", type=int, default=3)

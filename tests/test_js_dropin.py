@@ -57,3 +57,10 @@ def test_js_adjoint_and_tide_on_gpu():
 @pytest.mark.gpu
 def test_js_device_diagnostics_and_recorded_runs_on_gpu():
     assert run_node("tests/js/record_gpu.mjs") == 0
+
+
+@needs_node
+@pytest.mark.gpu
+def test_js_gridlands_import_gates_on_gpu():
+    """tools/verify-gridlands.mjs's gates through js/gridlands.mjs and the drop-in."""
+    assert run_node("tests/js/gridlands_gpu.mjs") == 0

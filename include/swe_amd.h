@@ -129,8 +129,9 @@ int swe_step(swe_ctx *ctx, double dt, const double ext[8], int want_theta, int w
              double *dt_taken, double *mass_floored);
 /* Device-resident run of nsteps steps (no host round trip): dt policy SELF
  * (maxDt every step), FIXED (dt_value) or UNTIL (min(maxDt, t_end - t)); the
- * flather forcing comes from swe_set_harmonic.  Stops early once a step is
- * refused. */
+ * flather forcing comes from swe_set_harmonic.  An UNTIL (or SAMPLED) run ends
+ * once t reaches t_end: the steps launched past it leave the state untouched,
+ * and the host stops launching within 256 steps. */
 int swe_advance(swe_ctx *ctx, int64_t nsteps, int dt_mode, double dt_value, double t_end,
                 int64_t *steps_taken);
 /* Equilibrium-tide body force (no reference counterpart: the reference leaves
@@ -190,6 +191,17 @@ int swe_set_recorder(swe_ctx *ctx, int ng, const int32_t *cells, int every, doub
  * times t; either may be NULL.  SWE_ESTATE when more samples were due than fit
  * (the first `capacity` are returned). */
 int swe_get_record(swe_ctx *ctx, double *eta, double *times, int64_t *count);
+/* Projection recorder of swe_advance (no counterpart in swe.mjs: the mode
+ * projection of src/globe.mjs:1417-1424, which measureModePeriod evaluates on
+ * the host after every step): after each step that runs, s = sum over interior
+ * rows j of coef[j] * (sum over the row of b + h), each sum left to right in
+ * the reference's order, is appended with the step's end time.  coef: ny
+ * values (globe.mjs uses (wRow_j / nx) * P_n(sin lat_j)); NULL disarms.
+ * Clears earlier samples. */
+int swe_set_projection(swe_ctx *ctx, const double *coef, int64_t capacity);
+/* The projection samples so far (min(count, capacity) of them) and their times;
+ * either may be NULL.  SWE_ESTATE when more were due than fit. */
+int swe_get_projection(swe_ctx *ctx, double *vals, double *times, int64_t *count);
 
 /* ---- batched ensembles (no reference counterpart: the reference runs one
  * ShallowWater per JS object; these advance many uneven basins in one launch

@@ -205,6 +205,7 @@ void swe_destroy(swe_ctx *c) {
   (void)hipFree(c->rh); (void)hipFree(c->theta);
   (void)hipFree(c->dctl); (void)hipFree(c->mf_part);
   (void)hipFree(c->rec_cells); (void)hipFree(c->rec_series); (void)hipFree(c->rec_times);
+  (void)hipFree(c->proj_coef); (void)hipFree(c->proj_vals); (void)hipFree(c->proj_times);
   (void)hipFree(c->diag);
   if (c->hctl) (void)hipHostFree(c->hctl);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -297,8 +298,15 @@ int swe_set_bc(swe_ctx *c, int side, int kind) {
   return SWE_OK;
 }
 
+static int pull_ctl(swe_ctx *c);
+
 int swe_set_scalar(swe_ctx *c, int which, double v) {
   if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  if (which == SWE_S_T || which == SWE_S_STEPS) {
+    // the rest of the control block (counters the device advanced) is kept
+    int rc = pull_ctl(c);
+    if (rc) return rc;
+  }
   switch (which) {
     case SWE_S_MANNING: c->manning = v; break;
     case SWE_S_CORIOLIS: c->coriolis = v; break;
@@ -466,6 +474,7 @@ static int enqueue_step(swe_ctx *c, int want_theta, int want_rh, bool rec = fals
   } else {
     HIPCHK(c, launch_finalize(c->dctl, c->mf_part, np, c->P.cfl, 1, c->stream));
   }
+  if (rec && c->proj_cap > 0) HIPCHK(c, launch_projection(c, c->st[c->C][0], c->b));
   std::swap(c->A, c->C);
   return SWE_OK;
 }
@@ -681,10 +690,10 @@ int swe_advance(swe_ctx *c, int64_t nsteps, int dt_mode, double dt_value, double
   HIPCHK(c, launch_finalize(c->dctl, c->mf_part, 0, c->P.cfl, 0, c->stream));
   if (c->use_graph && c->graph_chunk != c->chunk) { drop_graphs(c); c->graph_chunk = c->chunk; }
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-  // a sampled run ends at t_end: its length is checked every SAMPLED_BATCH steps
+  // a sampled or UNTIL run ends at t_end: its length is checked every SAMPLED_BATCH steps
   constexpr int64_t SAMPLED_BATCH = 256;
   for (int64_t n = 0; n < nsteps; ++n) {
-    if (dt_mode == SWE_DT_SAMPLED && n > 0 && n % SAMPLED_BATCH == 0) {
+    if ((dt_mode == SWE_DT_SAMPLED || dt_mode == SWE_DT_UNTIL) && n > 0 && n % SAMPLED_BATCH == 0) {
       if ((rc = pull_ctl(c))) return rc;
       if (!(h->t < t_end)) break;
     }
@@ -813,6 +822,54 @@ int swe_set_recorder(swe_ctx *c, int ng, const int32_t *cells, int every, double
   if ((rc = push_ctl(c))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return SWE_OK;
+}
+
+int swe_set_projection(swe_ctx *c, const double *coef, int64_t capacity) {
+  if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  if (coef && capacity < 1) return fail(c, SWE_EINVAL, "projection: capacity must be >= 1");
+  if (coef)
+    for (int j = 0; j < c->P.ny; ++j)
+      if (!std::isfinite(coef[j])) return fail(c, SWE_EINVAL, "projection: coefficients must be finite");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);     // graphs bake the recorder's buffers in
+  (void)hipFree(c->proj_coef); (void)hipFree(c->proj_vals); (void)hipFree(c->proj_times);
+  c->proj_coef = nullptr; c->proj_vals = nullptr; c->proj_times = nullptr;
+  c->proj_cap = 0;
+  int rc = pull_ctl(c);
+  if (rc) return rc;
+  c->hctl->proj_count = 0;
+  if (coef) {
+    if ((rc = ensure_diag(c))) return rc;
+    hipError_t e;
+    if ((e = hipMalloc(&c->proj_coef, sizeof(double) * c->P.ny)) != hipSuccess ||
+        (e = hipMalloc(&c->proj_vals, sizeof(double) * capacity)) != hipSuccess ||
+        (e = hipMalloc(&c->proj_times, sizeof(double) * capacity)) != hipSuccess) {
+      (void)hipFree(c->proj_coef); (void)hipFree(c->proj_vals); (void)hipFree(c->proj_times);
+      c->proj_coef = nullptr; c->proj_vals = nullptr; c->proj_times = nullptr;
+      return fail(c, e == hipErrorOutOfMemory ? SWE_ENOMEM : SWE_EHIP, std::string("projection: ") + hipGetErrorString(e));
+    }
+    HIPCHK(c, hipMemcpyAsync(c->proj_coef, coef, sizeof(double) * c->P.ny, hipMemcpyHostToDevice, c->stream));
+    c->proj_cap = capacity;
+  }
+  if ((rc = push_ctl(c))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SWE_OK;
+}
+
+int swe_get_projection(swe_ctx *c, double *vals, double *times, int64_t *count) {
+  if (!c || !count) return fail(c, SWE_EINVAL, "null argument");
+  int rc = pull_ctl(c);
+  if (rc) return rc;
+  const long long n = c->hctl->proj_count;
+  *count = n;
+  const long long rows = n < c->proj_cap ? n : c->proj_cap;
+  if (rows > 0 && vals)
+    HIPCHK(c, hipMemcpyAsync(vals, c->proj_vals, sizeof(double) * rows, hipMemcpyDeviceToHost, c->stream));
+  if (rows > 0 && times)
+    HIPCHK(c, hipMemcpyAsync(times, c->proj_times, sizeof(double) * rows, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return n > c->proj_cap ? fail(c, SWE_ESTATE, "projection: more samples than its capacity (the first ones kept)")
+                         : SWE_OK;
 }
 
 int swe_get_record(swe_ctx *c, double *eta, double *times, int64_t *count) {

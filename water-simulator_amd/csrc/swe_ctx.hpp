@@ -65,6 +65,9 @@ struct swe_ctx {
   int rec_n = 0;
   long long rec_cap = 0;
   // device diagnostics: per-row sums, per-block partials, maxima, result
+  // projection recorder of swe_advance: ny coefficients, proj_cap samples and times
+  double *proj_coef = nullptr, *proj_vals = nullptr, *proj_times = nullptr;
+  long long proj_cap = 0;
   double *diag = nullptr;
   int diag_blocks = 0;
   double diag_ms = 0.0;       // duration of the last swe_diagnostics' kernels
@@ -86,4 +89,8 @@ int ctx_pull(swe_ctx *c);
 int ctx_push(swe_ctx *c);
 SweepArgs ctx_base_args(swe_ctx *c);
 int ctx_enqueue_step(swe_ctx *c, int want_theta, int want_rh);
+// the diagnostics' device scratch (allocated on first use)
+int ensure_diag(swe_ctx *c);
+// the projection recorder's kernels on a state (after a step's finalize)
+hipError_t launch_projection(swe_ctx *c, const double *h, const double *b);
 }  // namespace swe

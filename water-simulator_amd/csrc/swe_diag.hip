@@ -56,6 +56,9 @@ __device__ __forceinline__ DD dd_merge(DD a, DD b) {
   return s;
 }
 
+// PROJ: only the row sums of the surface b + h (the mode projection of
+// globe.mjs:1417-1424), nothing else.
+template <bool PROJ>
 __global__ __launch_bounds__(DG_THREADS) void k_diag_rows(DiagArgs a) {
   // two tiles of the block's rows (h and energy terms), alternating, and the
   // running row sums between the wavefronts that take turns adding
@@ -89,7 +92,9 @@ __global__ __launch_bounds__(DG_THREADS) void k_diag_rows(DiagArgs a) {
       }
       // x + (-0) == x for every x: a skipped cell or a column past the grid adds nothing
       double tvv = -0.0, tee = -0.0;
-      if (j < a.ny && i < a.nx) {
+      if (PROJ) {
+        if (j < a.ny && i < a.nx) tvv = bk + hk;   // globe.mjs:1420: row += b + h
+      } else if (j < a.ny && i < a.nx) {
         tvv = hk;
         vol = dd_add(vol, hk);
         const bool eOn = !(hk <= 0);            // swe.mjs:869 (a NaN depth is not skipped)
@@ -140,6 +145,7 @@ __global__ __launch_bounds__(DG_THREADS) void k_diag_rows(DiagArgs a) {
   __syncthreads();
   if (t < 2 * DG_ROWS && blockIdx.x * DG_ROWS + (t >> 1) < a.ny)
     ((t & 1) ? a.row_e : a.row_v)[blockIdx.x * DG_ROWS + (t >> 1)] = racc[t];
+  if (PROJ) return;
   // finite() covers the ghost ring too (swe.mjs:912-917): 2 ng full rows, then
   // the 2 ng ghost columns of each interior row
   const int band = a.ng * a.W;
@@ -271,7 +277,74 @@ __global__ __launch_bounds__(DG_THREADS) void k_diag_final(DiagArgs a) {
   }
 }
 
+// The step's mode projection s = sum_j coef_j * (row sum of b + h), rows in
+// order (globe.mjs:1417-1424 with coef_j = (wRow_j / nx) * P_n(lat_j)), as one
+// sample of the projection series -- when the step ran.
+__global__ __launch_bounds__(DG_THREADS) void k_proj_final(DiagArgs a, const double *coef, Ctl *ctl,
+                                                          double *vals, double *times, long long cap) {
+  __shared__ double sr[FC], sc[FC];
+  const int t = threadIdx.x;
+  const bool ran = ctl->dt_last > 0;
+  if (!ran) return;
+  double acc = 0.0;
+  for (int j0 = 0; j0 < a.ny; j0 += FC) {
+    const int n = min(FC, a.ny - j0);
+    const int n8 = (n + 7) & ~7;
+    for (int q = t; q < n8; q += DG_THREADS) {
+      const bool in = q < n;
+      sr[q] = in ? a.row_v[j0 + q] : 0.0;
+      sc[q] = in ? coef[j0 + q] : 0.0;
+    }
+    __syncthreads();
+    if (t == 0) {
+      for (int c = 0; c < n8; c += 8) {
+        double x[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) x[q] = sc[c + q] * sr[c + q];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (c + q < n) acc += x[q];
+      }
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    const long long k = ctl->proj_count;
+    if (k < cap) { vals[k] = acc; times[k] = ctl->t; }
+    ctl->proj_count = k + 1;
+  }
+}
+
 }  // namespace
+
+// The projection recorder's kernels for one step, on the state the step left
+// (library-internal; enqueued after the step's finalize, graph-capturable).
+hipError_t launch_projection(swe_ctx *c, const double *h, const double *b) {
+  DiagArgs a{};
+  a.h = h; a.b = b;
+  a.row_v = c->diag; a.row_e = c->diag + c->P.ny;
+  a.nx = c->P.nx; a.ny = c->P.ny; a.W = c->P.W; a.H = c->P.H; a.ng = 2;
+  const int nblocks = (c->P.ny + DG_ROWS - 1) / DG_ROWS;
+  hipLaunchKernelGGL(k_diag_rows<true>, dim3(nblocks), dim3(DG_THREADS), 0, c->stream, a);
+  hipLaunchKernelGGL(k_proj_final, dim3(1), dim3(DG_THREADS), 0, c->stream, a, c->proj_coef, c->dctl,
+                     c->proj_vals, c->proj_times, c->proj_cap);
+  return hipGetLastError();
+}
+
+// the diagnostics' scratch (row sums, block partials, accumulators, results)
+int ensure_diag(swe_ctx *c) {
+  if (c->diag) return SWE_OK;
+  const int ny = c->P.ny, nblocks = (ny + DG_ROWS - 1) / DG_ROWS;
+  const size_t nd = 2 * (size_t)ny + 2 * (size_t)nblocks + 3 + DG_OUT;
+  hipError_t e = hipMalloc(&c->diag, nd * sizeof(double));
+  if (e != hipSuccess) {
+    c->diag = nullptr;
+    return ctx_fail(c, e == hipErrorOutOfMemory ? SWE_ENOMEM : SWE_EHIP, std::string("diagnostics: ") + hipGetErrorString(e));
+  }
+  HIPCHK(c, hipMemsetAsync(c->diag, 0, nd * sizeof(double), c->stream));
+  c->diag_blocks = nblocks;
+  return SWE_OK;
+}
 }  // namespace swe
 
 using namespace swe;
@@ -281,16 +354,8 @@ extern "C" int swe_diagnostics(swe_ctx *c, double eta_ref, double out[SWE_D_COUN
   const int nx = c->P.nx, ny = c->P.ny, ng = 2;
   const int nblocks = (ny + DG_ROWS - 1) / DG_ROWS;
   // layout: row_v[ny], row_e[ny], blk[2 nblocks], acc[3], out[DG_OUT]
-  const size_t nd = 2 * (size_t)ny + 2 * (size_t)nblocks + 3 + DG_OUT;
-  if (!c->diag) {
-    hipError_t e = hipMalloc(&c->diag, nd * sizeof(double));
-    if (e != hipSuccess) {
-      c->diag = nullptr;
-      return ctx_fail(c, e == hipErrorOutOfMemory ? SWE_ENOMEM : SWE_EHIP, std::string("diagnostics: ") + hipGetErrorString(e));
-    }
-    HIPCHK(c, hipMemsetAsync(c->diag, 0, nd * sizeof(double), c->stream));
-    c->diag_blocks = nblocks;
-  }
+  int rc = ensure_diag(c);
+  if (rc) return rc;
   DiagArgs a{};
   a.h = c->st[c->A][0]; a.hu = c->st[c->A][1]; a.hv = c->st[c->A][2];
   a.b = c->b; a.area = c->rows[R_AREA];
@@ -305,7 +370,7 @@ extern "C" int swe_diagnostics(swe_ctx *c, double eta_ref, double out[SWE_D_COUN
   a.dx = c->hrows[SWE_ROW_DX].empty() ? 0.0 : c->hrows[SWE_ROW_DX][ng];
   a.dy = c->hrows[SWE_ROW_DYN].empty() ? 0.0 : c->hrows[SWE_ROW_DYN][ng];
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-  hipLaunchKernelGGL(k_diag_rows, dim3(nblocks), dim3(DG_THREADS), 0, c->stream, a);
+  hipLaunchKernelGGL(k_diag_rows<false>, dim3(nblocks), dim3(DG_THREADS), 0, c->stream, a);
   HIPCHK(c, hipGetLastError());
   hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(DG_THREADS), 0, c->stream, a);
   HIPCHK(c, hipGetLastError());

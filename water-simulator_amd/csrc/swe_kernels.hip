@@ -514,7 +514,7 @@ __global__ void k_apply_bc(Params P, StateW S, double *b, const Ctl *ctl) {
   if (t >= 4 * P.W + 4 * (P.H - 4)) return;
   // the steps a sampled run launches past its end are not steps of the run:
   // they leave even the ghost ring alone
-  if (ctl->dt_mode == DT_SAMPLED && ctl->idle) return;
+  if ((ctl->dt_mode == DT_SAMPLED || ctl->dt_mode == DT_UNTIL) && ctl->idle) return;
   apply_bc_cell(P, S, b, ctl->ext, t);
 }
 
@@ -661,7 +661,7 @@ __device__ __forceinline__ void finalize_ctl(Ctl *ctl, double s1, double s2, dou
     // tools/waves.mjs:222-223: while (t < T) step(min(maxDt, max(1e-6, min(T, next) - t)));
     // at t >= T the step is refused (dt 0), which leaves the state as it is
     ctl->dt = ctl->t < ctl->t_end ? jmin(dtn, jmax(1e-6, jmin(ctl->t_end, ctl->rec_next) - ctl->t)) : 0.0;
-  ctl->idle = ctl->dt_mode == DT_SAMPLED && !(ctl->t < ctl->t_end);
+  ctl->idle = (ctl->dt_mode == DT_SAMPLED || ctl->dt_mode == DT_UNTIL) && !(ctl->t < ctl->t_end);
   tide_times(ctl);
   if (ctl->harm_on) {
     for (int side = 0; side < 4; ++side) {

@@ -44,7 +44,7 @@ struct Ctl {
   unsigned long long cfl_bits;   // max-reduction accumulator (bit pattern of a non-negative double)
   long long steps;
   int dt_mode, nonfinite, harm_on;
-  int idle;            // DT_SAMPLED run past t_end: the remaining launched steps do nothing
+  int idle;            // DT_SAMPLED / DT_UNTIL run at t_end: the remaining launched steps do nothing
   double dt_last;      // dt the last step ran with (0 if it was refused)
   Harm harm[4];
   // equilibrium-tide constituents: angular frequency and phase, and
@@ -60,6 +60,7 @@ struct Ctl {
   int rec_every, rec_flag;
   long long rec_count, rec_row, rec_since;
   double rec_next, rec_t0, rec_dt;
+  long long proj_count;   // samples of the projection recorder so far
 };
 
 // Gauge recorder arguments of k_finalize (cells == nullptr: no recorder).

@@ -83,6 +83,8 @@ def lib():
             "swe_diagnostics": (i, [vp, d, dp]),
             "swe_set_recorder": (i, [vp, i, ctypes.POINTER(ctypes.c_int32), i, d, d, i64]),
             "swe_get_record": (i, [vp, dp, dp, ctypes.POINTER(i64)]),
+            "swe_set_projection": (i, [vp, dp, i64]),
+            "swe_get_projection": (i, [vp, dp, dp, ctypes.POINTER(i64)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -279,6 +281,26 @@ class Solver:
         self._rec_n = c.size
         self._ck(lib().swe_set_recorder(self._h, c.size, c.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                         int(every), float(t0), float(dt_rec), int(capacity)), "swe_set_recorder")
+
+    def set_projection(self, coef, capacity):
+        """Arm the projection recorder of advance(): after every step that runs,
+        sum_j coef[j] * (row sum of b + h) in the reference's order (the mode
+        projection of globe.mjs:1417-1424).  coef=None disarms."""
+        if coef is None:
+            self._ck(lib().swe_set_projection(self._h, None, 0), "swe_set_projection")
+            return
+        c = np.ascontiguousarray(coef, dtype=np.float64)
+        if c.size != self.ny:
+            raise ValueError(f"projection: {c.size} coefficients for {self.ny} rows")
+        self._ck(lib().swe_set_projection(self._h, _dp(c), int(capacity)), "swe_set_projection")
+
+    def projection_series(self):
+        """(times, values) of the projection recorder so far."""
+        n = ctypes.c_int64()
+        self._ck(lib().swe_get_projection(self._h, None, None, ctypes.byref(n)), "swe_get_projection")
+        vals, times = np.zeros(n.value), np.zeros(n.value)
+        self._ck(lib().swe_get_projection(self._h, _dp(vals), _dp(times), ctypes.byref(n)), "swe_get_projection")
+        return times, vals
 
     def record(self):
         """(times, eta): the samples so far, eta of shape (samples, gauges)."""

@@ -393,6 +393,8 @@ def main():
     ap.add_argument("--inversion-iters", type=int, default=0, help="L-BFGS iterations of the config 4 inversion")
     ap.add_argument("--sphere-steps", type=int, default=100, help="0 skips the spherical ensemble")
     ap.add_argument("--sphere-adjoint-steps", type=int, default=100)
+    ap.add_argument("--globe-deg", type=float, default=1.0,
+                    help="grid spacing of the n = 2 mode-period measurement (globe harness); 0 skips it")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -469,6 +471,15 @@ def main():
         out["inversion"] = inversion_measure(args.inversion_iters)
     if args.sphere_steps > 0:
         out["sphere"] = sphere_measure(args.sphere_steps, args.sphere_adjoint_steps)
+    if args.globe_deg > 0:
+        import globe
+        log(f"[bench] globe: n = 2 mode period at {args.globe_deg} deg")
+        r = globe.measure_mode_period(n=2, deg=args.globe_deg, g_ref=9.80665)
+        out["globe"] = {"workload": f"mode_period_n2_{args.globe_deg:g}deg_4000m_aquaplanet",
+                        "nx": r["nx"], "ny": r["ny"], "steps": r["steps"], "seconds": r["gpu_ms"] / 1e3,
+                        "cell_updates_per_s": r["cell_updates"] / (r["gpu_ms"] * 1e-3),
+                        "measured_h": r["measured"] / 3600, "closed_h": r["closed"] / 3600, "rel": r["rel"],
+                        "polar_tax": r["polar_tax"]["ratio"]}
     if not args.no_cpu:
         log("[bench] cpu baseline")
         out["cpu_baseline"] = cpu_baseline()

@@ -3,7 +3,7 @@ time_reference.mjs: writes the workload's bed and initial surface (the same
 seeded recipes the benchmark uses, at the stated size) and a spec, and runs
 Node on them.  Build-container only: the reference does not travel.
 
-    python oracle/time_reference.py [--workload large|inversion|sphere|gridlands] [--n 512] [--steps 5] [--reps 3]
+    python oracle/time_reference.py [--workload large|inversion|sphere|gridlands|mode2] [--n 512] [--steps 5] [--reps 3]
 
 large: the 4096x4096 recipe at n x n (dx = 10 m, west Flather M2 tide of 1 m,
 Manning 0.025, f = 1e-4); inversion: the 1024x1024 shelf at its full size
@@ -11,7 +11,9 @@ Manning 0.025, f = 1e-4); inversion: the 1024x1024 shelf at its full size
 the 2160x1080 global ocean at its full size (periodic in longitude, walls at
 the poles, its Manning n; the reference has no equilibrium-tide forcing, so the
 step is timed without it); gridlands: the settled synthetic gridlands-terrain
-export of the import tests (120 x 90 at 50 m, closed box, no friction).
+export of the import tests (120 x 90 at 50 m, closed box, no friction); mode2:
+the n = 2 zonal mode of globe.mjs's measureModePeriod at --n degrees (4000 m
+aquaplanet, no rotation).
 """
 import argparse
 import json
@@ -56,13 +58,20 @@ def workload(name, n):
         spec = dict(nx=t["W"], ny=t["H"], dx=t["cellMeters"], sphere=None, manning=0.0, coriolis=0.0, cfl=0.45,
                     bc=["reflect"] * 4, flather=None)
         return b, b + h, spec
+    if name == "mode2":
+        import globe
+        m = globe.mode_problem(n=2, deg=n)
+        spec = dict(nx=m["nx"], ny=m["ny"], dx=None, sphere=dict(R=6371e3, lat0=-90.0, lat1=90.0, omega=0.0),
+                    manning=0.0, coriolis=0.0, cfl=0.45, bc=["periodic", "periodic", "reflect", "reflect"],
+                    flather=None)
+        return m["b"], m["b"] + m["h"], spec
     raise SystemExit(f"unknown workload {name}")
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="large", choices=["large", "inversion", "sphere", "gridlands"])
-    ap.add_argument("--n", type=int, default=512, help="grid size of the large workload")
+    ap.add_argument("--workload", default="large", choices=["large", "inversion", "sphere", "gridlands", "mode2"])
+    ap.add_argument("--n", type=int, default=512, help="grid size of the large workload; degrees for mode2")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--reps", type=int, default=3)
     a = ap.parse_args()

@@ -83,7 +83,7 @@ static int default_chunk(const Params &P) {
   if (chunks < 1) chunks = 1;
   int rows = (P.H + chunks - 1) / chunks;
   if (rows > P.H) rows = P.H;
-  if (rows < MIN_CHUNK) rows = MIN_CHUNK;
+  if (rows < DEFAULT_MIN_CHUNK) rows = DEFAULT_MIN_CHUNK;
   return rows;
 }
 
@@ -568,7 +568,7 @@ int swe_use_graph(swe_ctx *c, int on) {
 int swe_set_chunk_rows(swe_ctx *c, int rows) {
   if (!c) return fail(nullptr, SWE_EINVAL, "null context");
   if (rows == 0) rows = default_chunk(c->P);
-  if (rows < MIN_CHUNK) return fail(c, SWE_EINVAL, "chunk rows below minimum (8)");
+  if (rows < MIN_CHUNK) return fail(c, SWE_EINVAL, "chunk rows must be >= 1");
   if (rows != c->chunk) drop_graphs(c);
   c->chunk = rows;
   return SWE_OK;
@@ -625,6 +625,11 @@ int swe_autotune(swe_ctx *c, int reps, int *best_rows) {
   const int slots = cus * (per_cu > 0 ? per_cu : 4);
   const int strips = (c->P.W + SWEEP_OUT - 1) / SWEEP_OUT;
   std::vector<int> cand = {8, 12, 16, 24, 32, 40, 48, 64, 96};
+  // a grid whose 8-row chunks do not fill the resident slots is latency-bound:
+  // each wavefront marches its chunk (plus 4 halo rows) one row after another,
+  // so shorter chunks finish sooner
+  if (sweep_waves(c->P, DEFAULT_MIN_CHUNK) < slots)
+    for (int rows : {1, 2, 3, 4, 6}) cand.push_back(rows);
   for (int rounds = 1; rounds <= 4; ++rounds) {
     const int chunks = (rounds * slots + strips - 1) / strips;
     if (chunks >= 1) cand.push_back((c->P.H + chunks - 1) / chunks);

@@ -26,7 +26,11 @@ hipError_t launch_tide_times(Ctl *ctl, hipStream_t st);
 int tide_table(const Params &P, int n, const int32_t *m, const double *amp, const double *lat, const double *lon,
                std::vector<double> &out);
 
-constexpr int MIN_CHUNK = 8;
+// rows per wavefront chunk of the stage kernels: any height from 1 gives the
+// same results; the default never goes below DEFAULT_MIN_CHUNK, the autotuner
+// tries shorter chunks on grids too small to fill the device
+constexpr int MIN_CHUNK = 1;
+constexpr int DEFAULT_MIN_CHUNK = 8;
 // device row arrays of a context: the divisors, their correctly rounded
 // reciprocals, and the coefficients
 enum { R_DX, R_RDX, R_DYN, R_RDYN, R_DYS, R_RDYS, R_DYCFL, R_RDYCFL, R_BED, R_RBED, R_TAN, R_F, R_GEO,

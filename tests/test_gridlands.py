@@ -60,6 +60,7 @@ def test_refusals(doc):
     bad = []
     for mutate in (lambda d: d.update(format="other"), lambda d: d.update(version=2),
                    lambda d: d["layers"]["bedM"].update(data=d["layers"]["bedM"]["data"][:1000]),
+                   lambda d: d["layers"]["bedM"].update(data=d["layers"]["bedM"]["data"][:1001]),
                    lambda d: d["layers"]["surfaceM"].update(encoding="b64f64le"),
                    lambda d: d["layers"].pop("bedM"),
                    lambda d: d["grid"].update(width=0),
@@ -69,7 +70,7 @@ def test_refusals(doc):
         with pytest.raises(gl.GridlandsError):
             gl.gridlands_terrain(d)
         bad.append(d)
-    assert len(bad) == 7
+    assert len(bad) == 8
     d = copy.deepcopy(doc)
     d["layers"].pop("manningN")
     t = gl.gridlands_terrain(d)

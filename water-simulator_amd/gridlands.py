@@ -11,6 +11,7 @@ included) exactly as the reference's constructor samples its closures
     n = gridlands.manning_field(t)              # W*H, for Solver.set_manning_field
 """
 import base64
+import binascii
 import json
 import math
 import sys
@@ -28,7 +29,10 @@ def _layer(layer, n, name):
         raise GridlandsError(f"gridlands: layer {name} is missing")
     if layer.get("encoding") != "b64f32le":
         raise GridlandsError(f"gridlands: layer {name} is encoded as {layer.get('encoding')}; only b64f32le is read")
-    raw = base64.b64decode(layer["data"])
+    try:
+        raw = base64.b64decode(layer["data"], validate=True)
+    except (binascii.Error, ValueError, TypeError) as e:
+        raise GridlandsError(f"gridlands: layer {name} is not valid base64 ({e})") from None
     if len(raw) != 4 * n:
         raise GridlandsError(f"gridlands: layer {name} holds {len(raw)} bytes; the grid needs {4 * n} ({n} cells x 4)")
     return np.frombuffer(raw, dtype="<f4").copy()

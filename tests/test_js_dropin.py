@@ -64,3 +64,10 @@ def test_js_device_diagnostics_and_recorded_runs_on_gpu():
 def test_js_gridlands_import_gates_on_gpu():
     """tools/verify-gridlands.mjs's gates through js/gridlands.mjs and the drop-in."""
     assert run_node("tests/js/gridlands_gpu.mjs") == 0
+
+
+@needs_node
+@pytest.mark.gpu
+def test_js_globe_harness_on_gpu():
+    """src/globe.mjs's measureModePeriod / polarTimestepTax checks through js/globe.mjs."""
+    assert run_node("tests/js/globe_gpu.mjs") == 0

@@ -311,6 +311,33 @@ export class ShallowWater {
   }
 
   /**
+   * Run device-resident until `until` (steps min(maxDt, until - t)), recording
+   * after every step s = sum_j coef[j] * (sum over row j of b + h), both sums in
+   * the reference's order -- the mode projection of src/globe.mjs:1417-1424 --
+   * with no counterpart in swe.mjs.  coef: ny values.  Returns { times, vals }.
+   */
+  projectedRun(coef, until, capacity) {
+    this._boundaries();
+    this._syncParams();
+    if (this.forcing) throw new Error('projectedRun runs without a host forcing hook');
+    addon.setScalar(this._dev, S.T, this.t);
+    this._upload(['B', 'H', 'HU', 'HV']);
+    addon.setProjection(this._dev, Float64Array.from(coef), capacity);
+    try {
+      const taken = addon.advance(this._dev, Number.MAX_SAFE_INTEGER, 2, 0, until);
+      const n = addon.getProjection(this._dev, null, null);
+      const vals = new Float64Array(n), times = new Float64Array(n);
+      addon.getProjection(this._dev, vals, times);
+      this._download(['B', 'H', 'HU', 'HV']);
+      this.t = addon.getScalar(this._dev, S.T);
+      this.steps += taken;
+      return { times, vals };
+    } finally {
+      addon.setProjection(this._dev, null, 0);
+    }
+  }
+
+  /**
    * volume, energy, maxSurfaceDeviation(ref), maxSpeed and finite of the device
    * state, computed on the device (no counterpart in the reference, whose
    * methods below loop over the arrays).  The device state is the arrays'

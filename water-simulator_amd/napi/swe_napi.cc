@@ -328,6 +328,40 @@ napi_value GetRecord(napi_env env, napi_callback_info info) {
   return num(env, (double)count);
 }
 
+// setProjection(h, Float64Array coef|null, capacity)
+napi_value SetProjection(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  swe_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  double *coef = nullptr;
+  size_t n = 0;
+  if (!get_f64(env, argv[1], &coef, &n)) {
+    napi_throw_type_error(env, nullptr, "setProjection: Float64Array coef or null");
+    return nullptr;
+  }
+  if (swe_set_projection(c, coef, (int64_t)get_double(env, argv[2])) != SWE_OK)
+    return throw_swe(env, c, "swe_set_projection");
+  return nullptr;
+}
+
+// getProjection(h, Float64Array vals|null, Float64Array times|null) -> samples recorded
+napi_value GetProjection(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  swe_ctx *c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  double *vals = nullptr, *times = nullptr;
+  size_t n1 = 0, n2 = 0;
+  if (!get_f64(env, argv[1], &vals, &n1) || !get_f64(env, argv[2], &times, &n2)) {
+    napi_throw_type_error(env, nullptr, "getProjection: Float64Array or null");
+    return nullptr;
+  }
+  int64_t count = 0;
+  if (swe_get_projection(c, vals, times, &count) != SWE_OK) return throw_swe(env, c, "swe_get_projection");
+  return num(env, (double)count);
+}
+
 // hllc(hL, huL, hvL, hR, huR, hvR, dry, g) -> [m, n, t]
 napi_value Hllc(napi_env env, napi_callback_info info) {
   napi_value argv[8];
@@ -545,7 +579,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"adjCreate", AdjCreate}, {"adjSetGauges", AdjSetGauges}, {"adjForward", AdjForward}, {"adjLoss", AdjLoss},
       {"adjBackward", AdjBackward}, {"adjSeries", AdjSeries}, {"adjTiming", AdjTiming},
       {"allocPinned", AllocPinned}, {"diagnostics", Diagnostics}, {"setRecorder", SetRecorder},
-      {"getRecord", GetRecord},
+      {"getRecord", GetRecord}, {"setProjection", SetProjection}, {"getProjection", GetProjection},
   };
   for (auto &f : fns) {
     napi_value fn;

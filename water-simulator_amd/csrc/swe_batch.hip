@@ -47,6 +47,7 @@ struct Basin {
   size_t N = 0;
   size_t off_field = 0;    // into the field pool: b, st[3][3], manning, sponge
   size_t off_rows = 0;     // into the row pool: R_COUNT rows of H
+  size_t off_rowsA = 0;    // into the row pool: the rows interleaved, GEO_AOS per row (Params::geoA)
   size_t off_part = 0;     // into the partial pool: 2 * max_parts
   int max_parts = 0;
   double manning = 0.0, coriolis = 0.0;
@@ -135,6 +136,7 @@ static int refresh(swe_batch *b) {
     G.dyS = rowp(b, B, R_DYS); G.rdyS = rowp(b, B, R_RDYS); G.dyCFL = rowp(b, B, R_DYCFL);
     G.rdyCFL = rowp(b, B, R_RDYCFL); G.bedPhi = rowp(b, B, R_BED); G.rbedPhi = rowp(b, B, R_RBED);
     G.tanPhi = rowp(b, B, R_TAN); G.fRow = rowp(b, B, R_F); G.geo = rowp(b, B, R_GEO);
+    P.geoA = b->rows + B.off_rowsA;
     const int np = strips_of(P) * chunks_of(P, b->chunk);
     if (np > B.max_parts) return bfail(b, SWE_EINVAL, "chunk too small for a basin");
     BasinDev &D = dev[i];
@@ -237,6 +239,7 @@ int swe_batch_create(const swe_desc *descs, int nb, swe_batch **out) {
     P.rstride = (int)up(P.H);
     B.off_field = fpool; fpool += FS_COUNT * up(B.N);
     B.off_rows = rpool; rpool += R_COUNT * up(P.H);
+    B.off_rowsA = rpool; rpool += GEO_AOS * up(P.H);
     B.max_parts = strips_of(P) * chunks_of(P, MIN_CHUNK);
     B.off_part = ppool; ppool += up(2 * (size_t)B.max_parts);
     b->max_ghost = std::max(b->max_ghost, 4 * P.W + 4 * (P.H - 4));
@@ -278,7 +281,11 @@ int swe_batch_create(const swe_desc *descs, int nb, swe_batch **out) {
       const double *src = descs[i].rows[src_of[r]];
       if (!src) { swe_batch_destroy(b); return bfail(nullptr, SWE_EINVAL, "missing geometry row"); }
       double *dst = host.data() + B.off_rows + (size_t)r * up(B.P.H);
-      for (int j = 0; j < B.P.H; ++j) dst[j] = recip[r] ? 1.0 / src[j] : src[j];
+      double *dstA = host.data() + B.off_rowsA + r;
+      for (int j = 0; j < B.P.H; ++j) {
+        dst[j] = recip[r] ? 1.0 / src[j] : src[j];
+        dstA[(size_t)j * GEO_AOS] = dst[j];
+      }
     }
   }
   if ((e = hipMemcpyAsync(b->rows, host.data(), sizeof(double) * rpool, hipMemcpyHostToDevice, b->stream)) != hipSuccess)

@@ -34,6 +34,7 @@ static void refresh_params(swe_ctx *c) {
   G.dyS = c->rows[R_DYS]; G.rdyS = c->rows[R_RDYS]; G.dyCFL = c->rows[R_DYCFL]; G.rdyCFL = c->rows[R_RDYCFL];
   G.bedPhi = c->rows[R_BED]; G.rbedPhi = c->rows[R_RBED]; G.tanPhi = c->rows[R_TAN]; G.fRow = c->rows[R_F];
   G.geo = c->rows[R_GEO];
+  P.geoA = c->rowsA;
 }
 
 static void drop_graphs(swe_ctx *c) {
@@ -67,6 +68,20 @@ static int upload_row(swe_ctx *c, int row, const double *host) {
     default: return fail(c, SWE_EINVAL, "bad row id");
   }
   if (rc) return rc;
+  // the interleaved copy the stage kernels read: every array's value at row j
+  // (reciprocals of the divisors as above; rows not set yet hold zeros)
+  static const int src_of[R_COUNT] = {SWE_ROW_DX, SWE_ROW_DX, SWE_ROW_DYN, SWE_ROW_DYN, SWE_ROW_DYS,
+                                      SWE_ROW_DYS, SWE_ROW_DYCFL, SWE_ROW_DYCFL, SWE_ROW_BEDPHI, SWE_ROW_BEDPHI,
+                                      SWE_ROW_TANPHI, SWE_ROW_F, SWE_ROW_GEO, SWE_ROW_AREA};
+  static const bool recip[R_COUNT] = {false, true, false, true, false, true, false, true, false, true,
+                                      false, false, false, false};
+  std::vector<double> a((size_t)H * GEO_AOS, 0.0);
+  for (int k = 0; k < R_COUNT; ++k) {
+    const std::vector<double> &src = c->hrows[src_of[k]];
+    if (src.empty()) continue;
+    for (int j = 0; j < H; ++j) a[(size_t)j * GEO_AOS + k] = recip[k] ? 1.0 / src[j] : src[j];
+  }
+  HIPCHK(c, hipMemcpyAsync(c->rowsA, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return SWE_OK;
 }
@@ -165,6 +180,7 @@ int swe_create(const swe_desc *d, swe_ctx **out) {
   }
   if ((e = hipMalloc(&c->rows[0], sizeof(double) * P.H * R_COUNT)) != hipSuccess) return bail("hipMalloc rows", e);
   for (int r = 1; r < R_COUNT; ++r) c->rows[r] = c->rows[0] + (size_t)r * P.H;
+  if ((e = hipMalloc(&c->rowsA, sizeof(double) * P.H * GEO_AOS)) != hipSuccess) return bail("hipMalloc rows", e);
   P.fstride = (int)c->N;
   P.rstride = P.H;
   if ((e = hipMalloc(&c->dctl, sizeof(Ctl))) != hipSuccess) return bail("hipMalloc ctl", e);
@@ -201,6 +217,7 @@ void swe_destroy(swe_ctx *c) {
   (void)hipFree(c->b);
   for (auto &s : c->st) (void)hipFree(s[0]);
   (void)hipFree(c->rows[0]);
+  (void)hipFree(c->rowsA);
   (void)hipFree(c->mfield); (void)hipFree(c->sponge); (void)hipFree(c->tide);
   (void)hipFree(c->rh); (void)hipFree(c->theta);
   (void)hipFree(c->dctl); (void)hipFree(c->mf_part);

@@ -43,6 +43,7 @@ struct swe_ctx {
   swe::Params P{};
   size_t N = 0;
   double *rows[swe::R_COUNT] = {};
+  double *rowsA = nullptr;      // the rows interleaved per row (Params::geoA)
   std::vector<double> hrows[SWE_NROWS];
   double *b = nullptr;
   double *st[3][3] = {};       // [buffer][h,hu,hv]

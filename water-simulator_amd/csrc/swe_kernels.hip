@@ -193,10 +193,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
   const double md = P.min_depth;
   const bool o2 = P.order >= 2;
   const bool bedOn = o2 || P.sphere;
-  // the per-row arrays, rstride apart from dx (Params' layout contract)
-  const double *Rw = P.geo.dx;
-  const int rs = P.rstride;
-  auto rowv = [&](int which, int j) -> double { return Rw[which * rs + j]; };
+  // the per-row values, interleaved per row (Params::geoA)
+  const double *RA = P.geoA;
+  auto rowv = [&](int which, int j) -> double { return RA[j * GEO_AOS + which]; };
   enum { DX, RDX, DYN, RDYN, DYS, RDYS, DYCFL, RDYCFL, BED, RBED, TAN, FROW, GEOC };
 
   auto load_row = [&](int r) -> Row {

@@ -94,7 +94,13 @@ struct Params {
   // geo.dx.  The kernels then carry one base pointer per buffer instead of
   // three, and one for all the rows.
   int fstride, rstride;
+  // The same 13 per-row values interleaved for the stage kernels:
+  // geoA[j * GEO_AOS + k] is array k (the order above) at row j, so a row's
+  // values are one scalar base plus immediate offsets (no per-array offsets
+  // held in scalar registers across the row loop).
+  const double *geoA;
 };
+constexpr int GEO_AOS = 16;
 
 struct StateP { const double *h, *hu, *hv; };
 struct StateW { double *h, *hu, *hv; };

@@ -393,6 +393,7 @@ def main():
     ap.add_argument("--inversion-iters", type=int, default=0, help="L-BFGS iterations of the config 4 inversion")
     ap.add_argument("--sphere-steps", type=int, default=100, help="0 skips the spherical ensemble")
     ap.add_argument("--sphere-adjoint-steps", type=int, default=100)
+    ap.add_argument("--no-diagnostics", action="store_true", help="skip the device diagnostics / recorder measurement")
     ap.add_argument("--globe-deg", type=float, default=1.0,
                     help="grid spacing of the n = 2 mode-period measurement (globe harness); 0 skips it")
     args = ap.parse_args()
@@ -444,7 +445,7 @@ def main():
                 "unit": "wave64 FP64-issue instr/s", "frac": rate / VALU_PEAK_WAVE_INSTR_S,
                 "instructions_per_step": i1 + i2}
     finite = sim.get_scalar(swe_amd.S_NONFINITE) == 0
-    diag = diagnostics_measure(sim, swe_amd, nx, ny)
+    diag = None if args.no_diagnostics else diagnostics_measure(sim, swe_amd, nx, ny)
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": n,
         "warmup": args.warmup, "ms_per_step": 1e3 * el / n, "higher_is_better": True, "scaling": "weak",

@@ -272,6 +272,19 @@ class Solver:
         r["finite"] = bool(r["finite"])
         return r
 
+    # the reference's method names over the device diagnostics
+    def volume(self):
+        return self.diagnostics()["volume"]
+
+    def energy(self):
+        return self.diagnostics()["energy"]
+
+    def max_speed(self):
+        return self.diagnostics()["maxSpeed"]
+
+    def max_surface_deviation(self, ref):
+        return self.diagnostics(ref)["maxSurfaceDeviation"]
+
     def set_recorder(self, cells, every=0, t0=0.0, dt_rec=0.0, capacity=1024):
         """Arm the gauge recorder of advance(): eta = b + h at `cells` after every
         `every`-th step, or (every=0) at the sample times t0, t0 + dt_rec, ...

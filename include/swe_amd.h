@@ -149,6 +149,11 @@ int swe_set_tide(swe_ctx *ctx, int n, const int32_t *m, const double *amp, const
 int swe_set_harmonic(swe_ctx *ctx, int side, int n_eta, double msl, const double *amp,
                      const double *omega, const double *phase, int n_u, double u0,
                      const double *uamp, const double *uomega, const double *uphase);
+/* The wavemaker form of a side's harmonic forcing (src/waves.mjs:163-175,
+ * regularWave): every amplitude faded in by min(1, t / ramp) (ramp > 0; 0 =
+ * none) and, with u_from_eta, u(t) = u_scale (eta(t) - msl) in place of the
+ * u harmonics.  Call after swe_set_harmonic for that side. */
+int swe_set_harmonic_ramp(swe_ctx *ctx, int side, double ramp, int u_from_eta, double u_scale);
 /* capture one device-resident step in a HIP graph and replay it (1 = on) */
 int swe_use_graph(swe_ctx *ctx, int on);
 /* rows of grid per wavefront chunk of the fused stage kernel (autotuned; 0 = default) */

@@ -1,0 +1,103 @@
+"""Numbers the reference records for its own runs, reproduced on the GPU
+through the C ABI by restating the runs that produced them.
+
+* tools/verify.mjs:405-449, the closed-basin seiche (Merian): L = 5 km, 20 m
+  deep, 500 x 1 cells, eta = 0.05 cos(pi x / L), reflecting ends, sampled once
+  a second by sub-stepping min(maxDt, 1 - acc), 8.4 closed-form periods; the
+  reference's comment (:426-427) records 714.0411 s (against Merian's
+  714.0435 s) and 9 downward crossings.
+* tools/verify-tide.mjs:492-570, quarter-wave resonance: a 91 km, 40 m bay
+  behind a 39 km, 400 m approach (200 x 1 cells), Manning 0.010, CFL 0.4,
+  forced at the west by Flather with a 0.5 m cosine faded in over 3 periods
+  and u = sqrt(g / 400) eta, recorded at the head 240 times a period over
+  periods 8-12, at eleven forcing periods; the parabola through the peak
+  gain and its neighbours puts resonance at T / T_res = 1.0220 (:681, "fit
+  1.02200" in its mutation table), and the gains either side of the peak
+  read 2.7820 (0.975) and 2.7866 (1.025) (:548-552).  Run device-resident:
+  the forcing with swe_set_harmonic_ramp, the samples with the recorder on
+  waves.mjs's schedule, which is this loop's.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import swe_amd
+
+pytestmark = pytest.mark.gpu
+G_REF = 9.80665
+
+
+def test_merian_seiche_period_is_the_recorded_714_0411():
+    L, h0, nx = 5000.0, 20.0, 500
+    dx = L / nx
+    rows = swe_amd.geometry_cartesian(nx, 1, dx)
+    s = swe_amd.Solver(nx, 1, rows, manning=0.0, dx=dx, dy=dx)
+    W, H = nx + 4, 5
+    x = (np.arange(W) - 2 + 0.5) * dx
+    eta = np.array([0.05 * math.cos(math.pi * xi / L) for xi in x])
+    b = np.full(W * H, -h0)
+    h = np.maximum(0.0, np.tile(eta, H) - b)
+    z = np.zeros_like(h)
+    s.load_state(b, h, z, z)
+    s.set_bc(("reflect", "reflect", "periodic", "periodic"))
+    want = 2 * L / math.sqrt(G_REF * h0)
+    k = 2 * W + 4                                   # sim.idx(2, 0)
+    hist = []
+    while s.t < 8.4 * want:
+        acc = 0.0
+        while acc < 1.0 - 1e-12:
+            acc += s.step(min(s.max_dt(), 1.0 - acc))
+        hist.append((s.t, float(s.b[k] + s.h[k])))
+    zc = []
+    for n in range(1, len(hist)):
+        if hist[n - 1][1] > 0 and hist[n][1] <= 0:
+            f = hist[n - 1][1] / (hist[n - 1][1] - hist[n][1])
+            zc.append(hist[n - 1][0] + f * (hist[n][0] - hist[n - 1][0]))
+    measured = (zc[-1] - zc[0]) / (len(zc) - 1)
+    print(f"  Merian seiche: {measured!r} s, {len(zc)} crossings (recorded 714.0411 s, 9)", flush=True)
+    assert len(zc) == 9
+    assert abs(measured - 714.0411) < 5e-5
+    assert abs(measured - want) / want < 1e-5
+
+
+def test_resonance_sweep_peaks_at_the_recorded_1_0220():
+    Lbay, hBay, Lapp, hApp = 91000.0, 40.0, 39000.0, 400.0
+    t_res = 4 * Lbay / math.sqrt(G_REF * hBay)
+    L, nx = Lapp + Lbay, 200
+    dx = L / nx
+    rows = swe_amd.geometry_cartesian(nx, 1, dx)
+    W, H = nx + 4, 5
+    x = (np.arange(W) - 2 + 0.5) * dx
+    b = np.tile(np.where(x < Lapp, -hApp, -hBay), H)
+    h = np.maximum(0.0, 0.0 - b)
+    z = np.zeros_like(h)
+    head = 2 * W + (nx - 2 + 2)                     # sim.eta(nx - 2, 0)
+    ratios = [0.55, 0.80, 0.90, 0.95, 0.975, 1.00, 1.025, 1.05, 1.10, 1.25, 1.70]
+    gains = []
+    for r in ratios:
+        period = t_res * r
+        s = swe_amd.Solver(nx, 1, rows, manning=0.010, cfl=0.4, dx=dx, dy=dx)
+        s.load_state(b, h, z, z)
+        s.set_bc(("flather", "reflect", "periodic", "periodic"))
+        s.set_harmonic(0, amp=[0.5], omega=[2 * math.pi / period], phase=[0.0], ramp=3 * period,
+                       u_scale=math.sqrt(G_REF / hApp))
+        s.set_recorder([head], every=0, t0=8 * period, dt_rec=period / 240, capacity=2000)
+        s.autotune(reps=2)
+        s.use_graph(True)
+        s.advance(10_000_000, swe_amd.DT_SAMPLED, t_end=12 * period)
+        _, eta = s.record()
+        s.close()
+        gains.append(0.5 * (eta.max() - eta.min()) / 0.5)
+        print(f"  T/T_res = {r:.3f}: gain {gains[-1]:.5f} ({eta.shape[0]} samples)", flush=True)
+    i = int(np.argmax(gains))
+    (p0, g0), (p1, g1), (p2, g2) = [(ratios[k], gains[k]) for k in (i - 1, i, i + 1)]
+    d1, d2, d3 = (p0 - p1) * (p0 - p2), (p1 - p0) * (p1 - p2), (p2 - p0) * (p2 - p1)
+    A2 = g0 / d1 + g1 / d2 + g2 / d3
+    B2 = -(g0 * (p1 + p2) / d1 + g1 * (p0 + p2) / d2 + g2 * (p0 + p1) / d3)
+    refined = -B2 / (2 * A2)
+    print(f"  fitted resonance T/T_res = {refined:.6f} (recorded 1.02200)", flush=True)
+    assert A2 < 0 and 0 < i < len(ratios) - 1
+    assert abs(refined - 1.02200) < 1e-5
+    assert abs(gains[ratios.index(0.975)] - 2.7820) < 1e-4
+    assert abs(gains[ratios.index(1.025)] - 2.7866) < 1e-4

@@ -575,6 +575,21 @@ int swe_set_harmonic(swe_ctx *c, int side, int n_eta, double msl, const double *
   return SWE_OK;
 }
 
+int swe_set_harmonic_ramp(swe_ctx *c, int side, double ramp, int u_from_eta, double u_scale) {
+  if (!c || side < 0 || side > 3 || !(ramp >= 0) || !std::isfinite(ramp) || !std::isfinite(u_scale))
+    return fail(c, SWE_EINVAL, "bad harmonic ramp");
+  int rc = pull_ctl(c);
+  if (rc) return rc;
+  Harm &hm = c->hctl->harm[side];
+  hm.ramp = ramp;
+  hm.u_from_eta = u_from_eta ? 1 : 0;
+  hm.u_scale = u_scale;
+  rc = push_ctl(c);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SWE_OK;
+}
+
 int swe_use_graph(swe_ctx *c, int on) {
   if (!c) return fail(nullptr, SWE_EINVAL, "null context");
   c->use_graph = on != 0;

@@ -610,10 +610,17 @@ hipError_t launch_tide_times(Ctl *ctl, hipStream_t st) {
   return hipGetLastError();
 }
 
+// c0 + sum_k a_k cos(om_k t - ph_k), a_k = amp_k, or amp_k min(1, t / ramp)
+// when ramp > 0 (src/waves.mjs:166-167: msl + ramp(t) * a * cos(w t))
 __device__ __forceinline__ double harm_eval(int n, double c0, const double *amp, const double *om,
-                                            const double *ph, double t) {
+                                            const double *ph, double t, double ramp) {
   double e = c0;
-  for (int k = 0; k < n; ++k) e += amp[k] * cos(om[k] * t - ph[k]);
+  if (ramp > 0) {
+    const double m = fmin(1.0, t / ramp);
+    for (int k = 0; k < n; ++k) e += (m * amp[k]) * cos(om[k] * t - ph[k]);
+  } else {
+    for (int k = 0; k < n; ++k) e += amp[k] * cos(om[k] * t - ph[k]);
+  }
   return e;
 }
 
@@ -665,8 +672,10 @@ __device__ __forceinline__ void finalize_ctl(Ctl *ctl, double s1, double s2, dou
   if (ctl->harm_on) {
     for (int side = 0; side < 4; ++side) {
       const Harm &h = ctl->harm[side];
-      ctl->ext[2 * side] = harm_eval(h.n_eta, h.msl, h.amp, h.omega, h.phase, ctl->t);
-      ctl->ext[2 * side + 1] = harm_eval(h.n_u, h.u0, h.uamp, h.uomega, h.uphase, ctl->t);
+      const double e = harm_eval(h.n_eta, h.msl, h.amp, h.omega, h.phase, ctl->t, h.ramp);
+      ctl->ext[2 * side] = e;
+      ctl->ext[2 * side + 1] = h.u_from_eta ? h.u_scale * (e - h.msl)
+                                            : harm_eval(h.n_u, h.u0, h.uamp, h.uomega, h.uphase, ctl->t, h.ramp);
     }
   }
 }

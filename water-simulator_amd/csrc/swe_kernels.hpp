@@ -26,11 +26,15 @@ struct Geo {
 
 // Harmonic boundary forcing evaluated on the device for device-resident runs:
 // eta(t) = msl + sum_k amp_k cos(omega_k t - phase_k); u(t) likewise.
+// With ramp > 0 every amplitude is scaled by min(1, t / ramp) (a wavemaker
+// fading in); with u_from_eta, u(t) = u_scale (eta(t) - msl) instead of its own
+// harmonics (the shallow-water characteristic of src/waves.mjs:163-175).
 struct Harm {
-  int n_eta, n_u, pad0, pad1;
+  int n_eta, n_u, u_from_eta, pad1;
   double msl, u0;
   double amp[MAX_HARM], omega[MAX_HARM], phase[MAX_HARM];
   double uamp[MAX_HARM], uomega[MAX_HARM], uphase[MAX_HARM];
+  double ramp, u_scale;
 };
 
 // Device-resident control block of one basin.

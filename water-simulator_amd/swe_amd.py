@@ -72,6 +72,7 @@ def lib():
             "swe_step": (i, [vp, d, dp, i, i, dp, dp]),
             "swe_advance": (i, [vp, i64, i, d, d, ctypes.POINTER(i64)]),
             "swe_set_harmonic": (i, [vp, i, i, d, dp, dp, dp, i, d, dp, dp, dp]),
+            "swe_set_harmonic_ramp": (i, [vp, i, d, i, d]),
             "swe_use_graph": (i, [vp, i]),
             "swe_set_chunk_rows": (i, [vp, i]),
             "swe_sync": (i, [vp]),
@@ -324,7 +325,11 @@ class Solver:
         self._ck(lib().swe_get_record(self._h, _dp(eta), _dp(times), ctypes.byref(n)), "swe_get_record")
         return times, eta
 
-    def set_harmonic(self, side, amp=(), omega=(), phase=(), msl=0.0, uamp=(), uomega=(), uphase=(), u0=0.0):
+    def set_harmonic(self, side, amp=(), omega=(), phase=(), msl=0.0, uamp=(), uomega=(), uphase=(), u0=0.0,
+                     ramp=0.0, u_scale=None):
+        """Flather forcing of swe_advance: eta = msl + sum amp cos(omega t - phase),
+        u likewise; ramp > 0 fades the amplitudes in over `ramp` seconds and
+        u_scale makes u = u_scale (eta - msl) (regularWave, src/waves.mjs:163-175)."""
         def arr(x):
             a = np.zeros(4)
             a[:len(x)] = x
@@ -333,6 +338,9 @@ class Solver:
         self._keep = (A, O, P, UA, UO, UP)
         self._ck(lib().swe_set_harmonic(self._h, side, len(amp), msl, _dp(A), _dp(O), _dp(P), len(uamp), u0,
                                         _dp(UA), _dp(UO), _dp(UP)), "swe_set_harmonic")
+        if ramp > 0 or u_scale is not None:
+            self._ck(lib().swe_set_harmonic_ramp(self._h, side, float(ramp), int(u_scale is not None),
+                                                 float(u_scale or 0.0)), "swe_set_harmonic_ramp")
 
     def set_tide(self, constituents, lat, lon):
         """Equilibrium-tide body force (spherical grids): constituents = [(m, amp, omega, phase), ...],

@@ -16,6 +16,13 @@ through the C ABI by restating the runs that produced them.
   read 2.7820 (0.975) and 2.7866 (1.025) (:548-552).  Run device-resident:
   the forcing with swe_set_harmonic_ramp, the samples with the recorder on
   waves.mjs's schedule, which is this loop's.
+* tools/verify-tide.mjs:1131-1195 and README.md:284-287, a real tide over a
+  drying flat: 30 km, 150 cells, bed -6 + 7.4 (x/L)^1.5, Manning 0.025, CFL
+  0.4, Flather with Tide({M2: [1.6, 0], S2: [0.5, 30]}) and u = sqrt(g/6) eta,
+  sampled every 120 s over three M2 periods (the last one and a half for the
+  gauges); the README records the wetted fraction swinging 86.7% to 100.0%,
+  the range falling from 6.53 m at the mouth to 4.84 m in the inner basin,
+  mean flood 2.83 h against mean ebb 9.47 h, an asymmetry of 107.9%.
 """
 import math
 
@@ -101,3 +108,53 @@ def test_resonance_sweep_peaks_at_the_recorded_1_0220():
     assert abs(refined - 1.02200) < 1e-5
     assert abs(gains[ratios.index(0.975)] - 2.7820) < 1e-4
     assert abs(gains[ratios.index(1.025)] - 2.7866) < 1e-4
+
+
+def test_drying_flat_tide_matches_the_recorded_figures():
+    L, nx = 30000.0, 150
+    dx = L / nx
+    rows = swe_amd.geometry_cartesian(nx, 1, dx)
+    W, H = nx + 4, 5
+    x = (np.arange(W) - 2 + 0.5) * dx
+    # the west ghost columns (x < 0) get NaN, as the reference's constructor gives
+    # them; the boundary fill before the first stage replaces them
+    with np.errstate(invalid="ignore"):
+        brow = np.array([-6 + 7.4 * (xi / L) ** 1.5 for xi in x])
+    b = np.tile(brow, H)
+    h = np.maximum(0.0, 0.0 - b)
+    z = np.zeros_like(h)
+    s = swe_amd.Solver(nx, 1, rows, manning=0.025, cfl=0.4, dx=dx, dy=dx)
+    s.load_state(b, h, z, z)
+    s.set_bc(("flather", "reflect", "periodic", "periodic"))
+    # Tide.eta: msl + sum amp cos(omega t - phase pi / 180), largest amplitude first (tide.mjs:87-107)
+    m2, s2 = 12.4206012 * 3600, 12.0 * 3600
+    s.set_harmonic(0, amp=[1.6, 0.5], omega=[2 * math.pi / m2, 2 * math.pi / s2],
+                   phase=[0 * math.pi / 180, 30 * math.pi / 180], u_scale=math.sqrt(G_REF / 6))
+    cells = [2 * W + (i + 2) for i in range(nx)]
+    T, T0 = 3 * m2, 1.5 * m2
+    s.set_recorder(cells, every=0, t0=0.0, dt_rec=120.0, capacity=2000)
+    s.use_graph(True)
+    s.advance(10_000_000, swe_amd.DT_SAMPLED, t_end=T)
+    times, eta = s.record()
+    assert s.finite()
+    wet = ((eta - brow[2:-2][None, :]) > 1e-3).sum(axis=1) / nx
+    late = times >= T0
+    mouth, inner, ti = eta[late, 4], eta[late, nx - 30], times[late]
+    ex = []
+    for n in range(1, len(inner) - 1):
+        a, c, d = inner[n - 1], inner[n], inner[n + 1]
+        if c > a and c >= d:
+            ex.append((ti[n], "H"))
+        if c < a and c <= d:
+            ex.append((ti[n], "L"))
+    flood = [ex[n + 1][0] - ex[n][0] for n in range(len(ex) - 1) if ex[n][1] == "L" and ex[n + 1][1] == "H"]
+    ebb = [ex[n + 1][0] - ex[n][0] for n in range(len(ex) - 1) if ex[n][1] == "H" and ex[n + 1][1] == "L"]
+    fm, em = sum(flood) / len(flood), sum(ebb) / len(ebb)
+    asym = abs(fm - em) / ((fm + em) / 2)
+    rng_mouth, rng_inner = mouth.max() - mouth.min(), inner.max() - inner.min()
+    print(f"  wetted {100 * wet.min():.1f}% to {100 * wet.max():.1f}%, range {rng_mouth:.2f} m -> {rng_inner:.2f} m, "
+          f"flood {fm / 3600:.2f} h, ebb {em / 3600:.2f} h, asymmetry {100 * asym:.1f}%", flush=True)
+    assert f"{100 * wet.min():.1f}" == "86.7" and f"{100 * wet.max():.1f}" == "100.0"
+    assert f"{rng_mouth:.2f}" == "6.53" and f"{rng_inner:.2f}" == "4.84"
+    assert f"{fm / 3600:.2f}" == "2.83" and f"{em / 3600:.2f}" == "9.47"
+    assert f"{100 * asym:.1f}" == "107.9"

@@ -1,0 +1,40 @@
+// Figures the reference records for its own still-water runs, reproduced through
+// the drop-in on the GPU by restating those runs (tools/verify.mjs:104-127,
+// the reference's own beds and closures): the lake-at-rest table of README.md:50-55
+// (max |eta - 2 m| and max speed after 400 steps on 200 x 4 cells, dx = 5 m, over
+// four beds) and the volume drift 2.0662e-16 that swe.mjs:821-823 pins the
+// Cartesian volume()'s summation order to (the smooth bump's).
+//
+//   node tests/js/reference_numbers_gpu.mjs
+import { ShallowWater, reflect, periodic } from '../../water-simulator_amd/js/swe.mjs';
+
+let fails = 0, checks = 0;
+function report(label, ok, note) {
+  checks++;
+  if (!ok) fails++;
+  console.log(`${ok ? 'PASS' : 'FAIL'}  ${label}${note ? '  -- ' + note : ''}`);
+}
+// tools/verify.mjs:57-61
+const fmt = (v) => (v === 0 ? '0' : (Math.abs(v) < 1e-3 || Math.abs(v) >= 1e5) ? v.toExponential(4) : v.toFixed(6));
+
+const beds = {
+  'smooth bump': [(x) => 0.8 * Math.exp(-(((x - 500) / 120) ** 2)), '4.4409e-16', '2.3915e-14'],
+  'sharp step': [(x) => (x > 480 && x < 620 ? 1.2 : 0.1), '0', '1.5877e-14'],
+  'random rough': [(x) => 0.6 * (Math.sin(x * 0.031) + Math.sin(x * 0.0173) * Math.cos(x * 0.0077)), '8.8818e-16', '3.2266e-14'],
+  'island (pierces surface)': [(x) => 2.6 * Math.exp(-(((x - 500) / 90) ** 2)), '8.8818e-16', '1.8395e-14'],
+};
+for (const [name, [bfn, wantEta, wantSpeed]] of Object.entries(beds)) {
+  const sim = new ShallowWater({ nx: 200, ny: 4, dx: 5, bed: (x, y) => bfn(x) + 0.15 * Math.sin(y * 0.05), eta0: 2.0, manning: 0 });
+  sim.boundaries = { west: reflect, east: reflect, south: periodic, north: periodic };
+  const v0 = sim.volume();
+  for (let n = 0; n < 400; n++) sim.step();
+  const dev = fmt(sim.maxSurfaceDeviation(2.0)), spd = fmt(sim.maxSpeed());
+  report(`lake at rest, ${name}: max |eta - 2 m| is the recorded ${wantEta}`, dev === wantEta, dev);
+  report(`lake at rest, ${name}: max speed is the recorded ${wantSpeed}`, spd === wantSpeed, spd);
+  if (name === 'smooth bump') {
+    const drift = fmt(Math.abs(sim.volume() - v0) / v0);
+    report('smooth bump: volume drift is the 2.0662e-16 swe.mjs pins volume() to', drift === '2.0662e-16', drift);
+  }
+}
+console.log(fails ? `${fails} FAILED of ${checks}` : `ALL PASS ${checks}/${checks}`);
+process.exitCode = fails ? 1 : 0;

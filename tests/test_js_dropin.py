@@ -71,3 +71,10 @@ def test_js_gridlands_import_gates_on_gpu():
 def test_js_globe_harness_on_gpu():
     """src/globe.mjs's measureModePeriod / polarTimestepTax checks through js/globe.mjs."""
     assert run_node("tests/js/globe_gpu.mjs") == 0
+
+
+@needs_node
+@pytest.mark.gpu
+def test_js_reference_recorded_still_water_figures_on_gpu():
+    """README.md:50-55's lake-at-rest table and swe.mjs:821-823's drift, reproduced exactly."""
+    assert run_node("tests/js/reference_numbers_gpu.mjs") == 0

@@ -6,9 +6,9 @@
 // buffers {h,hu,hv}[3]: stage 1 reads A and writes B, stage 2 reads B (stencil)
 // and A (pointwise) and writes C, then A and C swap.  b is shared.  Geometry is
 // 13 per-row arrays of length H (divisors and their correctly rounded
-// reciprocals).  Ghost cells are never read from memory in a step: every load of
-// a ghost cell recomputes the boundary condition from the interior (the
-// reference's applyBC, swe.mjs:289-303, folded into the loads).
+// reciprocals).  Before each stage k_apply_bc fills the ghost ring of the
+// stage's input state (the reference's applyBC, swe.mjs:289-303, in its order),
+// and the stage kernels read the ghosts from memory like any other cell.
 #pragma once
 #include <hip/hip_runtime.h>
 

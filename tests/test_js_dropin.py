@@ -78,3 +78,11 @@ def test_js_globe_harness_on_gpu():
 def test_js_reference_recorded_still_water_figures_on_gpu():
     """README.md:50-55's lake-at-rest table and swe.mjs:821-823's drift, reproduced exactly."""
     assert run_node("tests/js/reference_numbers_gpu.mjs") == 0
+
+
+@needs_node
+@pytest.mark.gpu
+def test_js_verify_sections_to_the_printed_digit_on_gpu():
+    """tools/verify.mjs sections 3-5, 7, 8 (Ritter, celerity, Thacker, self-convergence,
+    dissipation) through the drop-in: every figure equal to the one the reference prints."""
+    assert run_node("tests/js/verify_sections_gpu.mjs") == 0

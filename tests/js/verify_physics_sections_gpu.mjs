@@ -1,0 +1,169 @@
+// Sections 1, 6 and 7 of the reference's tools/verify-physics.mjs (the inertial
+// oscillation under Coriolis, the HLLC contact wave, gravity recovered from a
+// basin's long-wave period), restated over the drop-in solver on the GPU, with
+// every figure those sections print held to the digits the reference prints.
+// The expected strings are the reference's own output under Node 12.22.9 (the
+// engine the GPU host runs this script with).  None of these runs has friction,
+// so Math.pow's engine-dependent last bit never enters; the host-side Math.hypot
+// and Math.atan2 of the measurement run in the same engine as the reference's.
+//
+//   node tests/js/verify_physics_sections_gpu.mjs
+import { ShallowWater, reflect, periodic } from '../../water-simulator_amd/js/swe.mjs';
+
+const G = 9.80665;
+let fails = 0, checks = 0;
+// the reference's number format (tools/verify-physics.mjs:59-64)
+const fmt = (v) => {
+  if (v === 0) return '0';
+  if (!Number.isFinite(v)) return String(v);
+  const a = Math.abs(v);
+  return (a < 1e-3 || a >= 1e5) ? v.toExponential(4) : v.toFixed(6);
+};
+function same(label, got, want) {
+  checks++;
+  const ok = got === want;
+  if (!ok) fails++;
+  console.log(`${ok ? 'PASS' : 'FAIL'}  ${label}: ${got}${ok ? '' : `  (the reference prints ${want})`}`);
+}
+
+// ---- 1. the inertial oscillation (tools/verify-physics.mjs:115-213) -------------------------
+{
+  const f = 1e-4, T = 2 * Math.PI / f, h0 = 10, U = 0.5, N = 2000, dt = T / N, th = f * dt;
+  const sim = new ShallowWater({ nx: 8, ny: 8, dx: 5000, bed: () => -h0, eta0: 0, manning: 0, coriolis: f });
+  sim.boundaries = { west: periodic, east: periodic, south: periodic, north: periodic };
+  for (let j = 0; j < sim.ny; j++) for (let i = 0; i < sim.nx; i++) {
+    const k = sim.idx(i, j);
+    sim.h[k] = h0; sim.hu[k] = h0 * U; sim.hv[k] = h0 * 0;
+  }
+  same('inertial: fixed step, s', fmt(dt), '31.415927');
+  same('inertial: CFL limit, s', fmt(sim.maxDt()), '110.806291');
+  const k0 = sim.idx(4, 4);
+  const u = () => sim.vel(sim.hu[k0], sim.h[k0]), v = () => sim.vel(sim.hv[k0], sim.h[k0]);
+  // the parcel's path by trapezoidal integration of the centre cell's velocity,
+  // and its turning angle by unwrapped atan2 increments
+  let px = 0, py = 0, pu = u(), pv = v(), turn = 0, prev = Math.atan2(pv, pu), quarterV = null;
+  const path = [[0, 0]];
+  const s0 = Math.hypot(pu, pv);
+  for (let n = 0; n < N; n++) {
+    sim.step(dt);
+    const cu = u(), cv = v();
+    px += 0.5 * (pu + cu) * dt; py += 0.5 * (pv + cv) * dt;
+    path.push([px, py]);
+    pu = cu; pv = cv;
+    let d = Math.atan2(cv, cu) - prev;
+    while (d > Math.PI) d -= 2 * Math.PI;
+    while (d < -Math.PI) d += 2 * Math.PI;
+    turn += d; prev = Math.atan2(cv, cu);
+    if (n === N / 4 - 1) quarterV = cv;
+  }
+  const s1 = Math.hypot(u(), v());
+  let dep = 0;
+  const kr = sim.idx(0, 0);
+  for (let j = 0; j < sim.ny; j++) for (let i = 0; i < sim.nx; i++) {
+    const k = sim.idx(i, j);
+    dep = Math.max(dep, Math.abs(sim.h[k] - sim.h[kr]), Math.abs(sim.hu[k] - sim.hu[kr]), Math.abs(sim.hv[k] - sim.hv[kr]));
+  }
+  same('inertial: departure from uniformity', String(dep), '0');
+  same('inertial: measured period, s', fmt(T * (2 * Math.PI) / Math.abs(turn)), '62831.749718');
+  same('inertial: 2 pi / f, s', fmt(T), '62831.853072');
+  same('inertial: SSP-RK2 phase error (f dt)^2/6', fmt(th * th / 6), '1.6449e-6');
+  same('inertial: total turn, revolutions', (turn / (2 * Math.PI)).toFixed(6), '-1.000002');
+  same('inertial: v after a quarter period, m/s', fmt(quarterV), '-0.500000');
+  same('inertial: speed after one revolution, m/s', fmt(s1), '0.500000');
+  same('inertial: speed drift %', fmt(100 * (s1 - s0) / s0), '2.4352e-6');
+  same('inertial: SSP-RK2 growth N (f dt)^4/8 %', fmt(100 * N * th ** 4 / 8), '2.4352e-6');
+  const cx = path.reduce((a, p) => a + p[0], 0) / path.length;
+  const cy = path.reduce((a, p) => a + p[1], 0) / path.length;
+  let rMin = Infinity, rMax = 0;
+  for (const [x, y] of path) {
+    const r = Math.hypot(x - cx, y - cy);
+    if (r < rMin) rMin = r;
+    if (r > rMax) rMax = r;
+  }
+  same('inertial: circle radius, m', fmt(0.5 * (rMin + rMax)), '4999.987693');
+  same('inertial: out-of-round, m', fmt(rMax - rMin), '5.013991');
+}
+
+// ---- 6. the HLLC contact wave (tools/verify-physics.mjs:769-919) ----------------------------
+{
+  const h0 = 4, U0 = -1.2, V = 0.4, L = 240, N = 120, T = L / Math.abs(U0);
+  // one traversal in equal steps just inside the CFL limit
+  const plan = (sim) => { const dtc = sim.maxDt(), ns = Math.ceil(T / (0.95 * dtc)); return { dt: T / ns, ns, dtc }; };
+  {
+    const dx = L / N, ny = 4;
+    const sim = new ShallowWater({ nx: N, ny, dx, bed: () => -h0, eta0: 0, manning: 0 });
+    sim.boundaries = { west: periodic, east: periodic, south: periodic, north: periodic };
+    const wrap = (x) => ((x % L) + L) % L;
+    const vf = (x) => (wrap(x) > L / 4 && wrap(x) < 3 * L / 4 ? V : -V);
+    for (let j = 0; j < ny; j++) for (let i = 0; i < N; i++) {
+      const k = sim.idx(i, j);
+      sim.h[k] = h0; sim.hu[k] = h0 * U0; sim.hv[k] = h0 * vf((i + 0.5) * dx);
+    }
+    const { dt, ns, dtc } = plan(sim);
+    for (let s = 0; s < ns; s++) sim.step(dt);
+    let smeared = 0, over = 0, l1 = 0, drift = 0;
+    for (let i = 0; i < N; i++) {
+      const k = sim.idx(i, 2), vv = sim.vel(sim.hv[k], sim.h[k]);
+      if (Math.abs(vv) < 0.8 * V) smeared++;
+      over = Math.max(over, Math.abs(vv) - V);
+      l1 += Math.abs(vv - vf((i + 0.5) * dx - U0 * sim.t)) / V;
+      drift = Math.max(drift, Math.abs(sim.h[k] - h0), Math.abs(sim.vel(sim.hu[k], sim.h[k]) - U0));
+    }
+    same('contact: steps', String(ns), '3305');
+    same('contact: dt, s', fmt(dt), '0.060514');
+    same('contact: CFL limit, s', fmt(dtc), '0.063711');
+    same('contact: departure of h and u from uniform', String(drift), '0');
+    same('contact: overshoot max |v| - V, m/s', fmt(over), '0');
+    same('contact: cells inside the transitions', String(smeared), '12');
+    same('contact: L1 against the translated profile', fmt(l1 / N), '0.066138');
+  }
+  {
+    const dy = L / N, nx = 4, kw = 4 * Math.PI / L;
+    const sim = new ShallowWater({ nx, ny: N, dx: dy, bed: () => -h0, eta0: 0, manning: 0 });
+    sim.boundaries = { west: periodic, east: periodic, south: periodic, north: periodic };
+    const uf = (y) => V * Math.sin(kw * y);
+    for (let j = 0; j < N; j++) for (let i = 0; i < nx; i++) {
+      const k = sim.idx(i, j);
+      sim.h[k] = h0; sim.hv[k] = h0 * U0; sim.hu[k] = h0 * uf((j + 0.5) * dy);
+    }
+    const { dt, ns } = plan(sim);
+    for (let s = 0; s < ns; s++) sim.step(dt);
+    // the profile's sine and cosine components against the translated phase
+    let a = 0, b = 0;
+    for (let j = 0; j < N; j++) {
+      const k = sim.idx(2, j), uu = sim.vel(sim.hu[k], sim.h[k]), ph = kw * ((j + 0.5) * dy - U0 * sim.t);
+      a += uu * Math.sin(ph); b += uu * Math.cos(ph);
+    }
+    a *= 2 / N; b *= 2 / N;
+    same('contact: transverse amplitude after one traversal', fmt(Math.hypot(a, b) / V), '0.997358');
+    same('contact: transverse phase error, rad', fmt(Math.atan2(b, a)), '0.007697');
+  }
+}
+
+// ---- 7. gravity from the basin's long-wave period (tools/verify-physics.mjs:921-954) ---------
+{
+  const L = 2000, h0 = 10, nx = 200, A = 1e-3, T1 = 2 * L / Math.sqrt(G * h0);
+  const sim = new ShallowWater({ nx, ny: 1, dx: L / nx, bed: () => -h0, manning: 0,
+    eta0: (x) => A * Math.cos(Math.PI * x / L) });
+  sim.boundaries = { west: reflect, east: reflect, south: reflect, north: reflect };
+  const dt = 0.9 * sim.maxDt(), k0 = sim.idx(0, 0);
+  const eta = () => sim.b[k0] + sim.h[k0];
+  let prev = eta(), prevT = sim.t, steps = 0;
+  const cross = [];
+  while (sim.t < 3 * T1) {
+    sim.step(dt);
+    steps++;
+    const e = eta();
+    if ((prev > 0 && e <= 0) || (prev < 0 && e >= 0)) cross.push(prevT + dt * prev / (prev - e));
+    prev = e; prevT = sim.t;
+  }
+  const T = 2 * (cross[cross.length - 1] - cross[0]) / (cross.length - 1);
+  same('gravity: zero crossings at the west wall', String(cross.length), '6');
+  same('gravity: steps', String(steps), '5927');
+  same('gravity: measured period, s', fmt(T), '403.913707');
+  same('gravity: 2L/sqrt(gh), s', fmt(T1), '403.923995');
+  same('gravity: g recovered, m/s^2', fmt((2 * L / T) ** 2 / h0), '9.807150');
+}
+
+console.log(fails ? `${fails} FAILED of ${checks}` : `ALL PASS ${checks}/${checks}`);
+process.exitCode = fails ? 1 : 0;

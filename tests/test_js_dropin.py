@@ -86,3 +86,19 @@ def test_js_verify_sections_to_the_printed_digit_on_gpu():
     """tools/verify.mjs sections 3-5, 7, 8 (Ritter, celerity, Thacker, self-convergence,
     dissipation) through the drop-in: every figure equal to the one the reference prints."""
     assert run_node("tests/js/verify_sections_gpu.mjs") == 0
+
+
+@needs_node
+@pytest.mark.gpu
+def test_js_verify_physics_sections_to_the_printed_digit_on_gpu():
+    """tools/verify-physics.mjs sections 1, 6, 7 (inertial oscillation, HLLC contact wave,
+    gravity from a basin's period) through the drop-in: every printed figure equal."""
+    assert run_node("tests/js/verify_physics_sections_gpu.mjs") == 0
+
+
+@needs_node
+@pytest.mark.gpu
+def test_js_verify_tide_sections_to_the_printed_digit_on_gpu():
+    """tools/verify-tide.mjs sections 3 and 3c (Flather absorption against a wall, and the
+    finite-amplitude table) through the drop-in: every printed figure equal."""
+    assert run_node("tests/js/verify_tide_sections_gpu.mjs") == 0

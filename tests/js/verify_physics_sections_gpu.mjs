@@ -1,14 +1,15 @@
-// Sections 1, 6 and 7 of the reference's tools/verify-physics.mjs (the inertial
-// oscillation under Coriolis, the HLLC contact wave, gravity recovered from a
-// basin's long-wave period), restated over the drop-in solver on the GPU, with
-// every figure those sections print held to the digits the reference prints.
+// Sections 1, 6, 7, 8 and 10 of the reference's tools/verify-physics.mjs (the
+// inertial oscillation under Coriolis, the HLLC contact wave, gravity recovered
+// from a basin's long-wave period, the CFL step, the open boundaries), restated
+// over the drop-in solver on the GPU, with every figure those sections print
+// held to the digits the reference prints.
 // The expected strings are the reference's own output under Node 12.22.9 (the
 // engine the GPU host runs this script with).  None of these runs has friction,
 // so Math.pow's engine-dependent last bit never enters; the host-side Math.hypot
 // and Math.atan2 of the measurement run in the same engine as the reference's.
 //
 //   node tests/js/verify_physics_sections_gpu.mjs
-import { ShallowWater, reflect, periodic } from '../../water-simulator_amd/js/swe.mjs';
+import { ShallowWater, reflect, periodic, outflow, makeSponge } from '../../water-simulator_amd/js/swe.mjs';
 
 const G = 9.80665;
 let fails = 0, checks = 0;
@@ -24,6 +25,23 @@ function same(label, got, want) {
   const ok = got === want;
   if (!ok) fails++;
   console.log(`${ok ? 'PASS' : 'FAIL'}  ${label}: ${got}${ok ? '' : `  (the reference prints ${want})`}`);
+}
+
+// a uniform interior state, and the largest departure from one
+function uniform(sim, h0, u0, v0) {
+  for (let j = 0; j < sim.ny; j++) for (let i = 0; i < sim.nx; i++) {
+    const k = sim.idx(i, j);
+    sim.h[k] = h0; sim.hu[k] = h0 * u0; sim.hv[k] = h0 * v0;
+  }
+}
+function departure(sim) {
+  const kr = sim.idx(0, 0);
+  let m = 0;
+  for (let j = 0; j < sim.ny; j++) for (let i = 0; i < sim.nx; i++) {
+    const k = sim.idx(i, j);
+    m = Math.max(m, Math.abs(sim.h[k] - sim.h[kr]), Math.abs(sim.hu[k] - sim.hu[kr]), Math.abs(sim.hv[k] - sim.hv[kr]));
+  }
+  return m;
 }
 
 // ---- 1. the inertial oscillation (tools/verify-physics.mjs:115-213) -------------------------
@@ -163,6 +181,111 @@ function same(label, got, want) {
   same('gravity: measured period, s', fmt(T), '403.913707');
   same('gravity: 2L/sqrt(gh), s', fmt(T1), '403.923995');
   same('gravity: g recovered, m/s^2', fmt((2 * L / T) ** 2 / h0), '9.807150');
+}
+
+// ---- 8. the CFL step (tools/verify-physics.mjs:1003-1060) -----------------------------------
+{
+  const h0 = 3, u0 = 1.7, v0 = -0.9, c = Math.sqrt(G * h0);
+  const want = { 7: ['0.282552', '0.442166', '1.564902'], 11: ['0.290123', '0.498101', '1.716865'] };
+  for (const [dx, dy] of [[7, 11], [11, 7]]) {
+    const sim = new ShallowWater({ nx: 6, ny: 5, dx, dy, bed: () => -h0, eta0: 0, manning: 0 });
+    uniform(sim, h0, u0, v0);
+    const sum = (Math.abs(u0) + c) / dx + (Math.abs(v0) + c) / dy;
+    const mx = Math.max((Math.abs(u0) + c) / dx, (Math.abs(v0) + c) / dy);
+    same(`maxDt at dx = ${dx}, dy = ${dy}, s`, fmt(sim.maxDt()), want[dx][0]);
+    same(`maxDt at dx = ${dx}, dy = ${dy} equals cfl / sum bit for bit`, String(sim.maxDt() === sim.cfl / sum), 'true');
+    same(`the two directions maxed instead, s`, fmt(sim.cfl / mx), want[dx][1]);
+    same(`the ratio`, fmt(sum / mx), want[dx][2]);
+  }
+  const sim = new ShallowWater({ nx: 6, ny: 5, dx: 7, dy: 11, bed: () => -h0, eta0: 0, manning: 0 });
+  uniform(sim, h0, u0, v0);
+  const dt0 = sim.maxDt(), k = sim.idx(2, 2), hd = 0.9 * sim.minDepth;
+  sim.h[k] = hd; sim.hu[k] = hd * 100; sim.hv[k] = 0;
+  const ud = sim.vel(sim.hu[k], hd), cd = Math.sqrt(G * hd);
+  same('a cell below minDepth leaves maxDt alone', String(sim.maxDt() === dt0), 'true');
+  same('its depth, m', fmt(hd), '9.0000e-4');
+  same('its speed, m/s', fmt(ud), '89.013676');
+  same('the step counting it would give, s', fmt(sim.cfl / ((ud + cd) / 7 + cd / 11)), '0.035327');
+}
+
+// ---- 10. the open boundaries (tools/verify-physics.mjs:1254-1422) ---------------------------
+{
+  {
+    const nx = 7, ny = 5;
+    const sim = new ShallowWater({ nx, ny, dx: 3, dy: 4, manning: 0,
+      bed: (x, y) => -5 + 0.3 * Math.sin(x * 0.2) + 0.2 * Math.cos(y * 0.3),
+      eta0: (x, y) => 0.1 * Math.sin(x * 0.15) - 0.05 * Math.cos(y * 0.11) });
+    sim.boundaries = { west: outflow, east: outflow, south: outflow, north: outflow };
+    for (let j = 0; j < ny; j++) for (let i = 0; i < nx; i++) {
+      const k = sim.idx(i, j);
+      sim.hu[k] = sim.h[k] * (0.5 + 0.1 * i);
+      sim.hv[k] = sim.h[k] * (-0.3 - 0.05 * j);
+    }
+    sim.applyBC();
+    let ghosts = 0, bad = 0, worst = 0;
+    for (let j = -sim.ng; j < ny + sim.ng; j++) for (let i = -sim.ng; i < nx + sim.ng; i++) {
+      if (i >= 0 && i < nx && j >= 0 && j < ny) continue;
+      ghosts++;
+      const g = sim.idx(i, j), src = sim.idx(Math.max(0, Math.min(nx - 1, i)), Math.max(0, Math.min(ny - 1, j)));
+      for (const f of ['b', 'h', 'hu', 'hv']) {
+        const d = Math.abs(sim[f][g] - sim[f][src]);
+        if (d !== 0) { bad++; worst = Math.max(worst, d); }
+      }
+    }
+    same('outflow ghosts: count, mismatched fields, worst', `${ghosts} ${bad} ${worst}`, '64 0 0');
+  }
+  {
+    const h0 = 4, u0 = 1.5;
+    const sim = new ShallowWater({ nx: 30, ny: 4, dx: 5, bed: () => -h0, eta0: 0, manning: 0 });
+    sim.boundaries = { west: outflow, east: outflow, south: periodic, north: periodic };
+    uniform(sim, h0, u0, 0);
+    const v0 = sim.volume(), dt = 0.5 * sim.maxDt();
+    for (let s = 0; s < 200; s++) sim.step(dt);
+    same('uniform stream through outflow: departure, travel m, volume drift',
+      `${departure(sim)} ${fmt(u0 * sim.t)} ${((sim.volume() - v0) / v0).toExponential(3)}`, '0 24.062063 0.000e+0');
+  }
+  {
+    const nx = 200, dx = 5, h0 = 10, A = 0.1, T = 150;
+    const run = (east, sponge) => {
+      const sim = new ShallowWater({ nx, ny: 1, dx, bed: () => -h0, manning: 0,
+        eta0: (x) => A * Math.exp(-(((x - 250) / 80) ** 2)) });
+      sim.boundaries = { west: reflect, east, south: reflect, north: reflect };
+      for (let i = 0; i < nx; i++) {
+        const k = sim.idx(i, 0);
+        sim.hu[k] = sim.h[k] * (sim.b[k] + sim.h[k]) * Math.sqrt(G / h0);
+      }
+      if (sponge) sim.forcing = makeSponge(sim, { side: 'east', etaRef: 0, width: 40, strength: 0.08 });
+      // the gauge at x = 700 m sees the pulse pass before t = 75 s and what comes back after
+      const kg = sim.idx(Math.round(700 / dx), 0), dt = 0.9 * sim.maxDt();
+      let inc = 0, back = 0;
+      while (sim.t < T) {
+        sim.step(dt);
+        const e = Math.abs(sim.b[kg] + sim.h[kg]);
+        if (sim.t < 75) inc = Math.max(inc, e); else back = Math.max(back, e);
+      }
+      return { inc, R: back / inc };
+    };
+    const wall = run(reflect, false), open = run(outflow, false), sp = run(reflect, true);
+    same('solid wall returns, %', (100 * wall.R).toFixed(3), '99.559');
+    same('of a pulse of, m', fmt(wall.inc), '0.098698');
+    same('outflow returns, %', (100 * open.R).toFixed(3), '0.126');
+    same('a width-40 sponge before the wall returns, %', (100 * sp.R).toFixed(3), '0.123');
+  }
+  {
+    const sim = new ShallowWater({ nx: 40, ny: 8, dx: 10, eta0: 0, manning: 0,
+      bed: (x, y) => -8 + 3 * Math.sin(x * 0.01) * Math.cos(y * 0.02) });
+    sim.boundaries = { west: reflect, east: reflect, south: reflect, north: reflect };
+    sim.forcing = makeSponge(sim, { side: 'east', etaRef: 0 });
+    const h0 = Float64Array.from(sim.h), dt = 0.5 * sim.maxDt();
+    for (let s = 0; s < 40; s++) sim.step(dt);
+    let d = 0;
+    for (let j = 0; j < sim.ny; j++) for (let i = 0; i < sim.nx; i++) {
+      const k = sim.idx(i, j);
+      d = Math.max(d, Math.abs(sim.h[k] - h0[k]));
+    }
+    same('sponge over a lake at rest: max |dh| m, max speed m/s', `${d.toExponential(3)} ${fmt(sim.maxSpeed())}`,
+      '8.882e-16 5.0856e-15');
+  }
 }
 
 console.log(fails ? `${fails} FAILED of ${checks}` : `ALL PASS ${checks}/${checks}`);

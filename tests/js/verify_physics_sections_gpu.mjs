@@ -1,6 +1,7 @@
-// Sections 1, 6, 7, 8 and 10 of the reference's tools/verify-physics.mjs (the
-// inertial oscillation under Coriolis, the HLLC contact wave, gravity recovered
-// from a basin's long-wave period, the CFL step, the open boundaries), restated
+// Sections 1, 3 (its positivity rigs), 6, 7, 8 and 10 of the reference's
+// tools/verify-physics.mjs (the inertial oscillation under Coriolis, the
+// positivity limiter, the HLLC contact wave, gravity recovered from a basin's
+// long-wave period, the CFL step, the open boundaries), restated
 // over the drop-in solver on the GPU, with every figure those sections print
 // held to the digits the reference prints.
 // The expected strings are the reference's own output under Node 12.22.9 (the
@@ -100,6 +101,52 @@ function departure(sim) {
   }
   same('inertial: circle radius, m', fmt(0.5 * (rMin + rMax)), '4999.987693');
   same('inertial: out-of-round, m', fmt(rMax - rMin), '5.013991');
+}
+
+// ---- 3. the positivity limiter's rigs (tools/verify-physics.mjs:470-583) --------------------
+{
+  {
+    // a 0.5 m layer torn apart at 9 m/s, one residual at 5x the CFL step
+    const nx = 40, h0 = 0.5;
+    const sim = new ShallowWater({ nx, ny: 1, dx: 2, bed: () => 0, eta0: h0, manning: 0 });
+    sim.boundaries = { west: reflect, east: reflect, south: reflect, north: reflect };
+    for (let i = 0; i < nx; i++) {
+      const k = sim.idx(i, 0);
+      sim.h[k] = h0; sim.hu[k] = h0 * (i < nx / 2 ? -9 : 9);
+    }
+    const dt = 5 * sim.maxDt();
+    sim.applyBC();
+    sim.residual(dt);
+    let clipped = 0, minTheta = 1, minH = Infinity;
+    for (let i = 0; i < nx; i++) {
+      const k = sim.idx(i, 0);
+      if (sim.theta[k] < 1) { clipped++; minTheta = Math.min(minTheta, sim.theta[k]); }
+      minH = Math.min(minH, sim.h[k] + dt * sim.rh[k]);
+    }
+    same('positivity: cells clipped, worst theta, dt', `${clipped} ${fmt(minTheta)} ${fmt(dt)}`, '38 0.663145 0.335103');
+    same('positivity: min(h + dt rh), m', minH === 0 ? '0' : minH.toExponential(3), '0');
+  }
+  {
+    // one wet cell in a dry row, one residual at 5x the CFL step
+    const nx = 9, dx = 2, h0 = 0.5, ic = 4;
+    const sim = new ShallowWater({ nx, ny: 1, dx, bed: () => 0, eta0: 0, manning: 0 });
+    sim.boundaries = { west: reflect, east: reflect, south: reflect, north: reflect };
+    const kc = sim.idx(ic, 0);
+    sim.h[kc] = h0;
+    const dt = 5 * sim.maxDt(), c = Math.sqrt(G * h0);
+    sim.applyBC();
+    sim.residual(dt);
+    const end = (i) => { const k = sim.idx(i, 0); return sim.h[k] + dt * sim.rh[k]; };
+    let untouched = 0;
+    for (let i = 0; i < nx; i++) if (i !== ic && sim.theta[sim.idx(i, 0)] === 1) untouched++;
+    // the solver's value, one ulp above the double nearest 2/3 (verify-physics.mjs:553-581)
+    same('positivity: theta of the over-drawing cell', sim.theta[kc].toPrecision(17), '0.66666666666666674');
+    same('positivity: 3 dx / (4 c dt)', fmt(3 * dx / (4 * c * dt)), '0.666667');
+    same('positivity: dt, s; HLL dry-face flux 2 c h0 / 3', `${fmt(dt)} ${fmt(2 * c * h0 / 3)}`, '1.016102 0.738115');
+    same('positivity: the cell ends the stage at, m', String(end(ic)), '0');
+    same('positivity: its neighbours end at, m', `${fmt(end(ic - 1))} + ${fmt(end(ic + 1))}`, '0.250000 + 0.250000');
+    same('positivity: untouched cells with theta = 1 exactly', String(untouched), '8');
+  }
 }
 
 // ---- 6. the HLLC contact wave (tools/verify-physics.mjs:769-919) ----------------------------

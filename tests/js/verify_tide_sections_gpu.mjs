@@ -1,7 +1,7 @@
-// Sections 3 and 3c of the reference's tools/verify-tide.mjs (a long-wave pulse
-// leaving through a Flather boundary against a solid-wall control, and the same
-// metric at five amplitudes split into its boundary part by a 140 km
-// counterfactual domain), restated over the drop-in solver on the GPU, with every
+// Sections 3, 3b and 3c of the reference's tools/verify-tide.mjs (a long-wave
+// pulse leaving through a Flather boundary against a solid-wall control, the
+// same boundary met at 0-60 degrees of incidence in 2-D, and the metric at five
+// amplitudes split into its boundary part by a 140 km counterfactual domain), restated over the drop-in solver on the GPU, with every
 // figure those sections print held to the digits the reference prints.  The
 // expected strings are the reference's own output under Node 12.22.9; no
 // friction enters, so the figures do not depend on the engine's Math.pow.
@@ -53,6 +53,58 @@ const peak = (f) => { let m = 0; for (let i = 10; i < 600 - 40; i++) m = Math.ma
   const rF = returned(absorber()), rW = returned(reflect);
   same('solid wall returns, %', (100 * rW).toFixed(1), WANT.wall);
   same('Flather returns, %', (100 * rF).toFixed(3), WANT.flather);
+}
+
+// ---- 3b. oblique incidence (tools/verify-tide.mjs:844-1000) ---------------------------------
+{
+  // a plane pulse at angle th to the east boundary in a channel periodic in y;
+  // what the short domain's boundary returns is read against a long domain
+  // whose boundary the wave has not reached, in the window 6-19 km
+  const dx = 200, Ly = 12000, ny = Ly / dx, X0 = 4000, SIG = 2500, A = 0.1, iW0 = 30, iW1 = 95;
+  const wave = (nx, th, bc) => {
+    const ky = th === 0 ? 0 : 2 * Math.PI / Ly;
+    const kx = th === 0 ? 2 * Math.PI / 6000 : ky / Math.tan(th);
+    const k = Math.hypot(kx, ky);
+    const sim = new ShallowWater({ nx, ny, dx, dy: dx, bed: () => -h, eta0: 0, manning: 0, cfl: 0.4 });
+    for (let j = 0; j < ny; j++) for (let i = 0; i < nx; i++) {
+      const x = (i + 0.5) * dx, y = (j + 0.5) * dx;
+      const E = Math.exp(-(((x - X0) / SIG) ** 2)), e = A * E * Math.cos(kx * x) * Math.cos(ky * y), g = sim.idx(i, j);
+      sim.h[g] = h + e;
+      sim.hu[g] = c0 * (kx / k) * e;
+      sim.hv[g] = -c0 * (ky / k) * A * E * Math.sin(kx * x) * Math.sin(ky * y);
+    }
+    sim.boundaries = { west: reflect, east: bc, south: periodic, north: periodic };
+    return { sim, lam: 2 * Math.PI / k };
+  };
+  const inWindow = (f) => { let m = 0; for (let j = 0; j < ny; j++) for (let i = iW0; i < iW1; i++) m = Math.max(m, f(i, j)); return m; };
+  const apart = (a, b) => inWindow((i, j) => Math.abs(a.eta(i, j) - b.eta(i, j)));
+  const want = {
+    0: '0 6000 30 0.022 0.076 79.2 84.8 0.089',
+    30: '30 6000 30 12.978 5.655 65.3 89.9 6.288',
+    45: '45 8485 42 8.961 7.317 50.7 75.3 9.721',
+    60: '60 10392 52 24.529 9.262 43.1 70.3 13.168',
+  };
+  for (const deg of [0, 30, 45, 60]) {
+    const th = deg * Math.PI / 180;
+    const S = wave(100, th, absorber()), C = wave(240, th, absorber()), W = wave(100, th, reflect);
+    const C2 = deg === 60 ? wave(320, th, absorber()) : null;
+    const dt = Math.min(S.sim.maxDt(), C.sim.maxDt()) * 0.98;
+    const steps = Math.ceil(((100 * dx - X0) + (100 - iW0) * dx) / (c0 * Math.cos(th)) / dt);
+    let mB = 0, mW = 0, mB2 = 0, mInc = 0;
+    for (let n = 0; n < steps; n++) {
+      S.sim.step(dt); C.sim.step(dt); W.sim.step(dt);
+      if (C2) C2.sim.step(dt);
+      mB = Math.max(mB, apart(S.sim, C.sim));
+      mW = Math.max(mW, apart(W.sim, C.sim));
+      mInc = Math.max(mInc, inWindow((i, j) => Math.abs(C.sim.eta(i, j))));
+      if (C2) mB2 = Math.max(mB2, apart(S.sim, C2.sim));
+    }
+    const naive = inWindow((i, j) => Math.abs(S.sim.eta(i, j))) / A;
+    same(`angle, wavelength m, cells/L, naive %, boundary %, wall %, arrived %, returned % (${deg} deg)`,
+      [deg, S.lam.toFixed(0), (S.lam / dx).toFixed(0), (100 * naive).toFixed(3), (100 * mB / A).toFixed(3),
+        (100 * mW / A).toFixed(1), (100 * mInc / A).toFixed(1), (100 * mB / mInc).toFixed(3)].join(' '), want[deg]);
+    if (C2) same('at 60 deg, the 48 km and 64 km controls, %', `${(100 * mB / A).toFixed(3)} ${(100 * mB2 / A).toFixed(3)}`, '9.262 9.262');
+  }
 }
 
 // ---- 3c. finite amplitude (tools/verify-tide.mjs:1002-1062) ---------------------------------

@@ -99,6 +99,6 @@ def test_js_verify_physics_sections_to_the_printed_digit_on_gpu():
 @needs_node
 @pytest.mark.gpu
 def test_js_verify_tide_sections_to_the_printed_digit_on_gpu():
-    """tools/verify-tide.mjs sections 3 and 3c (Flather absorption against a wall, and the
-    finite-amplitude table) through the drop-in: every printed figure equal."""
+    """tools/verify-tide.mjs sections 3, 3b and 3c (Flather absorption against a wall, the
+    oblique-incidence and finite-amplitude tables) through the drop-in: every printed figure equal."""
     assert run_node("tests/js/verify_tide_sections_gpu.mjs") == 0

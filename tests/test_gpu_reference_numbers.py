@@ -98,16 +98,28 @@ def test_resonance_sweep_peaks_at_the_recorded_1_0220():
         gains.append(0.5 * (eta.max() - eta.min()) / 0.5)
         print(f"  T/T_res = {r:.3f}: gain {gains[-1]:.5f} ({eta.shape[0]} samples)", flush=True)
     i = int(np.argmax(gains))
-    (p0, g0), (p1, g1), (p2, g2) = [(ratios[k], gains[k]) for k in (i - 1, i, i + 1)]
-    d1, d2, d3 = (p0 - p1) * (p0 - p2), (p1 - p0) * (p1 - p2), (p2 - p0) * (p2 - p1)
-    A2 = g0 / d1 + g1 / d2 + g2 / d3
-    B2 = -(g0 * (p1 + p2) / d1 + g1 * (p0 + p2) / d2 + g2 * (p0 + p1) / d3)
-    refined = -B2 / (2 * A2)
+
+    def vertex(ks):
+        """The vertex of the parabola through three (ratio, gain) points, and its curvature."""
+        (p0, g0), (p1, g1), (p2, g2) = [(ratios[k], gains[k]) for k in ks]
+        d1, d2, d3 = (p0 - p1) * (p0 - p2), (p1 - p0) * (p1 - p2), (p2 - p0) * (p2 - p1)
+        A2 = g0 / d1 + g1 / d2 + g2 / d3
+        B2 = -(g0 * (p1 + p2) / d1 + g1 * (p0 + p2) / d2 + g2 * (p0 + p1) / d3)
+        return -B2 / (2 * A2), A2
+
+    refined, A2 = vertex((i - 1, i, i + 1))
     print(f"  fitted resonance T/T_res = {refined:.6f} (recorded 1.02200)", flush=True)
     assert A2 < 0 and 0 < i < len(ratios) - 1
     assert abs(refined - 1.02200) < 1e-5
     assert abs(gains[ratios.index(0.975)] - 2.7820) < 1e-4
     assert abs(gains[ratios.index(1.025)] - 2.7866) < 1e-4
+    # the whole gain table and the wider stencils, as the suite prints them
+    # (verify-tide.mjs:596-640: "0.55:2.032 ... 1.7:2.418", "wider stencils give
+    # 1.01653 (0.975, 1.025, 1.1), 1.02700 (0.95, 1.025, 1.25)")
+    assert " ".join(f"{g:.3f}" for g in gains) == \
+        "2.032 2.603 2.742 2.774 2.782 2.784 2.787 2.783 2.768 2.683 2.418"
+    wide = [vertex([ratios.index(r) for r in rs])[0] for rs in ((0.975, 1.025, 1.10), (0.95, 1.025, 1.25))]
+    assert [f"{w:.5f}" for w in wide] == ["1.01653", "1.02700"]
 
 
 def test_drying_flat_tide_matches_the_recorded_figures():

@@ -28,7 +28,7 @@ function same(label, got, want) {
   if (!ok) fails++;
   console.log(`${ok ? 'PASS' : 'FAIL'}  ${label}: ${got}${ok ? '' : `  (the reference prints ${want})`}`);
 }
-// the drivers' stepping loop: step(min(maxDt, T - t)) until t reaches T
+// the suites' stepping loop: step(min(maxDt, T - t)) until t reaches T
 const runTo = (sim, T) => { while (sim.t < T) sim.step(Math.min(sim.maxDt(), T - sim.t)); };
 
 // ---- 3. Ritter's dam break on a dry bed (tools/verify.mjs:195-232) --------------------------

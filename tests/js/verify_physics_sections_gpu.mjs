@@ -1,14 +1,15 @@
-// Sections 1, 2, 3 (its positivity rigs), 6, 7, 8 and 10 of the reference's
+// Sections 1, 2, 3, 6, 7, 8 and 10 of the reference's
 // tools/verify-physics.mjs (the inertial oscillation under Coriolis, Manning
-// friction, the positivity limiter, the HLLC contact wave, gravity recovered from
+// friction, wetting and drying with the positivity limiter, the HLLC contact wave, gravity recovered from
 // a basin's long-wave period, the CFL step, the open boundaries), restated
 // over the drop-in solver on the GPU, with every figure those sections print
 // held to the digits the reference prints.
 // The expected strings are the reference's own output under Node 12.22.9 (the
-// engine the GPU host runs this script with).  Only section 2 has friction, and
-// its figures are printed to 4-6 digits, far above the one-ulp freedom of the
-// engine's Math.pow (the device uses fdlibm's); the spin-down identity's own
-// residual, which that ulp can move, is held to the reference's 1e-10 bar.  The
+// engine the GPU host runs this script with).  Where friction enters (sections 2
+// and 3's rigs) the figures held to the digit are printed to 4-6 digits or are
+// counts, far above the one-ulp freedom of the engine's Math.pow (the device uses
+// fdlibm's); the residuals that ulp can move (the spin-down identity, the closed
+// boxes' volume drift) are held to the reference's bars instead.  The
 // host-side Math.hypot and Math.atan2 run in the same engine as the reference's.
 //
 //   node tests/js/verify_physics_sections_gpu.mjs
@@ -142,6 +143,58 @@ function departure(sim) {
     }
     same(`spin-down h = ${h0} m: final u, stiffness dt A u0`, `${fmt(sim.vel(sim.hu[k], sim.h[k]))} ${fmt(dt * A * u0)}`, wantB[h0]);
     same(`spin-down h = ${h0} m: identity holds to 1e-10 over 200 steps`, String(worst <= 1e-10), 'true');
+  }
+}
+
+// ---- 3. wetting and drying in closed boxes (tools/verify-physics.mjs:362-465) ---------------
+{
+  // theta is refilled by every residual, so after step() this counts stage 2's clips
+  const clipsOf = (sim) => { let n = 0; for (let k = 0; k < sim.theta.length; k++) if (sim.theta[k] < 1) n++; return n; };
+  // run to T under the analytic step budget, counting dry cells every `every` steps
+  const wetDry = (sim, T, budget, every, withSpeed) => {
+    const nx = sim.nx, ny = sim.ny, v0 = sim.volume(), wasDry = new Set();
+    for (let j = 0; j < ny; j++) for (let i = 0; i < nx; i++) if (sim.h[sim.idx(i, j)] <= sim.minDepth) wasDry.add(j * nx + i);
+    const dry0 = wasDry.size;
+    let floored = 0, steps = 0, maxDry = 0, rewet = 0, clips = 0, spd = 0, finite = true;
+    while (sim.t < T && steps < budget) {
+      sim.step(Math.min(sim.maxDt(), T - sim.t));
+      steps++;
+      floored += sim.massFloored;
+      clips += clipsOf(sim);
+      if (!sim.finite()) { finite = false; break; }
+      if (steps % every === 0) {
+        let d = 0;
+        for (let j = 0; j < ny; j++) for (let i = 0; i < nx; i++) {
+          if (sim.h[sim.idx(i, j)] <= sim.minDepth) d++;
+          else if (wasDry.delete(j * nx + i)) rewet++;
+        }
+        maxDry = Math.max(maxDry, d);
+        if (withSpeed) spd = Math.max(spd, sim.maxSpeed());
+      }
+    }
+    const drift = Math.abs((sim.volume() - v0) / v0);
+    return { text: [steps, budget, fmt(sim.t), maxDry, rewet, dry0, floored, clips].concat(withSpeed ? [fmt(spd)] : []).join(' '),
+      ok: finite && drift <= 1e-12 };
+  };
+  {
+    const nx = 400, dx = 2.5, h0 = 3, c0 = Math.sqrt(G * h0);
+    const budget = Math.ceil(60 / (0.45 / (2 * ((2 * c0 + c0) / dx + c0 / dx))));
+    const sim = new ShallowWater({ nx, ny: 1, dx, bed: () => 0, eta0: (x) => (x < 400 ? h0 : 0), manning: 0.02 });
+    sim.boundaries = { west: reflect, east: reflect, south: reflect, north: reflect };
+    const r = wetDry(sim, 60, budget, 5, false);
+    same('dam break: steps, budget, t, most dry, re-wetted, dry at t = 0, floored, clips', r.text, '580 2315 60.000000 237 109 240 0 0');
+    same('dam break: finite, volume drift within 1e-12', String(r.ok), 'true');
+  }
+  {
+    const nx = 120, ny = 60, dx = 5, Hmax = 2.5 + 8.6, uB = 2 * Math.sqrt(G * Hmax), cB = Math.sqrt(G * Hmax);
+    const budget = Math.ceil(90 / (0.45 / ((uB + cB) / dx + cB / dx)));
+    const sim = new ShallowWater({ nx, ny, dx, dy: dx, manning: 0.025, eta0: (x) => (x < 180 ? 2.5 : 0),
+      bed: (x, y) => -8.6 + 12 * Math.max(0, x - 250) / 400 + 0.6 * Math.sin(y * 0.02) });
+    sim.boundaries = { west: reflect, east: reflect, south: reflect, north: reflect };
+    const r = wetDry(sim, 90, budget, 10, true);
+    same('run-up 2D: steps, budget, t, most dry, re-wetted, dry at t = 0, floored, clips, max speed', r.text,
+      '819 1670 90.000000 764 764 764 0 0 5.505386');
+    same('run-up 2D: finite, volume drift within 1e-12', String(r.ok), 'true');
   }
 }
 

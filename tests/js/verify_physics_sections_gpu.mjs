@@ -1,15 +1,16 @@
-// Sections 1, 2, 3, 6, 7, 8 and 10 of the reference's
+// Sections 1-8 and 10 of the reference's
 // tools/verify-physics.mjs (the inertial oscillation under Coriolis, Manning
-// friction, wetting and drying with the positivity limiter, the HLLC contact wave, gravity recovered from
+// friction, wetting and drying with the positivity limiter, the desingularised
+// velocity and a 1-D run-up, 2-D isotropy, the HLLC contact wave, gravity recovered from
 // a basin's long-wave period, the CFL step, the open boundaries), restated
 // over the drop-in solver on the GPU, with every figure those sections print
 // held to the digits the reference prints.
 // The expected strings are the reference's own output under Node 12.22.9 (the
-// engine the GPU host runs this script with).  Where friction enters (sections 2
-// and 3's rigs) the figures held to the digit are printed to 4-6 digits or are
+// engine the GPU host runs this script with).  Where friction enters (sections 2,
+// 3, 4's run-up and 5) the figures held to the digit are printed to 4-6 digits or are
 // counts, far above the one-ulp freedom of the engine's Math.pow (the device uses
 // fdlibm's); the residuals that ulp can move (the spin-down identity, the closed
-// boxes' volume drift) are held to the reference's bars instead.  The
+// boxes' volume drift, the transpose residuals) are held to the reference's bars.  The
 // host-side Math.hypot and Math.atan2 run in the same engine as the reference's.
 //
 //   node tests/js/verify_physics_sections_gpu.mjs
@@ -242,6 +243,77 @@ function departure(sim) {
     same('positivity: its neighbours end at, m', `${fmt(end(ic - 1))} + ${fmt(end(ic + 1))}`, '0.250000 + 0.250000');
     same('positivity: untouched cells with theta = 1 exactly', String(untouched), '8');
   }
+}
+
+// ---- 4. vel() and a 1-D run-up (tools/verify-physics.mjs:585-690) ---------------------------
+{
+  const sim = new ShallowWater({ nx: 4, ny: 4, dx: 1, bed: () => -1, eta0: 0 });
+  let worst = 0, at = null;
+  for (const h of [1.01e-3, 1e-2, 0.1, 0.5, 1, 3.7, 12, 100, 1e3]) for (const u of [-9.3, -0.4, 1e-6, 0.7, 5, 40]) {
+    const rel = Math.abs(sim.vel(h * u, h) - u) / Math.abs(u);
+    if (rel > worst) { worst = rel; at = [h, u]; }
+  }
+  same('vel() against hu/h: worst relative error, at h, u', `${fmt(worst)} ${at[0]} ${at[1]}`, '2.7756e-16 0.1 -0.4');
+  const s2 = new ShallowWater({ nx: 4, ny: 4, dx: 1, bed: () => -1, eta0: 0, minDepth: 1e-3 });
+  let peak = 0, raw = 0;
+  for (let e = 0; e <= 60; e++) {
+    const h = Math.pow(10, -e / 2);
+    peak = Math.max(peak, Math.abs(s2.vel(2e-3, h)));
+    raw = Math.max(raw, Math.abs(2e-3 / h));
+  }
+  same('vel() as h -> 0: peak, ceiling hq/minDepth, raw hu/h, vel(hq, 0)',
+    `${fmt(peak)} ${fmt(2e-3 / s2.minDepth)} ${fmt(raw)} ${s2.vel(2e-3, 0)}`, '2.000000 2.000000 2.0000e+27 0');
+  // a dam break over a dry flat onto a 1:25 beach
+  const nx = 300, dx = 4, H = 4, bed = (x) => (x < 500 ? 0 : (x - 500) / 25);
+  const run = new ShallowWater({ nx, ny: 1, dx, bed, eta0: (x) => (x < 300 ? H : 0), manning: 0.02 });
+  run.boundaries = { west: reflect, east: reflect, south: reflect, north: reflect };
+  const uB = 2 * Math.sqrt(G * H), cB = Math.sqrt(G * H), budget = Math.ceil(70 / (0.45 / ((uB + cB) / dx + cB / dx)));
+  let steps = 0, finite = true, spd = 0, runup = -Infinity, dryFlat = 0;
+  for (let i = 0; i < nx; i++) if (run.h[run.idx(i, 0)] === 0 && bed((i + 0.5) * dx) <= 0) dryFlat++;
+  while (run.t < 70 && steps < budget) {
+    run.step(Math.min(run.maxDt(), 70 - run.t));
+    steps++;
+    if (!run.finite()) { finite = false; break; }
+    spd = Math.max(spd, run.maxSpeed());
+    for (let i = nx - 1; i >= 0; i--) if (run.depth(i, 0) > run.minDepth) { runup = Math.max(runup, bed((i + 0.5) * dx)); break; }
+  }
+  same('run-up 1D: dry flat cells, steps, budget, t, finite, max speed, 2 sqrt(g H), highest bed reached',
+    `${dryFlat} ${steps} ${budget} ${fmt(run.t)} ${finite} ${fmt(spd)} ${fmt(uB)} ${fmt(runup)}`,
+    '50 488 975 70.000000 true 7.011743 12.526228 3.440000');
+}
+
+// ---- 5. 2-D isotropy: the transpose test (tools/verify-physics.mjs:692-767) -----------------
+{
+  const n = 64, d = 6;
+  const bedA = (x, y) => -6 + 1.8 * Math.sin(x * 0.011) + 1.1 * Math.cos(y * 0.019)
+    + 2.2 * Math.exp(-(((x - 150) / 70) ** 2) - (((y - 260) / 50) ** 2));
+  const etaA = (x, y) => 0.4 * Math.exp(-(((x - 90) / 60) ** 2) - (((y - 300) / 90) ** 2))
+    - 0.25 * Math.exp(-(((x - 320) / 80) ** 2) - (((y - 120) / 70) ** 2));
+  // the problem, and the same problem with x and y exchanged
+  const build = (T) => {
+    const sim = new ShallowWater({ nx: n, ny: n, dx: d, dy: d, manning: 0.02,
+      bed: T ? (x, y) => bedA(y, x) : bedA, eta0: T ? (x, y) => etaA(y, x) : etaA });
+    sim.boundaries = { west: reflect, east: reflect, south: reflect, north: reflect };
+    for (let j = 0; j < n; j++) for (let i = 0; i < n; i++) {
+      const k = sim.idx(i, j), x = T ? (j + 0.5) * d : (i + 0.5) * d, y = T ? (i + 0.5) * d : (j + 0.5) * d;
+      const u = 0.6 * Math.sin(x * 0.008), v = -0.35 * Math.cos(y * 0.013);
+      sim.hu[k] = sim.h[k] * (T ? v : u);
+      sim.hv[k] = sim.h[k] * (T ? u : v);
+    }
+    return sim;
+  };
+  const A = build(false), B = build(true), dts = [];
+  for (let s = 0; s < 200; s++) dts.push(A.step());
+  for (const dt of dts) B.step(dt);
+  let dh = 0, dm = 0, plain = 0;
+  for (let j = 0; j < n; j++) for (let i = 0; i < n; i++) {
+    const a = A.idx(i, j), b = B.idx(j, i), bs = B.idx(i, j);
+    dh = Math.max(dh, Math.abs(A.h[a] - B.h[b]));
+    dm = Math.max(dm, Math.abs(A.hu[a] - B.hv[b]), Math.abs(A.hv[a] - B.hu[b]));
+    plain = Math.max(plain, Math.abs(A.h[a] - B.h[bs]));
+  }
+  same('transpose: max |h_A - h_B| without transposing (the control), m', fmt(plain), '5.873562');
+  same('transpose: h within 1e-12 and momenta within 1e-11 of the transposed run', String(dh <= 1e-12 && dm <= 1e-11), 'true');
 }
 
 // ---- 6. the HLLC contact wave (tools/verify-physics.mjs:769-919) ----------------------------

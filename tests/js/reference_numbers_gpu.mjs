@@ -36,5 +36,19 @@ for (const [name, [bfn, wantEta, wantSpeed]] of Object.entries(beds)) {
     report('smooth bump: volume drift is the 2.0662e-16 swe.mjs pins volume() to', drift === '2.0662e-16', drift);
   }
 }
+// the closed basin of tools/verify.mjs:149-176 (600 self-timed steps, Manning 0.02):
+// max speed 0.838432 m/s as printed, and the volume drift within the suite's 1e-12
+// (its last digits move with the engine's Math.pow: 8.4777e-15 under Node 12, the
+// README's 8.3047e-15 under another engine)
+{
+  const sim = new ShallowWater({ nx: 120, ny: 80, dx: 4, manning: 0.02,
+    bed: (x, y) => -8 + 3 * Math.sin(x * 0.02) * Math.cos(y * 0.03), eta0: (x, y) => (x < 240 ? 1.5 : 0.0) });
+  const v0 = sim.volume();
+  for (let n = 0; n < 600; n++) sim.step();
+  const drift = Math.abs(sim.volume() - v0) / v0, spd = fmt(sim.maxSpeed());
+  report('closed basin: max speed after 600 steps is the recorded 0.838432 m/s', spd === '0.838432', spd);
+  report('closed basin: volume drift within 1e-12, state finite', drift <= 1e-12 && sim.finite(), fmt(drift));
+}
+
 console.log(fails ? `${fails} FAILED of ${checks}` : `ALL PASS ${checks}/${checks}`);
 process.exitCode = fails ? 1 : 0;

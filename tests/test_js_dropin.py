@@ -83,7 +83,7 @@ def test_js_reference_recorded_still_water_figures_on_gpu():
 @needs_node
 @pytest.mark.gpu
 def test_js_verify_sections_to_the_printed_digit_on_gpu():
-    """tools/verify.mjs sections 3-5, 7, 8 (Ritter, celerity, Thacker, self-convergence,
+    """tools/verify.mjs sections 3-8 (Ritter, celerity, Thacker, Merian, self-convergence,
     dissipation) through the drop-in: every figure equal to the one the reference prints."""
     assert run_node("tests/js/verify_sections_gpu.mjs") == 0
 

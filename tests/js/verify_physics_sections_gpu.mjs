@@ -1,7 +1,8 @@
-// Sections 1-8 and 10 of the reference's
+// Sections 1-10 of the reference's
 // tools/verify-physics.mjs (the inertial oscillation under Coriolis, Manning
 // friction, wetting and drying with the positivity limiter, the desingularised
-// velocity and a 1-D run-up, 2-D isotropy, the HLLC contact wave, gravity recovered from
+// velocity and a 1-D run-up, 2-D isotropy, the HLLC contact wave, the HLL
+// wave-speed estimate read back out of hllc(), gravity recovered from
 // a basin's long-wave period, the CFL step, the open boundaries), restated
 // over the drop-in solver on the GPU, with every figure those sections print
 // held to the digits the reference prints.
@@ -14,7 +15,7 @@
 // host-side Math.hypot and Math.atan2 run in the same engine as the reference's.
 //
 //   node tests/js/verify_physics_sections_gpu.mjs
-import { ShallowWater, reflect, periodic, outflow, makeSponge } from '../../water-simulator_amd/js/swe.mjs';
+import { ShallowWater, hllc, reflect, periodic, outflow, makeSponge } from '../../water-simulator_amd/js/swe.mjs';
 
 const G = 9.80665;
 let fails = 0, checks = 0;
@@ -420,6 +421,56 @@ function departure(sim) {
   same('its depth, m', fmt(hd), '9.0000e-4');
   same('its speed, m/s', fmt(ud), '89.013676');
   same('the step counting it would give, s', fmt(sim.cfl / ((ud + cd) / 7 + cd / 11)), '0.035327');
+}
+
+// ---- 9. the wave speeds hllc() uses, read back out of it (tools/verify-physics.mjs:1062-1252) --
+{
+  // the exact Riemann solution by bisection on the star depth
+  const exact = (hL, uL, hR, uR) => {
+    const cL = Math.sqrt(G * hL), cR = Math.sqrt(G * hR);
+    const f = (hh, hK) => (hh <= hK ? 2 * (Math.sqrt(G * hh) - Math.sqrt(G * hK))
+      : (hh - hK) * Math.sqrt(0.5 * G * (hh + hK) / (hh * hK)));
+    const F = (hh) => f(hh, hL) + f(hh, hR) + (uR - uL);
+    let lo = 1e-12, hi = Math.max(hL, hR);
+    while (F(hi) < 0) hi *= 2;
+    for (let n = 0; n < 200; n++) { const m = 0.5 * (lo + hi); if (F(m) > 0) hi = m; else lo = m; }
+    const hs = 0.5 * (lo + hi);
+    return { SL: hs > hL ? uL - cL * Math.sqrt(0.5 * (hs + hL) * hs / (hL * hL)) : uL - cL,
+      SR: hs > hR ? uR + cR * Math.sqrt(0.5 * (hs + hR) * hs / (hR * hR)) : uR + cR };
+  };
+  // the two-rarefaction estimate the solver declares
+  const estimate = (hL, uL, hR, uR) => {
+    const cL = Math.sqrt(G * hL), cR = Math.sqrt(G * hR), hStar = ((cL + cR) / 2 + (uL - uR) / 4) ** 2 / G;
+    return { hStar, sR: uR + cR * (hStar > hR ? Math.sqrt(0.5 * (hStar + hR) * hStar / (hR * hR)) : 1) };
+  };
+  // hllc() returns the pure upwind mass flux exactly when the whole fan moves one
+  // way, so the frame speed at which that starts is its wave speed
+  const out = [0, 0, 0];
+  const recover = (hL, uL, hR, uR) => {
+    const left = (sp) => { hllc(out, hL, hL * (uL + sp), hL * 0.3, hR, hR * (uR + sp), hR * -0.7, 1e-4); return out[0] === hL * (uL + sp); };
+    const right = (sp) => { hllc(out, hL, hL * (uL - sp), hL * 0.3, hR, hR * (uR - sp), hR * -0.7, 1e-4); return out[0] === hR * (uR - sp); };
+    const edge = (pred) => {
+      let lo = 0, hi = 1;
+      while (!pred(hi)) { hi *= 2; if (hi > 1e9) return NaN; }
+      for (let n = 0; n < 100; n++) { const m = 0.5 * (lo + hi); if (pred(m)) hi = m; else lo = m; }
+      return 0.5 * (lo + hi);
+    };
+    return { sL: -edge(left), sR: edge(right) };
+  };
+  const want = {
+    '10 0.1': '-9.902853 21.529455 21.529455 3.025000 74.6',
+    '10 1': '-9.902853 10.640362 10.640362 4.331139 8.380',
+    '1 0.9': '-3.131557 3.092744 3.092744 0.949342 0.001',
+  };
+  for (const [hL, hR] of [[10, 0.1], [10, 1], [1, 0.9]]) {
+    const e = exact(hL, 0, hR, 0), r = recover(hL, 0, hR, 0), est = estimate(hL, 0, hR, 0);
+    const over = 100 * (r.sR / e.SR - 1);
+    same(`hllc hL = ${hL} hR = ${hR}: recovered sL, sR; declared sR, h*; sR over the exact %`,
+      `${fmt(r.sL)} ${fmt(r.sR)} ${fmt(est.sR)} ${fmt(est.hStar)} ${over.toFixed(hL === 10 && hR === 0.1 ? 1 : 3)}`,
+      want[`${hL} ${hR}`]);
+    same(`hllc hL = ${hL} hR = ${hR}: recovered sR within 1e-12 of the declared estimate`,
+      String(Math.abs(r.sR - est.sR) / Math.abs(est.sR) <= 1e-12), 'true');
+  }
 }
 
 // ---- 10. the open boundaries (tools/verify-physics.mjs:1254-1422) ---------------------------

@@ -91,9 +91,9 @@ def test_js_verify_sections_to_the_printed_digit_on_gpu():
 @needs_node
 @pytest.mark.gpu
 def test_js_verify_physics_sections_to_the_printed_digit_on_gpu():
-    """tools/verify-physics.mjs sections 1-8 and 10 (Coriolis, Manning, wetting and drying,
-    positivity, vel and run-up, isotropy, the contact wave, gravity, the CFL step, the open
-    boundaries) through the drop-in: every printed figure equal."""
+    """tools/verify-physics.mjs sections 1-10 (Coriolis, Manning, wetting and drying,
+    positivity, vel and run-up, isotropy, the contact wave, gravity, the CFL step, hllc's wave
+    speeds, the open boundaries) through the drop-in: every printed figure equal."""
     assert run_node("tests/js/verify_physics_sections_gpu.mjs") == 0
 
 

@@ -127,6 +127,31 @@ int swe_max_dt(swe_ctx *ctx, double *out);
  * reference leaves in sim.theta / sim.rh after a step). */
 int swe_step(swe_ctx *ctx, double dt, const double ext[8], int want_theta, int want_rh,
              double *dt_taken, double *mass_floored);
+/* step(dt) split at its second applyBC (swe.mjs:784-812), for boundaries the
+ * device does not implement (a caller's own boundary function, run on the host
+ * over the caller's arrays; SWE_BC_STORED sides keep the ghost cells the caller
+ * uploaded).  swe_step_begin: dt (NaN: maxDt of the current state), applyBC of
+ * the device sides and stage 1 (swe.mjs:784-796).  Until swe_step_end the fields
+ * SWE_F_H/HU/HV are the stage-1 state, every cell as the reference leaves it
+ * (ghosts h + dt*rh), for the caller to download, fill the ghost ring of and
+ * upload.  swe_step_end: applyBC of the device sides, stage 2, friction, sponge
+ * and the next CFL step (swe.mjs:798-812); the result becomes the current state.
+ * ext: the flather values of the step (NULL in swe_step_end keeps swe_step_begin's).
+ * *dt_taken = 0: the step was refused (swe.mjs:784) and no swe_step_end follows.
+ * Between the two calls only field copies, scalars and swe_step_end are accepted. */
+int swe_step_begin(swe_ctx *ctx, double dt, const double ext[8], double *dt_taken);
+int swe_step_end(swe_ctx *ctx, const double ext[8], int want_theta, int want_rh, double *mass_floored);
+/* 1: every later step (swe_step, swe_step_end, swe_advance) also leaves stage 2's
+ * theta and rh/rhu/rhv in SWE_F_THETA/RH/RHU/RHV, as the reference's step leaves
+ * them in sim.theta and sim.rh after a step (swe.mjs:798-806). */
+int swe_keep_residual(swe_ctx *ctx, int on);
+/* The Flather (eta_ext, u_ext) per side the next step will use: the caller's
+ * values after swe_step / swe_apply_bc, the device-evaluated harmonics
+ * (swe_set_harmonic) during and after swe_advance. */
+int swe_get_ext(swe_ctx *ctx, double ext[8]);
+/* Copy only the ghost ring (two cells deep) of SWE_F_B/H/HU/HV into a W*H host
+ * array, leaving its interior untouched: what applyBC (swe.mjs:289-303) changes. */
+int swe_get_ghosts(swe_ctx *ctx, int field, double *host);
 /* Device-resident run of nsteps steps (no host round trip): dt policy SELF
  * (maxDt every step), FIXED (dt_value) or UNTIL (min(maxDt, t_end - t)); the
  * flather forcing comes from swe_set_harmonic.  An UNTIL (or SAMPLED) run ends
@@ -222,6 +247,8 @@ int swe_batch_set_harmonic(swe_batch *b, int basin, int side, int n_eta, double 
                            const double *omega, const double *phase, int n_u, double u0,
                            const double *uamp, const double *uomega, const double *uphase);
 int swe_batch_get_scalar(swe_batch *b, int basin, int which, double *value);
+/* swe_get_ext for one basin of the batch */
+int swe_batch_get_ext(swe_batch *b, int basin, double ext[8]);
 /* swe_set_tide for one basin of the batch */
 int swe_batch_set_tide(swe_batch *b, int basin, int n, const int32_t *m, const double *amp, const double *omega,
                        const double *phase, const double *lat, const double *lon);

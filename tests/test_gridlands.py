@@ -131,3 +131,85 @@ def test_python_and_js_adapters_sample_the_same_doubles(doc, settle):
     assert js.size == 3 * N
     for a, want in ((js[:N], b), (js[N:2 * N], h), (js[2 * N:], n)):
         assert np.array_equal(a.view(np.uint64), want.view(np.uint64))
+
+
+def _flood_fill_bodies(bed, surf, W, H):
+    """Brute force: breadth-first flood fill from each unvisited wet cell in index
+    order; per body its cells (sorted), level (first cell's surface), spread and
+    the lowest bed of its dry 4-neighbours."""
+    wet = (surf.astype(np.float64) - bed.astype(np.float64)) > 0
+    seen = np.zeros(W * H, bool)
+    out = []
+    for k0 in range(W * H):
+        if seen[k0] or not wet[k0]:
+            continue
+        seen[k0] = True
+        queue, cells, ring = [k0], [], []
+        while queue:
+            k = queue.pop(0)
+            cells.append(k)
+            r, c = divmod(k, W)
+            for rr, cc in ((r, c - 1), (r, c + 1), (r - 1, c), (r + 1, c)):
+                if 0 <= rr < H and 0 <= cc < W:
+                    m = rr * W + cc
+                    if not wet[m]:
+                        ring.append(float(bed[m]))
+                    elif not seen[m]:
+                        seen[m] = True
+                        queue.append(m)
+        level = float(surf[k0])
+        out.append(dict(level=level, cells=sorted(cells), ring_min=min(ring, default=math.inf),
+                        spread=max(abs(float(surf[k]) - level) for k in cells)))
+    return out
+
+
+def _random_grids(seed, n=12):
+    rng = np.random.default_rng(seed)
+    grids = []
+    for q in range(n):
+        W, H = int(rng.integers(1, 30)), int(rng.integers(1, 25))
+        bed = rng.normal(0, 1, W * H).astype(np.float32)
+        # lakes at a few shared levels (some perched, some uneven) over random beds
+        levels = rng.choice([-0.5, 0.0, 0.3, 0.8], W * H)
+        surf = np.where(rng.random(W * H) < 0.6, np.maximum(levels, bed), bed).astype(np.float32)
+        if q % 3 == 0:
+            surf = np.where(rng.random(W * H) < 0.1, surf + np.float32(0.01), surf).astype(np.float32)
+        grids.append((W, H, bed, surf))
+    return grids
+
+
+@pytest.mark.skipif(NODE is None, reason="node is not installed")
+def test_waterbodies_against_brute_force_flood_fill():
+    """Component labelling (both hosts) against a plain breadth-first flood fill on
+    random grids, single rows and columns included; then the invariant reports and
+    the settle step of the two hosts against each other."""
+    grids = _random_grids(7)
+    with tempfile.TemporaryDirectory() as d:
+        inp, out = os.path.join(d, "in.json"), os.path.join(d, "out.json")
+        with open(inp, "w") as f:
+            json.dump({"grids": [dict(W=W, H=H, bed=b.tolist(), surface=s.tolist()) for W, H, b, s in grids]}, f)
+        subprocess.run([NODE, os.path.join(ROOT, "tests", "js", "gridlands_bodies.mjs"), inp, out], check=True,
+                       timeout=120)
+        with open(out) as f:
+            js = json.load(f)
+    for (W, H, bed, surf), j in zip(grids, js):
+        want = _flood_fill_bodies(bed, surf, W, H)
+        py = gl.waterbodies(bed, surf, W, H)
+        assert len(py) == len(want) == len(j["bodies"])
+        for w, p, q in zip(want, py, j["bodies"]):
+            assert list(p["cells"]) == w["cells"] == q["cells"]
+            assert p["level"] == w["level"] == q["level"]
+            assert p["spread"] == w["spread"] == q["spread"]
+            assert p["ring_min"] == w["ring_min"] == (math.inf if q["ringMin"] is None else q["ringMin"])
+        inv = gl.verify_invariants(dict(bedM=bed, surfaceM=surf, W=W, H=H))
+        for k_py, k_js in (("wet", "wet"), ("surface_below_bed", "surfaceBelowBed"), ("bodies", "bodies"),
+                           ("lake_bodies", "lakeBodies"), ("lakes_not_flat", "lakesNotFlat"),
+                           ("perched_lakes", "perchedLakes"), ("perched_cells", "perchedCells"),
+                           ("largest_body", "largestBody"), ("worst_perch", "worstPerch")):
+            assert inv[k_py] == j["inv"][k_js], (k_py, inv[k_py], j["inv"][k_js])
+        s2 = surf.copy()
+        moved = gl.settle_lakes(bed, s2, W, H)
+        assert moved["bodies_moved"] == j["moved"]
+        assert np.array_equal(s2, np.asarray(j["settledSurface"], dtype=np.float32))
+        # after settling nothing is perched any more
+        assert gl.verify_invariants(dict(bedM=bed, surfaceM=s2, W=W, H=H))["perched_lakes"] == 0

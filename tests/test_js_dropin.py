@@ -17,10 +17,10 @@ needs_node = pytest.mark.skipif(NODE is None, reason="node is not installed")
 ADDON = os.path.join(ROOT, "water-simulator_amd", "napi", "swe_napi.node")
 
 
-def run_node(script, *args, timeout=600):
+def run_node(script, *args, timeout=600, env=None):
     print(f"[node] {script} {' '.join(args)}", flush=True)
     r = subprocess.run([NODE, os.path.join(ROOT, script), *args], cwd=ROOT, capture_output=True, text=True,
-                       timeout=timeout)
+                       timeout=timeout, env=env)
     sys.stdout.write(r.stdout[-6000:])
     sys.stderr.write(r.stderr[-3000:])
     return r.returncode
@@ -31,6 +31,22 @@ def test_js_host_pieces():
     if not os.path.exists(ADDON):
         pytest.skip("addon not built")
     assert run_node("tests/js/cpu_checks.mjs") == 0
+
+
+@needs_node
+def test_addon_refuses_arrays_of_the_wrong_length(tmp_path):
+    """Every addon binding that passes a typed array to the C library checks its
+    length first (W*H fields, H rows, samples x gauges series, one tide value per
+    constituent) and throws a RangeError, never calling the library with it.  Run
+    on the CPU against a stand-in libswe_amd.so (tests/napi_mock/swe_mock.c) that
+    the addon loads through LD_LIBRARY_PATH."""
+    if not os.path.exists(ADDON):
+        pytest.skip("addon not built")
+    lib = tmp_path / "libswe_amd.so"
+    subprocess.run(["gcc", "-O1", "-fPIC", "-shared", "-o", str(lib),
+                    os.path.join(ROOT, "tests", "napi_mock", "swe_mock.c")], check=True)
+    env = dict(os.environ, LD_LIBRARY_PATH=f"{tmp_path}:{os.environ.get('LD_LIBRARY_PATH', '')}")
+    assert run_node("tests/js/napi_lengths_cpu.mjs", env=env, timeout=120) == 0
 
 
 @needs_node

@@ -547,6 +547,7 @@ int swe_adj_create_levels(swe_ctx *c, int64_t nsteps, int64_t inner, int64_t out
   if (!c || !out || nsteps < 1 || inner < 1 || inner > (1 << 30) || outer < inner || outer % inner)
     return ctx_fail(c, SWE_EINVAL, "bad adjoint arguments (outer must be a multiple of inner)");
   *out = nullptr;
+  if (c->mid) return ctx_fail(c, SWE_ESTATE, "a step is in progress (swe_step_begin without swe_step_end)");
   for (int s = 0; s < 4; ++s)
     if (c->P.bc[s] != BC_REFLECT && c->P.bc[s] != BC_OUTFLOW && c->P.bc[s] != BC_PERIODIC)
       return ctx_fail(c, SWE_EINVAL, "the adjoint supports reflect, outflow and periodic boundaries");

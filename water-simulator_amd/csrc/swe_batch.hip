@@ -429,6 +429,14 @@ int swe_batch_get_scalar(swe_batch *b, int i, int which, double *v) {
   return SWE_OK;
 }
 
+int swe_batch_get_ext(swe_batch *b, int i, double ext[8]) {
+  if (!b || !ext || i < 0 || i >= (int)b->basins.size()) return bfail(b, SWE_EINVAL, "bad basin");
+  int rc = pull(b);
+  if (rc) return rc;
+  for (int k = 0; k < 8; ++k) ext[k] = b->hctl[i].ext[k];
+  return SWE_OK;
+}
+
 int swe_batch_set_chunk_rows(swe_batch *b, int rows, int nwaves) {
   if (!b || rows < MIN_CHUNK) return bfail(b, SWE_EINVAL, "chunk rows below minimum (8)");
   if (rows != b->chunk) b->dirty = true;

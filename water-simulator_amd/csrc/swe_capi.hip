@@ -20,6 +20,12 @@ static int fail(swe_ctx *c, int code, const std::string &msg) {
   return code;
 }
 
+// entry points that run kernels on the current state wait for a split step to end
+#define MIDCHK(c)                                                                                   \
+  do {                                                                                              \
+    if ((c)->mid) return fail((c), SWE_ESTATE, "a step is in progress (swe_step_begin without swe_step_end)"); \
+  } while (0)
+
 static void refresh_params(swe_ctx *c) {
   Params &P = c->P;
   P.friction = c->mfield ? 2 : ((c->manning != 0.0 && c->manning == c->manning) ? 1 : 0);
@@ -246,12 +252,15 @@ static int ensure_probe(swe_ctx *c) {
   return SWE_OK;
 }
 
+// the state the fields name: the current one, or mid-step the stage-1 state
+static int cur_buf(const swe_ctx *c) { return c->mid ? c->Bi : c->A; }
+
 static double *field_ptr(swe_ctx *c, int f) {
   switch (f) {
     case SWE_F_B: return c->b;
-    case SWE_F_H: return c->st[c->A][0];
-    case SWE_F_HU: return c->st[c->A][1];
-    case SWE_F_HV: return c->st[c->A][2];
+    case SWE_F_H: return c->st[cur_buf(c)][0];
+    case SWE_F_HU: return c->st[cur_buf(c)][1];
+    case SWE_F_HV: return c->st[cur_buf(c)][2];
     case SWE_F_MANNING: return c->mfield;
     case SWE_F_SPONGE: return c->sponge;
     case SWE_F_RH: return c->rh;
@@ -376,6 +385,7 @@ static int push_ctl(swe_ctx *c) {
 
 int swe_apply_bc(swe_ctx *c, const double ext[8]) {
   if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  MIDCHK(c);
   int rc = pull_ctl(c);
   if (rc) return rc;
   for (int i = 0; i < 8; ++i) c->hctl->ext[i] = ext ? ext[i] : 0.0;
@@ -400,6 +410,7 @@ static SweepArgs base_args(swe_ctx *c) {
 
 int swe_residual(swe_ctx *c, double dt) {
   if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  MIDCHK(c);
   int rc = ensure_probe(c);
   if (rc) return rc;
   if ((rc = pull_ctl(c))) return rc;
@@ -417,6 +428,7 @@ int swe_residual(swe_ctx *c, double dt) {
 
 int swe_apply_friction(swe_ctx *c, double dt) {
   if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  MIDCHK(c);
   if (c->P.friction == 0) return SWE_OK;                  // `if (!n) return;` (swe.mjs:689)
   StateW S = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
   HIPCHK(c, launch_friction(c->P, S, c->mfield, dt, c->stream));
@@ -426,6 +438,7 @@ int swe_apply_friction(swe_ctx *c, double dt) {
 
 int swe_dry_clean(swe_ctx *c, double *added) {
   if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  MIDCHK(c);
   int blocks = (c->P.nx * c->P.ny + 255) / 256;
   if (blocks > 2 * c->max_parts) blocks = 2 * c->max_parts;
   if (blocks < 1) blocks = 1;
@@ -442,6 +455,7 @@ int swe_dry_clean(swe_ctx *c, double *added) {
 
 int swe_max_dt(swe_ctx *c, double *out) {
   if (!c || !out) return fail(c, SWE_EINVAL, "null argument");
+  MIDCHK(c);
   int rc = pull_ctl(c);
   if (rc) return rc;
   const int saved = c->hctl->dt_mode;
@@ -457,12 +471,11 @@ int swe_max_dt(swe_ctx *c, double *out) {
   return push_ctl(c);
 }
 
-// one step on the device, stream-ordered; the caller handles the ctl block.
+// One step on the device, stream-ordered; the caller handles the ctl block.
 // applyBC before each stage (swe.mjs:784,798) writes the ghost ring of that
 // stage's input; a refused step still gets its applyBC, as in the reference.
-// rec: sample the gauge recorder after the step (device-resident runs)
-static int enqueue_step(swe_ctx *c, int want_theta, int want_rh, bool rec = false) {
-  const int np = sweep_waves(c->P, c->chunk);
+// Stage 1 (swe.mjs:784-796): applyBC on A, then A -> B.
+static int enqueue_stage1(swe_ctx *c, int np) {
   if (np > c->max_parts) return fail(c, SWE_EINVAL, "chunk too small");
   StateW SA = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
   StateW SB = {c->st[c->Bi][0], c->st[c->Bi][1], c->st[c->Bi][2]};
@@ -471,6 +484,15 @@ static int enqueue_step(swe_ctx *c, int want_theta, int want_rh, bool rec = fals
   a.in = {SA.h, SA.hu, SA.hv};
   a.out = SB;
   HIPCHK(c, launch_sweep(MODE_STAGE1, a, c->stream));
+  return SWE_OK;
+}
+
+// Stage 2 (swe.mjs:798-812, with friction, sponge and the next maxDt) and the
+// finalize: applyBC on B, then (A, B) -> C, and C becomes the current state.
+// rec: sample the gauge and projection recorders after the step (device-resident runs).
+static int enqueue_stage2(swe_ctx *c, int np, int want_theta, int want_rh, bool rec) {
+  StateW SA = {c->st[c->A][0], c->st[c->A][1], c->st[c->A][2]};
+  StateW SB = {c->st[c->Bi][0], c->st[c->Bi][1], c->st[c->Bi][2]};
   HIPCHK(c, launch_apply_bc(c->P, SB, c->b, c->dctl, c->stream));
   SweepArgs b2 = base_args(c);
   b2.in = {SB.h, SB.hu, SB.hv};
@@ -496,16 +518,21 @@ static int enqueue_step(swe_ctx *c, int want_theta, int want_rh, bool rec = fals
   return SWE_OK;
 }
 
-int swe_step(swe_ctx *c, double dt, const double ext[8], int want_theta, int want_rh,
-             double *dt_taken, double *mass_floored) {
-  if (!c) return fail(nullptr, SWE_EINVAL, "null context");
-  for (int s = 0; s < 4; ++s)
-    if (c->P.bc[s] == BC_STORED) return fail(c, SWE_EINVAL, "swe_step needs device boundary kinds on every side");
+static int enqueue_step(swe_ctx *c, int want_theta, int want_rh, bool rec = false) {
+  const int np = sweep_waves(c->P, c->chunk);
+  int rc = enqueue_stage1(c, np);
+  if (rc) return rc;
+  return enqueue_stage2(c, np, want_theta, want_rh, rec);
+}
+
+// Control block for a step taken by the host's call (swe_step, swe_step_begin):
+// the caller's Flather values, and dt from the argument or (NaN) from maxDt.
+static int begin_host_step(swe_ctx *c, double dt, const double ext[8]) {
   int rc = pull_ctl(c);
   if (rc) return rc;
   Ctl *h = c->hctl;
   for (int i = 0; i < 8; ++i) h->ext[i] = ext ? ext[i] : 0.0;
-  h->harm_on = 0;
+  h->harm_on = 0;          // this step uses the caller's values; swe_advance restores the harmonics
   h->cfl_bits = 0;
   const bool self = std::isnan(dt);
   h->dt_mode = self ? DT_SELF : DT_HOST;
@@ -517,13 +544,99 @@ int swe_step(swe_ctx *c, double dt, const double ext[8], int want_theta, int wan
     HIPCHK(c, launch_maxdt(c->P, S, c->dctl, c->stream));
     HIPCHK(c, launch_finalize(c->dctl, c->mf_part, 0, c->P.cfl, 0, c->stream));
   }
-  if ((rc = enqueue_step(c, want_theta, want_rh))) return rc;
-  if ((rc = pull_ctl(c))) return rc;
-  const double taken = h->dt_last;
-  if (dt_taken) *dt_taken = taken;
+  return SWE_OK;
+}
+
+static int end_host_step(swe_ctx *c, double *dt_taken, double *mass_floored) {
+  int rc = pull_ctl(c);
+  if (rc) return rc;
+  Ctl *h = c->hctl;
+  if (dt_taken) *dt_taken = h->dt_last;
   if (mass_floored) *mass_floored = h->mf;
   h->dt_mode = DT_HOST;
   return push_ctl(c);
+}
+
+int swe_step(swe_ctx *c, double dt, const double ext[8], int want_theta, int want_rh,
+             double *dt_taken, double *mass_floored) {
+  if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  MIDCHK(c);
+  for (int s = 0; s < 4; ++s)
+    if (c->P.bc[s] == BC_STORED) return fail(c, SWE_EINVAL, "swe_step needs device boundary kinds on every side");
+  int rc = begin_host_step(c, dt, ext);
+  if (rc) return rc;
+  if ((rc = enqueue_step(c, want_theta || c->keep_resid, want_rh || c->keep_resid))) return rc;
+  return end_host_step(c, dt_taken, mass_floored);
+}
+
+int swe_step_begin(swe_ctx *c, double dt, const double ext[8], double *dt_taken) {
+  if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  MIDCHK(c);
+  int rc = begin_host_step(c, dt, ext);
+  if (rc) return rc;
+  const int np = sweep_waves(c->P, c->chunk);
+  if ((rc = enqueue_stage1(c, np))) return rc;
+  if ((rc = pull_ctl(c))) return rc;
+  const double d = c->hctl->dt;
+  const bool ok = d > 0 && !std::isinf(d);
+  if (dt_taken) *dt_taken = ok ? d : 0.0;
+  if (!ok) {                  // refused (swe.mjs:784): the state stays as applyBC left it
+    c->hctl->dt_mode = DT_HOST;
+    c->hctl->dt_last = 0.0;
+    return push_ctl(c);
+  }
+  c->mid = 1;
+  c->mid_np = np;
+  return SWE_OK;
+}
+
+int swe_step_end(swe_ctx *c, const double ext[8], int want_theta, int want_rh, double *mass_floored) {
+  if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  if (!c->mid) return fail(c, SWE_ESTATE, "swe_step_end without swe_step_begin");
+  int rc = pull_ctl(c);
+  if (rc) return rc;
+  if (ext)
+    for (int i = 0; i < 8; ++i) c->hctl->ext[i] = ext[i];
+  if ((rc = push_ctl(c))) return rc;
+  c->mid = 0;
+  if ((rc = enqueue_stage2(c, c->mid_np, want_theta || c->keep_resid, want_rh || c->keep_resid, false))) return rc;
+  return end_host_step(c, nullptr, mass_floored);
+}
+
+int swe_keep_residual(swe_ctx *c, int on) {
+  if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  if (on) {
+    int rc = ensure_probe(c);
+    if (rc) return rc;
+  }
+  if ((on != 0) != (c->keep_resid != 0)) drop_graphs(c);
+  c->keep_resid = on != 0;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SWE_OK;
+}
+
+int swe_get_ext(swe_ctx *c, double ext[8]) {
+  if (!c || !ext) return fail(c, SWE_EINVAL, "null argument");
+  int rc = pull_ctl(c);
+  if (rc) return rc;
+  for (int i = 0; i < 8; ++i) ext[i] = c->hctl->ext[i];
+  return SWE_OK;
+}
+
+int swe_get_ghosts(swe_ctx *c, int f, double *host) {
+  if (!c || !host) return fail(c, SWE_EINVAL, "null argument");
+  if (f < SWE_F_B || f > SWE_F_HV) return fail(c, SWE_EINVAL, "ghost rings exist for b, h, hu, hv only");
+  const double *p = field_ptr(c, f);
+  const size_t W = c->P.W, H = c->P.H, row = W * sizeof(double);
+  // rows 0-1 and H-2..H-1 whole, then columns 0-1 and W-2..W-1 of rows 2..H-3
+  HIPCHK(c, hipMemcpyAsync(host, p, 2 * row, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(host + (H - 2) * W, p + (H - 2) * W, 2 * row, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpy2DAsync(host + 2 * W, row, p + 2 * W, row, 2 * sizeof(double), H - 4,
+                             hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpy2DAsync(host + 2 * W + W - 2, row, p + 2 * W + W - 2, row, 2 * sizeof(double), H - 4,
+                             hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SWE_OK;
 }
 
 int swe_set_tide(swe_ctx *c, int n, const int32_t *m, const double *amp, const double *omega, const double *phase,
@@ -569,6 +682,7 @@ int swe_set_harmonic(swe_ctx *c, int side, int n_eta, double msl, const double *
   for (int k = 0; k < n_eta; ++k) { hm.amp[k] = amp[k]; hm.omega[k] = omega[k]; hm.phase[k] = phase[k]; }
   for (int k = 0; k < n_u; ++k) { hm.uamp[k] = uamp[k]; hm.uomega[k] = uomega[k]; hm.uphase[k] = uphase[k]; }
   c->hctl->harm_on = 1;
+  c->harm_enabled = 1;
   rc = push_ctl(c);
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -599,6 +713,7 @@ int swe_use_graph(swe_ctx *c, int on) {
 
 int swe_set_chunk_rows(swe_ctx *c, int rows) {
   if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  MIDCHK(c);
   if (rows == 0) rows = default_chunk(c->P);
   if (rows < MIN_CHUNK) return fail(c, SWE_EINVAL, "chunk rows must be >= 1");
   if (rows != c->chunk) drop_graphs(c);
@@ -636,6 +751,7 @@ static int time_chunk(swe_ctx *c, int rows, int reps, float *ms) {
 
 int swe_autotune(swe_ctx *c, int reps, int *best_rows) {
   if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  MIDCHK(c);
   if (reps < 1) reps = 1;
   int rc = pull_ctl(c);
   if (rc) return rc;
@@ -692,7 +808,7 @@ static int build_graph(swe_ctx *c, int which) {
   // which = parity of the buffer rotation (A index 0 or 2)
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   const int saveA = c->A, saveC = c->C;
-  int rc = enqueue_step(c, 0, 0, true);
+  int rc = enqueue_step(c, c->keep_resid, c->keep_resid, true);
   c->A = saveA; c->C = saveC;
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(c->stream, &g);
@@ -705,6 +821,7 @@ static int build_graph(swe_ctx *c, int which) {
 
 int swe_advance(swe_ctx *c, int64_t nsteps, int dt_mode, double dt_value, double t_end, int64_t *taken) {
   if (!c) return fail(nullptr, SWE_EINVAL, "null context");
+  MIDCHK(c);
   if (nsteps < 0 || dt_mode < SWE_DT_SELF || dt_mode > SWE_DT_SAMPLED) return fail(c, SWE_EINVAL, "bad advance args");
   for (int s = 0; s < 4; ++s)
     if (c->P.bc[s] == BC_STORED) return fail(c, SWE_EINVAL, "swe_advance needs device boundary kinds on every side");
@@ -714,6 +831,7 @@ int swe_advance(swe_ctx *c, int64_t nsteps, int dt_mode, double dt_value, double
   if (dt_mode == SWE_DT_SAMPLED && (c->rec_n == 0 || h->rec_every != 0))
     return fail(c, SWE_EINVAL, "SWE_DT_SAMPLED needs a recorder armed with sample times (every = 0)");
   const long long steps0 = h->steps;
+  h->harm_on = c->harm_enabled;
   h->dt_mode = dt_mode == SWE_DT_SELF    ? DT_SELF
                : dt_mode == SWE_DT_FIXED ? DT_FIXED
                : dt_mode == SWE_DT_UNTIL ? DT_UNTIL
@@ -740,7 +858,7 @@ int swe_advance(swe_ctx *c, int64_t nsteps, int dt_mode, double dt_value, double
       HIPCHK(c, hipGraphLaunch(c->gexec[which], c->stream));
       std::swap(c->A, c->C);
     } else {
-      if ((rc = enqueue_step(c, 0, 0, true))) return rc;
+      if ((rc = enqueue_step(c, c->keep_resid, c->keep_resid, true))) return rc;
     }
   }
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -755,6 +873,7 @@ int swe_advance(swe_ctx *c, int64_t nsteps, int dt_mode, double dt_value, double
 
 int swe_profile_step(swe_ctx *c, int nsteps, double out_ms[3]) {
   if (!c || !out_ms || nsteps < 1) return fail(c, SWE_EINVAL, "bad profile args");
+  MIDCHK(c);
   for (int s = 0; s < 4; ++s)
     if (c->P.bc[s] == BC_STORED) return fail(c, SWE_EINVAL, "profiling needs device boundary kinds");
   int rc = pull_ctl(c);

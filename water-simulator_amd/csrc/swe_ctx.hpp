@@ -76,6 +76,15 @@ struct swe_ctx {
   double *diag = nullptr;
   int diag_blocks = 0;
   double diag_ms = 0.0;       // duration of the last swe_diagnostics' kernels
+  // Flather harmonics set by swe_set_harmonic: device-resident runs evaluate them
+  // every step; swe_step uses the caller's values for its own step only
+  int harm_enabled = 0;
+  // a step split around a host ghost fill (swe_step_begin .. swe_step_end) is in
+  // progress: the current fields are the stage-1 state in buffer Bi, and the
+  // stage kernels' massFloored partials of stage 1 (mid_np of them) wait for stage 2
+  int mid = 0, mid_np = 0;
+  // every step also leaves stage 2's theta and residual in the probe fields
+  int keep_resid = 0;
   std::string err;
 };
 

@@ -218,7 +218,7 @@ __global__ __launch_bounds__(DG_THREADS) void k_diag_final(DiagArgs a) {
       const bool in = q < n;
       se[q] = in ? a.row_e[j0 + q] : -0.0;
       sv[q] = in ? a.row_v[j0 + q] : -0.0;
-      sa[q] = in ? a.area[j0 + q + a.ng] : 1.0;
+      sa[q] = (in && a.sphere) ? a.area[j0 + q + a.ng] : 1.0;    // row areas: the sphere only
     }
     __syncthreads();
     if (t == 0 || (t == 64 && a.sphere)) {
@@ -351,6 +351,7 @@ using namespace swe;
 
 extern "C" int swe_diagnostics(swe_ctx *c, double eta_ref, double out[SWE_D_COUNT]) {
   if (!c || !out) return ctx_fail(c, SWE_EINVAL, "null argument");
+  if (c->mid) return ctx_fail(c, SWE_ESTATE, "a step is in progress (swe_step_begin without swe_step_end)");
   const int nx = c->P.nx, ny = c->P.ny, ng = 2;
   const int nblocks = (ny + DG_ROWS - 1) / DG_ROWS;
   // layout: row_v[ny], row_e[ny], blk[2 nblocks], acc[3], out[DG_OUT]
@@ -384,3 +385,4 @@ extern "C" int swe_diagnostics(swe_ctx *c, double eta_ref, double out[SWE_D_COUN
   std::memcpy(out, res, sizeof(double) * SWE_D_COUNT);
   return SWE_OK;
 }
+

@@ -383,12 +383,15 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
       if (MODE == MODE_PROBE) {
         A.rh[k] = rh; A.rh[k + fs] = rhu; A.rh[k + 2 * fs] = rhv; A.theta[k] = thPrev;
       } else if (MODE == MODE_STAGE1) {
-        if (interior) {                                   // swe.mjs:790-796
-          double nh = cur.h + dt * rh, nhu = cur.hu + dt * rhu, nhv = cur.hv + dt * rhv;
+        // swe.mjs:790-796: every cell, ghosts too (the next applyBC rewrites the
+        // ghosts; a caller's own boundary function sees them as the reference
+        // leaves them), then dryClean of the interior
+        double nh = cur.h + dt * rh, nhu = cur.hu + dt * rhu, nhv = cur.hv + dt * rhv;
+        if (interior) {
           if (nh < 0) { mf -= nh; nh = 0.0; }
           if (nh <= md) { nhu = 0.0; nhv = 0.0; }
-          outp[k] = nh; outp[k + fs] = nhu; outp[k + 2 * fs] = nhv;
         }
+        outp[k] = nh; outp[k + fs] = nhu; outp[k + 2 * fs] = nhv;
       } else {                                            // stage 2, swe.mjs:799-812
         const double h0 = h0p[k], hu0 = h0p[k + fs], hv0 = h0p[k + 2 * fs];
         double nh = 0.5 * (h0 + cur.h + dt * rh);

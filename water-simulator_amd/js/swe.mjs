@@ -6,14 +6,22 @@
 //
 // Coherence: callers read and write sim.h, sim.hu, sim.hv and sim.b between
 // calls, so every device call uploads the host arrays it needs first and
-// downloads what it changed afterwards (strict mode).  advance(n) runs n steps
-// device-resident for throughput work and downloads once at the end.
+// downloads what it changed afterwards (strict mode): a step uploads b, h, hu,
+// hv and downloads h, hu, hv and the ghost ring of b (the only part of b a step
+// changes).  sim.rh, sim.rhu, sim.rhv and sim.theta hold what the reference
+// leaves in them (the last residual: stage 2's after a step) and are downloaded
+// when first read.  advance(n) runs n steps device-resident for throughput work
+// and downloads once at the end.
 //
 // Boundaries are recognised by identity (reflect, outflow, periodic exported
 // here) or by the tag a flather() closure carries; the flather closure's
 // etaExt(t)/uExt(t) are evaluated here on the host once per step at sim.t and
 // handed to the device, exactly the values the reference's per-ghost-cell calls
-// produce (t does not move inside a step).  A makeSponge() forcing is recognised
+// produce (t does not move inside a step).  Any other boundary function runs
+// on the host, as the reference runs it: applyBC's loop over the host arrays
+// before each stage, the step split around the second one.  Device-resident
+// runs take a flather side's values from harmonicFlather()'s description,
+// evaluated on the device every step.  A makeSponge() forcing is recognised
 // the same way and runs inside the step on the device; any other forcing
 // callback runs on the host after the step, where the reference calls it.
 
@@ -29,7 +37,9 @@ const F = { B: 0, H: 1, HU: 2, HV: 3, MANNING: 4, RH: 5, RHU: 6, RHV: 7, THETA: 
 const S = { MANNING: 0, CORIOLIS: 1, CFL: 2, T: 3, SPONGE_REF: 4, STEPS: 5 };
 const ROWS = ['dxRow', 'dyRowN', 'dyRowS', 'dyCFL', 'bedPhi', 'tanPhi', 'fRow', 'geoCoef', 'area'];
 const LIMITERS = { minmod: 0, mc: 1, central: 2 };
-const BC = { REFLECT: 0, OUTFLOW: 1, PERIODIC: 2, FLATHER: 3 };
+const BC = { REFLECT: 0, OUTFLOW: 1, PERIODIC: 2, FLATHER: 3, STORED: 4 };
+const NO_EXT = new Float64Array(8);
+const RESIDUAL = ['rh', 'rhu', 'rhv', 'theta'];
 const SIDES = ['west', 'east', 'south', 'north'];
 const FLATHER_TAG = Symbol('swe.flather');
 const SPONGE_TAG = Symbol('swe.sponge');
@@ -63,8 +73,16 @@ export class ShallowWater {
     this.h = f64(N);
     this.hu = f64(N);
     this.hv = f64(N);
-    this.rh = f64(N); this.rhu = f64(N); this.rhv = f64(N);
-    this.theta = f64(N).fill(1);
+    // rh, rhu, rhv, theta: the last residual's, fetched from the device on first read
+    this._resid = { rh: f64(N), rhu: f64(N), rhv: f64(N), theta: f64(N).fill(1) };
+    this._residStale = false;
+    for (const name of RESIDUAL) {
+      Object.defineProperty(this, name, {
+        enumerable: true,
+        get() { if (this._residStale) this._pullResidual(); return this._resid[name]; },
+        set(v) { this._resid[name] = v; },
+      });
+    }
     for (let j = 0; j < this.H; j++) {
       for (let i = 0; i < this.W; i++) {
         const k = j * this.W + i;
@@ -93,6 +111,8 @@ export class ShallowWater {
     this._rowsSent = ROWS.map((r) => Float64Array.from(this.geom[r]));
     this._spongeSent = null;
     this._mfieldSent = null;
+    // every device step leaves stage 2's residual and theta on the device
+    addon.keepResidual(this._dev, true);
   }
 
   cellCentre(i, j) {
@@ -141,46 +161,99 @@ export class ShallowWater {
   }
 
   _upload(fields) { for (const f of fields) addon.setField(this._dev, F[f], this[f.toLowerCase()]); }
-  _download(fields) {
-    const map = { B: 'b', H: 'h', HU: 'hu', HV: 'hv', RH: 'rh', RHU: 'rhu', RHV: 'rhv', THETA: 'theta' };
-    for (const f of fields) addon.getField(this._dev, F[f], this[map[f]]);
+  _download(fields) { for (const f of fields) addon.getField(this._dev, F[f], this[f.toLowerCase()]); }
+  // after a device step or applyBC: h, hu, hv whole, and of b only the ghost ring
+  _downloadState() {
+    this._download(['H', 'HU', 'HV']);
+    addon.getGhosts(this._dev, F.B, this.b);
   }
 
-  // Boundary kinds and the Flather values at the current t; throws for a
-  // boundary function the device does not implement.
+  _pullResidual() {
+    this._residStale = false;
+    for (const name of RESIDUAL) addon.getField(this._dev, F[name.toUpperCase()], this._resid[name]);
+  }
+
+  // Boundary kinds and the Flather values at the current t.  Returns null when a
+  // side is a function the device does not implement: then every side is filled
+  // on the host (_hostApplyBC) and the device keeps the ghosts it is given.
   _boundaries() {
     const ext = new Float64Array(8);
+    const kinds = [];
     for (let s = 0; s < 4; s++) {
       const fn = this.boundaries[SIDES[s]];
-      let kind;
+      let kind = null;
       if (fn === reflect) kind = BC.REFLECT;
       else if (fn === outflow) kind = BC.OUTFLOW;
       else if (fn === periodic) kind = BC.PERIODIC;
-      else if (fn && fn[FLATHER_TAG]) {
-        kind = BC.FLATHER;
-        ext[2 * s] = fn[FLATHER_TAG].etaExt(this.t);
-        ext[2 * s + 1] = fn[FLATHER_TAG].uExt(this.t);
-      } else {
-        throw new Error(`boundary on ${SIDES[s]} is not one of reflect/outflow/periodic/flather; `
-          + 'the GPU backend runs only those');
-      }
-      addon.setBc(this._dev, s, kind);
+      else if (fn && fn[FLATHER_TAG]) kind = BC.FLATHER;
+      kinds.push(kind);
     }
-    return ext;
+    const host = kinds.includes(null);
+    for (let s = 0; s < 4; s++) {
+      addon.setBc(this._dev, s, host ? BC.STORED : kinds[s]);
+      if (!host && kinds[s] === BC.FLATHER) {
+        const tag = this.boundaries[SIDES[s]][FLATHER_TAG];
+        ext[2 * s] = tag.etaExt(this.t);
+        ext[2 * s + 1] = tag.uExt(this.t);
+      }
+    }
+    return host ? null : ext;
+  }
+
+  // applyBC's two passes (swe.mjs:289-303) over the host arrays with the
+  // boundary functions themselves: the west and east ghost columns over every
+  // row, then the south and north ghost rows over every column, corners included
+  _hostApplyBC() {
+    const { H, ng, nx, ny } = this;
+    const bc = this.boundaries;
+    for (let j = 0; j < H; j++) {
+      for (let g = 0; g < ng; g++) {
+        bc.west(this, -1 - g, j - ng, 0, g);
+        bc.east(this, nx + g, j - ng, 1, g);
+      }
+    }
+    for (let i = -ng; i < nx + ng; i++) {
+      for (let g = 0; g < ng; g++) {
+        bc.south(this, i, -1 - g, 2, g);
+        bc.north(this, i, ny + g, 3, g);
+      }
+    }
+  }
+
+  // device-resident runs: every side a device kind, a flather side described by harmonicFlather()
+  _deviceBoundaries(what) {
+    if (!this._boundaries()) {
+      throw new Error(`${what} runs device-resident: every boundary must be reflect, outflow, periodic or flather`);
+    }
+    for (let s = 0; s < 4; s++) {
+      const tag = this.boundaries[SIDES[s]][FLATHER_TAG];
+      if (!tag) continue;
+      const hm = tag.harmonic;
+      if (!hm) {
+        throw new Error(`${what} runs device-resident: the flather boundary on ${SIDES[s]} needs its forcing `
+          + 'described with harmonicFlather() (a flather(etaExt, uExt) closure is evaluated by step() only)');
+      }
+      const col = (terms, k) => Float64Array.from(terms.map((x) => (k === 'phase' ? (x.phase || 0) : x[k])));
+      const uTerms = hm.uScale == null ? hm.u : [];
+      addon.setHarmonic(this._dev, s, hm.msl, col(hm.eta, 'amp'), col(hm.eta, 'omega'), col(hm.eta, 'phase'),
+        hm.u0, col(uTerms, 'amp'), col(uTerms, 'omega'), col(uTerms, 'phase'));
+      addon.setHarmonicRamp(this._dev, s, hm.ramp, hm.uScale != null, hm.uScale == null ? 0 : hm.uScale);
+    }
   }
 
   applyBC() {
     const ext = this._boundaries();
+    if (!ext) { this._hostApplyBC(); return; }
     this._upload(['B', 'H', 'HU', 'HV']);
     addon.applyBc(this._dev, ext);
-    this._download(['B', 'H', 'HU', 'HV']);
+    for (const f of ['B', 'H', 'HU', 'HV']) addon.getGhosts(this._dev, F[f], this[f.toLowerCase()]);
   }
 
   residual(dt) {
     this._syncParams();
     this._upload(['B', 'H', 'HU', 'HV']);
     addon.residual(this._dev, dt);
-    this._download(['RH', 'RHU', 'RHV', 'THETA']);
+    this._residStale = true;
   }
 
   maxDt() {
@@ -236,10 +309,28 @@ export class ShallowWater {
       this._spongeSent = null;
     }
     addon.setScalar(this._dev, S.T, this.t);
-    this._upload(['B', 'H', 'HU', 'HV']);
-    const [taken, mf] = addon.step(this._dev, dt == null ? NaN : dt, ext, true, false);
-    this._download(['B', 'H', 'HU', 'HV', 'THETA']);
+    const dtArg = dt == null ? NaN : dt;
+    let taken, mf;
+    if (ext) {
+      this._upload(['B', 'H', 'HU', 'HV']);
+      [taken, mf] = addon.step(this._dev, dtArg, ext, true, true);
+      this._downloadState();
+    } else {
+      // a boundary function of the caller's: applyBC on the host before each
+      // stage (swe.mjs:784,798), the device step split between them
+      this._hostApplyBC();
+      this._upload(['B', 'H', 'HU', 'HV']);
+      taken = addon.stepBegin(this._dev, dtArg, NO_EXT);
+      if (taken > 0) {
+        this._download(['H', 'HU', 'HV']);
+        this._hostApplyBC();
+        this._upload(['B', 'H', 'HU', 'HV']);
+        mf = addon.stepEnd(this._dev, true, true);
+        this._download(['H', 'HU', 'HV']);
+      }
+    }
     if (!(taken > 0)) return 0;
+    this._residStale = true;
     this.massFloored = mf;
     if (this.forcing && !sp) this.forcing(this, taken);
     this.t += taken;
@@ -249,13 +340,14 @@ export class ShallowWater {
 
   /** n self-timed steps device-resident (no per-step host round trip), then one download. */
   advance(n) {
-    this._boundaries();
+    this._deviceBoundaries('advance(n)');
     this._syncParams();
     if (this.forcing) throw new Error('advance(n) runs without a host forcing hook');
     addon.setScalar(this._dev, S.T, this.t);
     this._upload(['B', 'H', 'HU', 'HV']);
     const taken = addon.advance(this._dev, n, 0, 0, Infinity);
-    this._download(['B', 'H', 'HU', 'HV']);
+    this._downloadState();
+    if (taken > 0) this._residStale = true;
     this.t = addon.getScalar(this._dev, S.T);
     this.steps += taken;
     return taken;
@@ -288,7 +380,7 @@ export class ShallowWater {
    * leaves the state as the host loop would.
    */
   recordRun(cells, { t0 = this.t, dtRec, until, capacity = 0 }) {
-    this._boundaries();
+    this._deviceBoundaries('recordRun');
     this._syncParams();
     if (this.forcing) throw new Error('recordRun runs without a host forcing hook');
     const c = Int32Array.from(cells);
@@ -301,7 +393,8 @@ export class ShallowWater {
       const n = addon.getRecord(this._dev, null, null);
       const eta = new Float64Array(n * c.length), times = new Float64Array(n);
       addon.getRecord(this._dev, eta, times);
-      this._download(['B', 'H', 'HU', 'HV']);
+      this._downloadState();
+      if (taken > 0) this._residStale = true;
       this.t = addon.getScalar(this._dev, S.T);
       this.steps += taken;
       return { times, eta };
@@ -317,7 +410,7 @@ export class ShallowWater {
    * with no counterpart in swe.mjs.  coef: ny values.  Returns { times, vals }.
    */
   projectedRun(coef, until, capacity) {
-    this._boundaries();
+    this._deviceBoundaries('projectedRun');
     this._syncParams();
     if (this.forcing) throw new Error('projectedRun runs without a host forcing hook');
     addon.setScalar(this._dev, S.T, this.t);
@@ -328,7 +421,8 @@ export class ShallowWater {
       const n = addon.getProjection(this._dev, null, null);
       const vals = new Float64Array(n), times = new Float64Array(n);
       addon.getProjection(this._dev, vals, times);
-      this._download(['B', 'H', 'HU', 'HV']);
+      this._downloadState();
+      if (taken > 0) this._residStale = true;
       this.t = addon.getScalar(this._dev, S.T);
       this.steps += taken;
       return { times, vals };
@@ -455,7 +549,7 @@ export class Adjoint {
 
   _prepare() {
     const sim = this.sim;
-    sim._boundaries();
+    if (!sim._boundaries()) throw new Error('Adjoint: every boundary must be reflect, outflow or periodic');
     sim._syncParams();
     sim._upload(['B']);
   }
@@ -551,6 +645,32 @@ export function flather(etaExt, uExt = () => 0) {
     else { sim.hv[g] = sim.h[g] * un; sim.hu[g] = sim.hu[s]; }
   };
   fn[FLATHER_TAG] = { etaExt, uExt };
+  return fn;
+}
+
+/**
+ * A flather() boundary whose far field is a sum of at most four cosines (no
+ * counterpart in src/swe.mjs; Tide.eta of src/tide.mjs:103-107 and regularWave
+ * of src/waves.mjs:163-175 are such sums):
+ *   etaExt(t) = msl + sum_k a_k cos(omega_k t - phase_k),
+ *   uExt(t)   = u0 + the same over `u`, or uScale * (etaExt(t) - msl),
+ * with a_k = amp_k, or amp_k * min(1, t / ramp) when ramp > 0.  step()
+ * evaluates the closures on the host like any flather(); advance(),
+ * recordRun() and projectedRun() evaluate the same sums on the device at each
+ * step's t (the device cos may differ from Math.cos in the last bit).
+ */
+export function harmonicFlather({ msl = 0, eta = [], u0 = 0, u = [], ramp = 0, uScale = null } = {}) {
+  if (eta.length > 4 || u.length > 4) throw new Error('harmonicFlather: at most 4 terms each for eta and u');
+  const sum = (c0, terms, t) => {
+    let e = c0;
+    const m = ramp > 0 ? Math.min(1, t / ramp) : 1;
+    for (const x of terms) e += (ramp > 0 ? m * x.amp : x.amp) * Math.cos(x.omega * t - (x.phase || 0));
+    return e;
+  };
+  const etaExt = (t) => sum(msl, eta, t);
+  const uExt = uScale == null ? (t) => sum(u0, u, t) : (t) => uScale * (etaExt(t) - msl);
+  const fn = flather(etaExt, uExt);
+  fn[FLATHER_TAG].harmonic = { msl, eta, u0, u, ramp, uScale };
   return fn;
 }
 

@@ -86,6 +86,11 @@ def lib():
             "swe_get_record": (i, [vp, dp, dp, ctypes.POINTER(i64)]),
             "swe_set_projection": (i, [vp, dp, i64]),
             "swe_get_projection": (i, [vp, dp, dp, ctypes.POINTER(i64)]),
+            "swe_step_begin": (i, [vp, d, dp, dp]),
+            "swe_step_end": (i, [vp, dp, i, i, dp]),
+            "swe_keep_residual": (i, [vp, i]),
+            "swe_get_ext": (i, [vp, dp]),
+            "swe_get_ghosts": (i, [vp, i, dp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -248,6 +253,40 @@ class Solver:
                                 int(want_rh), ctypes.byref(taken), ctypes.byref(mf)), "swe_step")
         self._mf = mf.value
         return taken.value
+
+    def step_begin(self, dt=None, ext=None):
+        """First half of step(): dt and stage 1; the fields then hold the stage-1
+        state until step_end() (for a host ghost fill in between).  Returns dt (0:
+        refused, no step_end follows)."""
+        e = _ext(ext)
+        taken = ctypes.c_double()
+        self._ck(lib().swe_step_begin(self._h, math.nan if dt is None else float(dt), _dp(e), ctypes.byref(taken)),
+                 "swe_step_begin")
+        return taken.value
+
+    def step_end(self, ext=None, want_theta=False, want_rh=False):
+        """Second half of step(): the second applyBC, stage 2 and the epilogue."""
+        e = None if ext is None else _ext(ext)
+        mf = ctypes.c_double()
+        self._ck(lib().swe_step_end(self._h, None if e is None else _dp(e), int(want_theta), int(want_rh),
+                                    ctypes.byref(mf)), "swe_step_end")
+        self._mf = mf.value
+        return mf.value
+
+    def keep_residual(self, on=True):
+        """Every later step also leaves stage 2's theta and rh/rhu/rhv in the probe fields."""
+        self._ck(lib().swe_keep_residual(self._h, int(on)), "swe_keep_residual")
+
+    def ext(self):
+        """The Flather (eta_ext, u_ext) per side the next step uses (device-evaluated in advance())."""
+        out = np.zeros(8)
+        self._ck(lib().swe_get_ext(self._h, _dp(out)), "swe_get_ext")
+        return out
+
+    def get_ghosts(self, f, into):
+        """Copy the ghost ring of field f into the W*H array `into` (interior untouched)."""
+        assert into.dtype == np.float64 and into.size == self.N and into.flags.c_contiguous
+        self._ck(lib().swe_get_ghosts(self._h, f, _dp(into)), "swe_get_ghosts")
 
     def sponge(self, dt, ref):
         """The sponge forcing runs inside step() on the device (set_sponge)."""
@@ -553,6 +592,7 @@ class Batch:
             "swe_batch_set_bc": (i, [vp, i, i, i]),
             "swe_batch_set_harmonic": (i, [vp, i, i, i, d, dp, dp, dp, i, d, dp, dp, dp]),
             "swe_batch_get_scalar": (i, [vp, i, i, dp]),
+            "swe_batch_get_ext": (i, [vp, i, dp]),
             "swe_batch_set_chunk_rows": (i, [vp, i, i]),
             "swe_batch_use_graph": (i, [vp, i]),
             "swe_batch_advance": (i, [vp, i64, ctypes.POINTER(i64)]),
@@ -624,6 +664,12 @@ class Batch:
         v = ctypes.c_double()
         self._ck(lib().swe_batch_get_scalar(self._h, basin, which, ctypes.byref(v)), "swe_batch_get_scalar")
         return v.value
+
+    def ext(self, basin):
+        """The Flather (eta_ext, u_ext) per side basin `basin`'s next step uses."""
+        out = np.zeros(8)
+        self._ck(lib().swe_batch_get_ext(self._h, basin, _dp(out)), "swe_batch_get_ext")
+        return out
 
     def set_chunk_rows(self, rows, nwaves=0):
         self._ck(lib().swe_batch_set_chunk_rows(self._h, int(rows), int(nwaves)), "swe_batch_set_chunk_rows")

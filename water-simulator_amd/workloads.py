@@ -179,12 +179,15 @@ def inversion(nx=1024, ny=1024, dx=50.0, seed=4, ngauges=32):
     return b.ravel(), h.ravel(), params, np.array(gauges, dtype=np.int32)
 
 
-def inversion_directions(n, nx, ny, dx, wet, seed=46):
+def inversion_directions(n, nx, ny, dx, wet, seed=46, near=None, reach=2000.0):
     """n seeded random directions (dh, dn) of the inversion's control space, for
     finite-difference checks of its gradient: dh is 1 cm times a sum of 4 Gaussian
     humps of the surface (amplitude U(-1,1), sigma U(1,5) km, like the truth), dn is
     1e-3 times a low Fourier mode of the Manning field (1 to 6 waves per side, random
-    phase); both vanish off the cells where wet (a bool array over W*H) is set."""
+    phase); both vanish off the cells where wet (a bool array over W*H) is set.
+    Hump centres are uniform over the domain, or, with `near` (cell indices k =
+    j*W + i, e.g. the gauges), within `reach` metres of a randomly chosen one of
+    those cells: inside the region a short run's gauges can see."""
     rng = SplitMix64(seed)
     W, H = nx + 4, ny + 4
     x = (np.arange(W) - 2 + 0.5) * dx
@@ -194,7 +197,12 @@ def inversion_directions(n, nx, ny, dx, wet, seed=46):
     for _ in range(n):
         dh = np.zeros((H, W))
         for _ in range(4):
-            cx, cy = rng.uniform(0, nx * dx), rng.uniform(0, ny * dx)
+            if near is None:
+                cx, cy = rng.uniform(0, nx * dx), rng.uniform(0, ny * dx)
+            else:
+                k = int(near[min(len(near) - 1, int(rng.uniform(0, len(near))))])
+                cx = (k % W - 2 + 0.5) * dx + rng.uniform(-reach, reach)
+                cy = (k // W - 2 + 0.5) * dx + rng.uniform(-reach, reach)
             a, sg = rng.uniform(-1, 1), rng.uniform(1000, 5000)
             dh += 0.01 * a * np.outer(np.exp(-((y - cy) / sg) ** 2), np.exp(-((x - cx) / sg) ** 2))
         kx, ky, ph = rng.uniform(1, 6), rng.uniform(1, 6), rng.uniform(0, 2 * math.pi)

@@ -148,11 +148,10 @@ def adjoint_measure(nsteps, every=10, fd_dirs=0):
     rows = swe_amd.geometry_cartesian(nx, ny, dx, coriolis=p["coriolis"])
     s = swe_amd.Solver(nx, ny, rows, manning=p["manning"], coriolis=p["coriolis"], cfl=p["cfl"], dx=dx, dy=dx)
     z = np.zeros_like(ht)
-    eta_t = np.where(ht > 0, b + ht, 0.0)
-    # first guess: half the truth surface plus 1 cm of seeded noise (the noise keeps
-    # every surface difference far from the limiters' kinks, which finite
-    # differences need)
-    h_eval = np.maximum(0.0, 0.5 * eta_t + 0.01 * workloads.SplitMix64(45).normals(W * H) - b)
+    # first guess: half the truth surface plus 5 cm of smooth seeded noise (the
+    # noise keeps the surface differences away from the limiters' kinks, which
+    # finite differences need)
+    h_eval = workloads.inversion_first_guess(b, ht, nx, ny)
     X, Y = np.meshgrid(np.arange(W), np.arange(H))
     n_truth = (0.025 * (1 + 0.3 * np.sin(X / 97.0) * np.cos(Y / 131.0))).ravel()
     n0 = np.full(W * H, 0.025)

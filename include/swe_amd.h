@@ -269,7 +269,9 @@ int swe_batch_autotune(swe_batch *b, int steps, int *best_rows, int *best_waves)
  * Loss: J = sum over samples s and gauges g of (eta_g(t_s) - obs[s][g])^2, eta
  * sampled after every `every` steps.  Gradients: dJ/dh0 (initial depth, zero
  * momentum; dJ/d eta0 on wet cells) and dJ/dn (per-cell Manning n, interior).
- * Boundaries: reflect, outflow, periodic; no sponge.  The adjoint drives the
+ * Boundaries: reflect, outflow, periodic and flather (its far-field values as the
+ * forward used them: swe_set_harmonic's, evaluated on the device, or the last
+ * ones given); the makeSponge forcing is differentiated too.  The adjoint drives the
  * context it is created on: its state after any swe_adj_* call is that of the
  * last forward step run, and a scalar Manning n becomes a per-cell field. */
 typedef struct swe_adj swe_adj;
@@ -303,16 +305,21 @@ int swe_adj_get_dt(swe_adj *a, double *dt);        /* nsteps recorded dt */
 int swe_adj_timing(swe_adj *a, double *fwd_ms, double *bwd_ms);
 /* The reverse passes run on host arrays (no GPU needed), for checking them:
  * lh += (d residual / d state)^T lr for residual(dt) of a state with stored
- * ghosts (swe.mjs:308-629); BC^T in place (reflect/outflow/periodic,
- * swe.mjs:289-303); the adjoint of the pre-dryClean state of a stage's end
- * (stage 1: dryClean; stage 2: dryClean then friction with Manning field nfield
- * or the scalar, accumulating dJ/dn into lN); hllc^T (swe.mjs:974-1011). */
+ * ghosts (swe.mjs:308-629); BC^T in place (reflect/outflow/periodic/flather,
+ * swe.mjs:289-303, :1084-1116: a flather side differentiates at the state
+ * h, hu, hv, b applyBC read with far-field values ext, which may be NULL
+ * without one); the adjoint of the pre-dryClean state of a stage's end (stage 1:
+ * dryClean; stage 2: dryClean, then friction with Manning field nfield or the
+ * scalar, accumulating dJ/dn into lN, then the sponge relaxation with per-cell
+ * strengths `sponge` or none, swe.mjs:1188-1197); hllc^T (swe.mjs:974-1011). */
 int swe_adj_host_residual_rev(const swe_desc *d, const double *h, const double *hu, const double *hv,
                               const double *b, double dt, const double *lrh, const double *lrhu,
                               const double *lrhv, double *lh, double *lhu, double *lhv);
-int swe_adj_host_bc_rev(const swe_desc *d, const int32_t bc[4], double *lh, double *lhu, double *lhv);
+int swe_adj_host_bc_rev(const swe_desc *d, const int32_t bc[4], const double ext[8], const double *h,
+                        const double *hu, const double *hv, const double *b, double *lh, double *lhu, double *lhv);
 int swe_adj_host_epilogue_rev(const swe_desc *d, int stage, const double *pre_h, const double *pre_hu,
-                              const double *pre_hv, const double *nfield, double dt, const double *lout_h,
+                              const double *pre_hv, const double *nfield, const double *sponge, double dt,
+                              const double *lout_h,
                               const double *lout_hu, const double *lout_hv, double *lpre_h, double *lpre_hu,
                               double *lpre_hv, double *lN);
 void swe_adj_host_hllc_rev(double out[6], double hL, double huL, double hvL, double hR, double huR, double hvR,

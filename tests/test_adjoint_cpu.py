@@ -183,10 +183,63 @@ def test_bc_rev_is_the_transpose(kinds, shape):
         assert not g.any()
 
 
-# ---- dryClean^T / friction^T ---------------------------------------------------------
-@pytest.mark.parametrize("stage", [1, 2])
+FLATHER_SETS = [
+    ("flather", "reflect", "periodic", "periodic"),
+    ("flather", "flather", "flather", "flather"),
+    ("reflect", "flather", "outflow", "flather"),
+]
+
+
+@pytest.mark.parametrize("kinds", FLATHER_SETS, ids=["-".join(k) for k in FLATHER_SETS])
+@pytest.mark.parametrize("shape", [(9, 7), (3, 5)])
+def test_flather_bc_rev_matches_fd(kinds, shape):
+    """Flather's ghost fill (swe.mjs:1084-1116) is nonlinear in its source cell:
+    its transpose against central differences of the oracle's applyBC, with a
+    dry source cell on every side (its ghost depends on the far field only)."""
+    nx, ny = shape
+    rows = swe_amd.geometry_cartesian(nx, ny, 2.0)
+    o = oracle.OracleSim(nx, ny, rows, dx=2.0, dy=2.0, manning=0.0)
+    desc, keep = swe_amd.make_desc(nx, ny, rows, manning=0.0)
+    rng = np.random.default_rng(3 + sum(shape))
+    W, H = nx + 4, ny + 4
+    N = W * H
+    b = -1.5 + 0.2 * rng.normal(size=N)
+    h = np.maximum(0.0, 0.1 * rng.normal(size=N) - b)
+    hu = 0.3 * rng.normal(size=N) * h
+    hv = 0.3 * rng.normal(size=N) * h
+    # a dry corner of the interior, next to the west and south sides
+    dry = np.zeros((H, W), bool)
+    dry[2, 2] = True
+    h[dry.ravel()] = 0.0
+    hu[dry.ravel()] = hv[dry.ravel()] = 0.0
+    ext = np.array([0.12, 0.03, -0.05, 0.02, 0.07, -0.04, 0.02, 0.05])
+    o.set_bc(kinds)
+
+    def F(v):
+        o.load_state(b, v[:N], v[N:2 * N], v[2 * N:])
+        o.apply_bc(ext)
+        return np.concatenate([o.h.copy(), o.hu.copy(), o.hv.copy()])
+
+    inter = np.zeros((H, W), bool)
+    inter[2:-2, 2:-2] = True
+    inter = inter.ravel()
+    lam = [rng.normal(size=N) for _ in range(3)]
+    d = np.concatenate([rng.normal(size=N) * inter * sc for sc in (0.05, 0.02, 0.02)])
+    x = np.concatenate([h, hu, hv])
+    fd = vjp_fd(F, x, np.concatenate(lam), d, 1e-7)
+    lt = swe_amd.bc_rev_host(desc, kinds, *lam, ext=ext, state=(b, h, hu, hv))
+    ad = float(np.dot(np.concatenate(lt), d))
+    assert abs(ad - fd) <= 1e-7 * sum(float(np.abs(a).sum()) for a in lam), (ad, fd)
+    for a in lt:                  # the transpose leaves nothing on the ghost ring
+        g = a.reshape(H, W).copy()
+        g[2:-2, 2:-2] = 0
+        assert not g.any()
+
+
+# ---- dryClean^T / friction^T / sponge^T -----------------------------------------------
+@pytest.mark.parametrize("stage,sponge", [(1, False), (2, False), (2, True)])
 @pytest.mark.parametrize("field", [False, True])
-def test_epilogue_rev_dot_product(stage, field):
+def test_epilogue_rev_dot_product(stage, field, sponge):
     nx, ny, dx = 15, 12, 4.0
     rows = swe_amd.geometry_cartesian(nx, ny, dx)
     man = 0.03
@@ -205,6 +258,14 @@ def test_epilogue_rev_dot_product(stage, field):
     interior[2:-2, 2:-2] = True
     interior = interior.ravel()
     lam = [rng.normal(size=N) * interior for _ in range(3)]      # ghosts of the output are not read
+    # makeSponge strengths (swe.mjs:1174-1186): a band of the domain, one cell relaxed fully (f = 1)
+    strength = None
+    if sponge:
+        st2 = np.zeros((ny + 4, nx + 4))
+        st2[2:-2, -8:-2] = np.linspace(0.002, 0.02, 6)
+        st2[5, -3] = 1.0
+        strength = st2.ravel()
+        o.set_sponge(strength)
 
     def out(state, n=nf):
         o.load_state(b, *state)
@@ -213,6 +274,8 @@ def test_epilogue_rev_dot_product(stage, field):
         o.dry_clean()
         if stage == 2:
             o.apply_friction(dt)
+            if sponge:
+                o.sponge(dt, 0.05)
         return np.concatenate([o.h.copy(), o.hu.copy(), o.hv.copy()])
 
     safe = (np.abs(h) > 0.01) & (h > 2e-3)
@@ -222,7 +285,7 @@ def test_epilogue_rev_dot_product(stage, field):
     f = lambda v: out((v[:N], v[N:2 * N], v[2 * N:]))
     eps = 1e-6
     fd = vjp_fd(f, x, np.concatenate(lam), dd, eps)
-    lh, lhu, lhv, lN = swe_amd.epilogue_rev_host(desc, stage, (h, hu, hv), nf, dt, lam)
+    lh, lhu, lhv, lN = swe_amd.epilogue_rev_host(desc, stage, (h, hu, hv), nf, dt, lam, sponge=strength)
     ad = float(np.dot(np.concatenate([lh, lhu, lhv]), dd))
     assert abs(ad - fd) <= 1e-6 * max(abs(fd), 1e-12), (ad, fd)
     # floored and dry cells pass nothing back
@@ -235,17 +298,18 @@ def test_epilogue_rev_dot_product(stage, field):
         assert abs(float(np.dot(lN, dn)) - fdn) <= 1e-6 * max(abs(fdn), 1e-12)
     elif stage == 2:
         # a scalar n is read at every cell: its per-cell adjoints are those of a uniform field
-        ref = swe_amd.epilogue_rev_host(desc, stage, (h, hu, hv), np.full(N, man), dt, lam)
+        ref = swe_amd.epilogue_rev_host(desc, stage, (h, hu, hv), np.full(N, man), dt, lam, sponge=strength)
         assert np.array_equal(lN, ref[3]) and lN.any()
     else:
         assert not lN.any()
 
 
 # ---- one whole step^T, composed from the host passes --------------------------------
-def step_rev_composed(o, desc, bc, b, A, dt, nf, lamC):
+def step_rev_composed(o, desc, bc, b, A, dt, nf, lamC, ext=None, sponge=None):
     """The reverse of one step(dt) (swe.mjs:754-814) in the order the device
     driver runs it: forward recompute with the oracle, then epilogue^T,
-    residual^T and BC^T of stage 2 and of stage 1."""
+    residual^T and BC^T of stage 2 and of stage 1.  ext: the step's Flather
+    values; sponge: the strengths of a sponge applied after the step."""
     N = b.size
     W = desc.nx + 4
     inter = np.zeros((desc.ny + 4, W), bool)
@@ -254,7 +318,7 @@ def step_rev_composed(o, desc, bc, b, A, dt, nf, lamC):
 
     def bc_of(S, bed):
         o.load_state(bed, *S)
-        o.apply_bc()
+        o.apply_bc(ext)
         st = o.state()
         return st[0], np.array(st[1:])
 
@@ -275,15 +339,15 @@ def step_rev_composed(o, desc, bc, b, A, dt, nf, lamC):
     bg, Bp = bc_of(B, bg)
     R2 = res(bg, Bp)
     pre2 = 0.5 * (Ap + Bp + dt * R2)
-    l2 = swe_amd.epilogue_rev_host(desc, 2, pre2, nf, dt, lamC)
+    l2 = swe_amd.epilogue_rev_host(desc, 2, pre2, nf, dt, lamC, sponge=sponge)
     lpre2, lN = np.array(l2[:3]), l2[3]
     lAp, lBp, lR2 = 0.5 * lpre2, 0.5 * lpre2, 0.5 * dt * lpre2
     lBp = lBp + np.array(swe_amd.residual_rev_host(desc, *Bp, bg, dt, *lR2))
-    lBp = np.array(swe_amd.bc_rev_host(desc, bc, *lBp))
+    lBp = np.array(swe_amd.bc_rev_host(desc, bc, *lBp, ext=ext, state=(bg, *Bp)))
     lpre1 = np.array(swe_amd.epilogue_rev_host(desc, 1, pre1, nf, dt, lBp)[:3])
     lAp = lAp + lpre1
     lAp = lAp + np.array(swe_amd.residual_rev_host(desc, *Ap, bg, dt, *(dt * lpre1)))
-    return np.array(swe_amd.bc_rev_host(desc, bc, *lAp)), lN
+    return np.array(swe_amd.bc_rev_host(desc, bc, *lAp, ext=ext, state=(bg, *Ap))), lN
 
 
 STEP_CASES = [
@@ -293,6 +357,9 @@ STEP_CASES = [
     dict(id="mc-wetdry", limiter="mc", bc=("reflect",) * 4, wetdry=True),
     dict(id="sphere-central", limiter="central", bc=("periodic", "periodic", "reflect", "reflect"), sphere=True),
     dict(id="sphere-mc", limiter="mc", bc=("periodic", "periodic", "reflect", "reflect"), sphere=True),
+    dict(id="mc-flather", limiter="mc", bc=("flather", "reflect", "periodic", "periodic"), flather=True),
+    dict(id="mc-flather-sponge", limiter="mc", bc=("flather", "flather", "reflect", "reflect"), flather=True,
+         sponge=True),
 ]
 
 
@@ -321,14 +388,24 @@ def test_step_rev_composition(case):
     inter = inter.ravel()
     lamC = np.array([rng.normal(size=N) * inter for _ in range(3)])
 
+    ext = np.array([0.08, 0.02, -0.05, 0.01, 0, 0, 0, 0]) if case.get("flather") else None
+    strength = None
+    if case.get("sponge"):
+        st2 = np.zeros((ny + 4, nx + 4))
+        st2[2:-2, -6:-2] = np.linspace(0.002, 0.01, 4)
+        strength = st2.ravel()
+        o.set_sponge(strength)
+
     def step(S, n=nf):
         o.field(oracle.MANNING)[:] = n
         o.load_state(b, *S)
-        o.step(dt)
+        o.step(dt, ext)
+        if strength is not None:
+            o.sponge(dt, 0.0)
         return np.concatenate([o.h.copy(), o.hu.copy(), o.hv.copy()])
 
     A = np.array([h, hu, hv])
-    lA, lN = step_rev_composed(o, desc, case["bc"], b, A, dt, nf, lamC)
+    lA, lN = step_rev_composed(o, desc, case["bc"], b, A, dt, nf, lamC, ext=ext, sponge=strength)
     safe = inter & (h > 0.05)
     d = np.array([rng.normal(size=N) * safe, rng.normal(size=N) * safe * 0.1, rng.normal(size=N) * safe * 0.1])
     f = lambda v: step(v.reshape(3, N))

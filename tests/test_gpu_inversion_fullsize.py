@@ -2,6 +2,8 @@
 dx = 50 m (shelf, 32 seamounts, wet/dry east coast, 32 gauges sampled every 10
 steps), 240 self-timed steps (src/swe.mjs:754-814 reversed).
 
+* The first guess is half the hidden truth surface plus 5 cm of smooth seeded
+  noise (workloads.inversion_first_guess).
 * Its forward is the solver's: the gauge series equals the oracle's host loop
   (the reference's step restated on the CPU) over the same 240 steps, to 1e-13
   (bit for bit in practice), the oracle running on a host thread meanwhile.
@@ -44,8 +46,7 @@ def test_inversion_adjoint_at_full_size():
     W, H = nx + 4, ny + 4
     rows = swe_amd.geometry_cartesian(nx, ny, dx, coriolis=p["coriolis"])
     z = np.zeros_like(ht)
-    eta_t = np.where(ht > 0, b + ht, 0.0)
-    h_eval = np.maximum(0.0, 0.5 * eta_t + 0.01 * workloads.SplitMix64(45).normals(W * H) - b)
+    h_eval = workloads.inversion_first_guess(b, ht, nx, ny)
     X, Y = np.meshgrid(np.arange(W), np.arange(H))
     n_truth = (0.025 * (1 + 0.3 * np.sin(X / 97.0) * np.cos(Y / 131.0))).ravel()
     n0 = np.full(W * H, 0.025)

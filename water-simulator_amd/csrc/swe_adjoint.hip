@@ -67,14 +67,14 @@ __global__ void k_adj_cells_slopes_rev(Params P, StateP S, const double *b, Stat
   int i, j;
   if (cell_of(P, i, j)) adj_cells_slopes_rev_at(P, S, b, lR, w, tcs, lS, i, j);
 }
-__global__ void k_adj_bc(Params P, StateW l, int pass) {
+__global__ void k_adj_bc(Params P, StateW l, int pass, StateP S, const double *b, const double *ext) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < (pass == 0 ? 4 * P.W : 4 * P.ny)) adj_bc_ghost(P, l, pass, t);
+  if (t < (pass == 0 ? 4 * P.W : 4 * P.ny)) adj_bc_ghost(P, l, pass, t, S, b, ext);
 }
-__global__ void k_adj_epilogue(Params P, StateP pre, StateP lOut, const double *nfield, double dt, double *lN,
-                               int stage, StateW lA, StateW lB, StateW lR) {
+__global__ void k_adj_epilogue(Params P, StateP pre, StateP lOut, const double *nfield, const double *sponge,
+                               double dt, double *lN, int stage, StateW lA, StateW lB, StateW lR) {
   int i, j;
-  if (cell_of(P, i, j)) adj_epilogue_rev_at(P, pre, lOut, nfield, dt, lN, stage, lA, lB, lR, i, j);
+  if (cell_of(P, i, j)) adj_epilogue_rev_at(P, pre, lOut, nfield, sponge, dt, lN, stage, lA, lB, lR, i, j);
 }
 
 __global__ void k_fill(size_t n, double *a, double v) {
@@ -82,23 +82,33 @@ __global__ void k_fill(size_t n, double *a, double v) {
   if (k < n) a[k] = v;
 }
 // the step about to run takes dt = dts[n] (and starts at t = ts[n] when ts is
-// given); the stage-time tide factors follow, and are also kept in tcs_out
-__global__ void k_ctl_dt(Ctl *ctl, const double *dts, const double *ts, int64_t n, double *tcs_out) {
+// given) and the recorded Flather values exts[8 n ..]; the stage-time tide
+// factors follow, and are also kept in tcs_out
+__global__ void k_ctl_dt(Ctl *ctl, const double *dts, const double *ts, const double *exts, int64_t n,
+                         double *tcs_out) {
   if (threadIdx.x != 0) return;
   ctl->dt = dts[n];
   if (ts) ctl->t = ts[n];
+  for (int q = 0; q < 8; ++q) ctl->ext[q] = exts[8 * n + q];
   ctl->dt_mode = DT_HOST;
   ctl->cfl_bits = 0ull;
   tide_times(ctl);
   if (tcs_out)
     for (int q = 0; q < 2 * MAX_TIDE * 2; ++q) tcs_out[q] = (&ctl->tide_cs[0][0][0])[q];
 }
-// self-timed steps from here on, from t = 0; the max-reduction starts from zero
-__global__ void k_ctl_self(Ctl *ctl) {
-  if (threadIdx.x == 0) { ctl->dt_mode = DT_SELF; ctl->cfl_bits = 0ull; ctl->t = 0.0; ctl->steps = 0; }
+// self-timed steps from here on, from t = 0, the Flather harmonics evaluated
+// on the device when the context has them; the max-reduction starts from zero
+__global__ void k_ctl_self(Ctl *ctl, int harm) {
+  if (threadIdx.x == 0) {
+    ctl->dt_mode = DT_SELF; ctl->cfl_bits = 0ull; ctl->t = 0.0; ctl->steps = 0; ctl->harm_on = harm;
+  }
 }
 __global__ void k_ctl_record(const Ctl *ctl, double *table, int64_t n) {
   if (threadIdx.x == 0) table[n] = ctl->dt_last;
+}
+// the Flather values the step about to run uses (set by the previous finalize)
+__global__ void k_ctl_save_ext(const Ctl *ctl, double *exts, int64_t n) {
+  if (threadIdx.x < 8) exts[8 * n + threadIdx.x] = ctl->ext[threadIdx.x];
 }
 __global__ void k_gauge_sample(int ng, const int *cells, const double *h, const double *b, double *out) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -147,7 +157,7 @@ int host_params(const swe_desc *d, const int32_t bc[4], Params &P) {
   P.nx = d->nx; P.ny = d->ny; P.W = d->nx + 4; P.H = d->ny + 4;
   P.sphere = d->kind == SWE_SPHERE; P.order = d->order; P.limiter = d->limiter;
   for (int s = 0; s < 4; ++s) {
-    if (bc && (bc[s] < SWE_BC_REFLECT || bc[s] > SWE_BC_PERIODIC)) return SWE_EINVAL;
+    if (bc && (bc[s] < SWE_BC_REFLECT || bc[s] > SWE_BC_FLATHER)) return SWE_EINVAL;
     P.bc[s] = bc ? bc[s] : BC_REFLECT;
   }
   P.g = d->g; P.half_g = 0.5 * d->g; P.neg_half_g = -d->g * 0.5; P.rg = 1.0 / d->g;
@@ -183,6 +193,7 @@ struct swe_adj {
   double *lN = nullptr;            // adjoint of the Manning field
   double *dt_dev = nullptr;        // recorded dt per step
   double *t_dev = nullptr;         // start time of every step
+  double *ext_dev = nullptr;       // recorded Flather values of every step, 8 per step
   int *gcells = nullptr;
   double *gobs = nullptr, *geta = nullptr;
   int ng = 0, every = 1, nsamp = 0;
@@ -249,11 +260,11 @@ int enqueue(swe_adj *a) {
   return SWE_OK;
 }
 
-// forward step n at the recorded dt (t accumulates on the device as in the
-// recorded run)
+// forward step n at the recorded dt and Flather values (t accumulates on the
+// device as in the recorded run)
 int replay_step(swe_adj *a, int64_t n) {
   hipLaunchKernelGGL(k_ctl_dt, dim3(1), dim3(64), 0, a->ctx->stream, a->ctx->dctl, a->dt_dev, (const double *)nullptr,
-                     n, (double *)nullptr);
+                     a->ext_dev, n, (double *)nullptr);
   return enqueue(a);
 }
 
@@ -275,11 +286,13 @@ int residual_rev(swe_adj *a, const StateP &S, double dt, const StateP &lR, const
   return SWE_OK;
 }
 
-int bc_rev(swe_adj *a, const StateW &l) {
+// BC^T of step n's applyBC on state S (whose interior Flather sides read)
+int bc_rev(swe_adj *a, const StateW &l, const StateP &S, int64_t n) {
   swe_ctx *c = a->ctx;
   const Params &P = c->P;
-  hipLaunchKernelGGL(k_adj_bc, grid_for(4 * P.W), dim3(256), 0, c->stream, P, l, 0);
-  hipLaunchKernelGGL(k_adj_bc, grid_for(4 * P.ny), dim3(256), 0, c->stream, P, l, 1);
+  const double *ext = a->ext_dev + 8 * n;
+  hipLaunchKernelGGL(k_adj_bc, grid_for(4 * P.W), dim3(256), 0, c->stream, P, l, 0, S, a->bed, ext);
+  hipLaunchKernelGGL(k_adj_bc, grid_for(4 * P.ny), dim3(256), 0, c->stream, P, l, 1, S, a->bed, ext);
   ACHK(a, hipGetLastError());
   return SWE_OK;
 }
@@ -308,7 +321,7 @@ int replay_segment(swe_adj *a, const double *src, int64_t n0, int64_t n1, int bu
     const int64_t q = n - n0;
     const StateW Ap = gslot(a, buf, q, G_AP), Bp = gslot(a, buf, q, G_BP), dc = gslot(a, buf, q, G_DC2);
     const StateW next = n + 1 < n1 ? gslot(a, buf, q + 1, G_AP) : tslot(a, T_CUR);
-    hipLaunchKernelGGL(k_ctl_dt, dim3(1), dim3(64), 0, st, c->dctl, a->dt_dev, a->t_dev, n,
+    hipLaunchKernelGGL(k_ctl_dt, dim3(1), dim3(64), 0, st, c->dctl, a->dt_dev, a->t_dev, a->ext_dev, n,
                        a->tcs_seg[buf] + (size_t)q * 2 * MAX_TIDE * 2);
     ACHK(a, launch_apply_bc(P, Ap, c->b, c->dctl, st));
     SweepArgs s1 = ctx_base_args(c);
@@ -347,7 +360,8 @@ int inner_checkpoints(swe_adj *a, int64_t so) {
   for (int64_t m = 1; m < nin; ++m) {
     for (int64_t n = o0 + (m - 1) * a->K; n < o0 + m * a->K; ++n) {
       const StateW X = tslot(a, T_A0 + cur), Y = tslot(a, T_A0 + (cur + 1) % 3), Z = tslot(a, T_A0 + (cur + 2) % 3);
-      hipLaunchKernelGGL(k_ctl_dt, dim3(1), dim3(64), 0, st, c->dctl, a->dt_dev, a->t_dev, n, (double *)nullptr);
+      hipLaunchKernelGGL(k_ctl_dt, dim3(1), dim3(64), 0, st, c->dctl, a->dt_dev, a->t_dev, a->ext_dev, n,
+                         (double *)nullptr);
       ACHK(a, launch_apply_bc(P, X, c->b, c->dctl, st));
       SweepArgs s1 = ctx_base_args(c);
       s1.in = sp(X);
@@ -382,14 +396,16 @@ int step_rev(swe_adj *a, int buf, int64_t q, int64_t n) {
   const double *tcs = tcs_of(a, buf, q);
   int rc;
   // stage 2: friction^T, dryClean^T, C0 = 0.5 (A' + B' + dt R2)^T, residual^T, BC^T
-  hipLaunchKernelGGL(k_adj_epilogue, g, blk, 0, st, P, sp(dc), sp(lam), c->mfield, dt, a->lN, 2, lAp, lBp, lR);
+  hipLaunchKernelGGL(k_adj_epilogue, g, blk, 0, st, P, sp(dc), sp(lam), c->mfield, c->sponge, dt, a->lN, 2, lAp,
+                     lBp, lR);
   if ((rc = residual_rev(a, sp(Bp), dt, sp(lR), lBp, tcs + 2 * MAX_TIDE, recslot(a, buf, q, 2)))) return rc;
-  if ((rc = bc_rev(a, lBp))) return rc;
+  if ((rc = bc_rev(a, lBp, sp(Bp), n))) return rc;
   // stage 1: dryClean^T (B's interior is stage 1's dryCleaned output), B0 = A' + dt R1 ^T,
   // residual^T, BC^T
-  hipLaunchKernelGGL(k_adj_epilogue, g, blk, 0, st, P, sp(Bp), sp(lBp), c->mfield, dt, a->lN, 1, lAp, none, lR);
+  hipLaunchKernelGGL(k_adj_epilogue, g, blk, 0, st, P, sp(Bp), sp(lBp), c->mfield, c->sponge, dt, a->lN, 1, lAp,
+                     none, lR);
   if ((rc = residual_rev(a, sp(Ap), dt, sp(lR), lAp, tcs, recslot(a, buf, q, 1)))) return rc;
-  if ((rc = bc_rev(a, lAp))) return rc;
+  if ((rc = bc_rev(a, lAp, sp(Ap), n))) return rc;
   std::swap(a->lam, a->lap);
   return SWE_OK;
 }
@@ -402,7 +418,7 @@ int set_initial(swe_adj *a, const double *h0, const double *nfield) {
   if (!rc) rc = swe_set_field(c, SWE_F_HV, z.data());
   if (!rc && nfield) rc = swe_set_field(c, SWE_F_MANNING, nfield);
   if (rc) return afail(a, rc, std::string("initial state: ") + swe_last_error(c));
-  hipLaunchKernelGGL(k_ctl_self, dim3(1), dim3(64), 0, c->stream, c->dctl);
+  hipLaunchKernelGGL(k_ctl_self, dim3(1), dim3(64), 0, c->stream, c->dctl, c->harm_enabled);
   ACHK(a, hipGetLastError());
   return SWE_OK;
 }
@@ -431,6 +447,7 @@ int forward(swe_adj *a, bool record, double *loss) {
     if (record && n % a->KO == 0)
       if ((rc = copy3(a, st3(a->ckpt + (size_t)(n / a->KO) * N3, a->N), sp(cur_state(c))))) return rc;
     if (record) {
+      hipLaunchKernelGGL(k_ctl_save_ext, dim3(1), dim3(64), 0, c->stream, c->dctl, a->ext_dev, n);
       if ((rc = enqueue(a))) return rc;
       hipLaunchKernelGGL(k_ctl_record, dim3(1), dim3(64), 0, c->stream, c->dctl, a->dt_dev, n);
     } else if ((rc = replay_step(a, n))) {
@@ -506,7 +523,7 @@ void swe_adj_destroy(swe_adj *a) {
   }
   (void)hipFree(a->bed);
   if (a->rs) (void)hipStreamDestroy(a->rs);
-  (void)hipFree(a->lN); (void)hipFree(a->dt_dev); (void)hipFree(a->t_dev);
+  (void)hipFree(a->lN); (void)hipFree(a->dt_dev); (void)hipFree(a->t_dev); (void)hipFree(a->ext_dev);
   (void)hipFree(a->gcells); (void)hipFree(a->gobs); (void)hipFree(a->geta);
   delete a;
 }
@@ -549,9 +566,8 @@ int swe_adj_create_levels(swe_ctx *c, int64_t nsteps, int64_t inner, int64_t out
   *out = nullptr;
   if (c->mid) return ctx_fail(c, SWE_ESTATE, "a step is in progress (swe_step_begin without swe_step_end)");
   for (int s = 0; s < 4; ++s)
-    if (c->P.bc[s] != BC_REFLECT && c->P.bc[s] != BC_OUTFLOW && c->P.bc[s] != BC_PERIODIC)
-      return ctx_fail(c, SWE_EINVAL, "the adjoint supports reflect, outflow and periodic boundaries");
-  if (c->sponge) return ctx_fail(c, SWE_EINVAL, "the adjoint does not differentiate a sponge forcing");
+    if (c->P.bc[s] == BC_STORED)
+      return ctx_fail(c, SWE_EINVAL, "the adjoint needs device boundary kinds (reflect, outflow, periodic, flather)");
   swe_adj *a = new swe_adj();
   a->ctx = c;
   a->nsteps = nsteps;
@@ -594,6 +610,7 @@ int swe_adj_create_levels(swe_ctx *c, int64_t nsteps, int64_t inner, int64_t out
   if ((e = hipMalloc(&a->lN, sizeof(double) * N)) != hipSuccess) return bail("Manning adjoint", e);
   if ((e = hipMalloc(&a->dt_dev, sizeof(double) * (size_t)nsteps)) != hipSuccess) return bail("dt record", e);
   if ((e = hipMalloc(&a->t_dev, sizeof(double) * (size_t)nsteps)) != hipSuccess) return bail("t record", e);
+  if ((e = hipMalloc(&a->ext_dev, sizeof(double) * 8 * (size_t)nsteps)) != hipSuccess) return bail("ext record", e);
   hipLaunchKernelGGL(k_fill, grid_for(nslots * N), dim3(256), 0, c->stream, nslots * N, a->work, 0.0);
   hipLaunchKernelGGL(k_fill, grid_for(T_COUNT * N3), dim3(256), 0, c->stream, T_COUNT * N3, a->tmp, 0.0);
   if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail("clear", e);
@@ -759,17 +776,23 @@ int swe_adj_host_residual_rev(const swe_desc *d, const double *h, const double *
   return SWE_OK;
 }
 
-int swe_adj_host_bc_rev(const swe_desc *d, const int32_t bc[4], double *lh, double *lhu, double *lhv) {
+int swe_adj_host_bc_rev(const swe_desc *d, const int32_t bc[4], const double ext[8], const double *h,
+                        const double *hu, const double *hv, const double *b, double *lh, double *lhu, double *lhv) {
   Params P;
   if (!bc || host_params(d, bc, P) || !lh || !lhu || !lhv) return SWE_EINVAL;
+  bool flather = false;
+  for (int s = 0; s < 4; ++s) flather = flather || bc[s] == SWE_BC_FLATHER;
+  if (flather && (!ext || !h || !hu || !hv || !b)) return SWE_EINVAL;
   const StateW l = {lh, lhu, lhv};
-  for (int t = 0; t < 4 * P.W; ++t) adj_bc_ghost(P, l, 0, t);
-  for (int t = 0; t < 4 * P.ny; ++t) adj_bc_ghost(P, l, 1, t);
+  const StateP S = {h, hu, hv};
+  for (int t = 0; t < 4 * P.W; ++t) adj_bc_ghost(P, l, 0, t, S, b, ext);
+  for (int t = 0; t < 4 * P.ny; ++t) adj_bc_ghost(P, l, 1, t, S, b, ext);
   return SWE_OK;
 }
 
 int swe_adj_host_epilogue_rev(const swe_desc *d, int stage, const double *pre_h, const double *pre_hu,
-                              const double *pre_hv, const double *nfield, double dt, const double *lout_h,
+                              const double *pre_hv, const double *nfield, const double *sponge, double dt,
+                              const double *lout_h,
                               const double *lout_hu, const double *lout_hv, double *lpre_h, double *lpre_hu,
                               double *lpre_hv, double *lN) {
   Params P;
@@ -777,6 +800,7 @@ int swe_adj_host_epilogue_rev(const swe_desc *d, int stage, const double *pre_h,
       !lout_hu || !lout_hv || !lpre_h || !lpre_hu || !lpre_hv)
     return SWE_EINVAL;
   if (nfield) P.friction = 2;
+  P.sponge = sponge ? 1 : 0;
   const size_t N = (size_t)P.W * P.H;
   // dryClean (swe.mjs:734-745) of the given state on the interior
   std::vector<double> dc(3 * N);
@@ -796,7 +820,8 @@ int swe_adj_host_epilogue_rev(const swe_desc *d, int stage, const double *pre_h,
   for (size_t k = 0; k < N; ++k) { lpre_h[k] = 0.0; lpre_hu[k] = 0.0; lpre_hv[k] = 0.0; }
   for (int j = 0; j < P.H; ++j)
     for (int i = 0; i < P.W; ++i)
-      adj_epilogue_rev_at(P, sp3(dc.data(), N), {lout_h, lout_hu, lout_hv}, nfield, dt, lN, stage, lA, lB, lR, i, j);
+      adj_epilogue_rev_at(P, sp3(dc.data(), N), {lout_h, lout_hu, lout_hv}, nfield, sponge, dt, lN, stage, lA, lB,
+                          lR, i, j);
   if (stage == 2)
     for (size_t k = 0; k < N; ++k) { lpre_h[k] *= 2.0; lpre_hu[k] *= 2.0; lpre_hv[k] *= 2.0; }
   return SWE_OK;

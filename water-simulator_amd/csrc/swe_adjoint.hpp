@@ -612,13 +612,50 @@ __host__ __device__ inline void adj_add(double *p, double v) {
 #endif
 }
 
+// Flather's ghost (swe.mjs:1084-1116) as a function of its wet source cell:
+// with r = sqrt(g / hInt), uInt = vel(q, hInt) of the normal momentum q,
+//   Rin = u0 + sgn e0 r,  Rout = uInt - sgn (b + hInt) r,
+//   un = (Rin + Rout) / 2,  en = sgn (Rin - Rout) / (2 r),
+//   ghost h = max(0, en - b), normal momentum = ghost h * un,
+// tangential momentum copied.  Its partial derivatives with respect to the
+// source's hInt and q (the clip at 0 takes the subgradient 0).
+struct FlatherD { double dh_dh, dh_dq, dq_dh, dq_dq; };
+__host__ __device__ inline FlatherD flather_deriv(const Params &P, int side, double hInt, double q, double bed,
+                                                  double e0, double u0) {
+  const double sgn = (side == 0 || side == 2) ? 1.0 : -1.0;
+  const double r = __builtin_sqrt(P.g / hInt);
+  const double dr = -0.5 * r / hInt;                  // d r / d hInt
+  const double uInt = vel1(q, hInt, P.eps4);
+  const VelD du = vel_deriv(q, hInt, P.eps4);
+  const double etaInt = bed + hInt;
+  const double Rin = u0 + sgn * e0 * r, Rout = uInt - sgn * etaInt * r;
+  const double un = 0.5 * (Rin + Rout);
+  const double en = sgn * (Rin - Rout) / (2 * r);
+  const double hg = jmax0(en - bed);
+  const double dRin_dh = sgn * e0 * dr;
+  const double dRout_dh = du.dh - sgn * (r + etaInt * dr), dRout_dq = du.dhq;
+  const double dun_dh = 0.5 * (dRin_dh + dRout_dh), dun_dq = 0.5 * dRout_dq;
+  const double den_dh = sgn * (dRin_dh - dRout_dh) / (2 * r) - en * dr / r;
+  const double den_dq = -sgn * dRout_dq / (2 * r);
+  const bool wet = en - bed > 0;
+  FlatherD d;
+  d.dh_dh = wet ? den_dh : 0.0;
+  d.dh_dq = wet ? den_dq : 0.0;
+  d.dq_dh = d.dh_dh * un + hg * dun_dh;
+  d.dq_dq = d.dh_dq * un + hg * dun_dq;
+  return d;
+}
+
 // Fold ghost t of one pass into its source and clear it.  applyBC writes the
 // W/E columns (every row) and then the S/N rows (every column, corners
 // included, a periodic source possibly a W/E ghost); its transpose runs the S/N
 // pass first (t < 4 W), then the W/E pass over the interior rows (t < 4 ny):
 // the corner rows' W/E ghosts are overwritten by the S/N pass, so they carry
-// no adjoint of their own.
-__host__ __device__ inline void adj_bc_ghost(const Params &P, const StateW &l, int pass, int t) {
+// no adjoint of their own.  A Flather ghost is a nonlinear function of its
+// (interior) source cell: S and b are the state and bed applyBC read, ext the
+// step's far-field values (eta_ext, u_ext per side); the linear kinds need none.
+__host__ __device__ inline void adj_bc_ghost(const Params &P, const StateW &l, int pass, int t, const StateP &S,
+                                             const double *b, const double *ext) {
   int gi, gj, side;
   if (pass == 0) {
     const int rr = t / P.W;
@@ -631,28 +668,41 @@ __host__ __device__ inline void adj_bc_ghost(const Params &P, const StateW &l, i
   int si, sj;
   adj_bc_src(kind, side, gi - 2, gj - 2, P.nx, P.ny, si, sj);
   const int g = gj * P.W + gi, s = (sj + 2) * P.W + (si + 2);
-  const bool flipU = kind == BC_REFLECT && side < 2, flipV = kind == BC_REFLECT && side >= 2;
   const double lh = l.h[g], lhu = l.hu[g], lhv = l.hv[g];
   l.h[g] = 0.0; l.hu[g] = 0.0; l.hv[g] = 0.0;
+  if (kind == BC_FLATHER) {
+    const double hInt = S.h[s];
+    // a dry source: the ghost is max(0, eta_ext - b), zero momentum, no state in it
+    if (hInt <= P.min_depth) return;
+    const bool we = side < 2;
+    const FlatherD d = flather_deriv(P, side, hInt, we ? S.hu[s] : S.hv[s], b[s], ext[2 * side], ext[2 * side + 1]);
+    const double lqn = we ? lhu : lhv, lqt = we ? lhv : lhu;
+    adj_add(&l.h[s], lh * d.dh_dh + lqn * d.dq_dh);
+    adj_add(&(we ? l.hu : l.hv)[s], lh * d.dh_dq + lqn * d.dq_dq);
+    adj_add(&(we ? l.hv : l.hu)[s], lqt);
+    return;
+  }
+  const bool flipU = kind == BC_REFLECT && side < 2, flipV = kind == BC_REFLECT && side >= 2;
   adj_add(&l.h[s], lh);
   adj_add(&l.hu[s], flipU ? -lhu : lhu);
   adj_add(&l.hv[s], flipV ? -lhv : lhv);
 }
 
-// ---- dryClean^T and friction^T (swe.mjs:659-704, :734-814) -------------------------
+// ---- dryClean^T, friction^T and sponge^T (swe.mjs:659-704, :734-814, :1188-1197) ----
 // Reverse of a stage's pointwise part at cell k, given the adjoint lOut of its
 // output (interior) and the stage's dryCleaned (pre-friction) state `dc`:
-// friction^T (stage 2, accumulating the Manning adjoint into lN), dryClean^T,
-// then the stage combination^T:
+// sponge^T (stage 2: the relaxation h += (target - h) f, q -= q f scales every
+// field by 1 - f, f = min(1, strength dt 60)), friction^T (stage 2, accumulating
+// the Manning adjoint into lN), dryClean^T, then the stage combination^T:
 //   stage 2: lA = 0.5 lC0, lB = 0.5 lC0, lR = 0.5 dt lC0
 //   stage 1: lA += lB0,    lR = dt lB0
 // dryClean floored the cells whose depth it left at exactly 0 (where the
 // combination gave 0 exactly the subgradient 0 is taken), and zeroed the
 // momentum of the cells at or below minDepth.  Ghost cells carry no adjoint.
 __host__ __device__ inline void adj_epilogue_rev_at(const Params &P, const StateP &dc, const StateP &lOut,
-                                                    const double *nfield, double dt, double *lN, int stage,
-                                                    const StateW &lA, const StateW &lB, const StateW &lR,
-                                                    int i, int j) {
+                                                    const double *nfield, const double *sponge, double dt,
+                                                    double *lN, int stage, const StateW &lA, const StateW &lB,
+                                                    const StateW &lR, int i, int j) {
   const int k = j * P.W + i;
   double lh = 0.0, lhu = 0.0, lhv = 0.0;
   if (i >= 2 && i <= P.W - 3 && j >= 2 && j <= P.H - 3) {
@@ -660,6 +710,13 @@ __host__ __device__ inline void adj_epilogue_rev_at(const Params &P, const State
     const bool floored = h == 0.0;
     const bool dry = h <= P.min_depth;
     lh = lOut.h[k]; lhu = lOut.hu[k]; lhv = lOut.hv[k];
+    if (stage == 2 && P.sponge && sponge) {
+      const double str = sponge[k];
+      if (str > 0) {
+        const double keep = 1.0 - jmin(1.0, str * dt * 60);
+        lh *= keep; lhu *= keep; lhv *= keep;
+      }
+    }
     if (stage == 2 && P.friction && !dry) {
       const double n = P.friction == 2 ? nfield[k] : P.manning;
       if (n != 0.0 && n == n) {

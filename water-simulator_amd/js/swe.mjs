@@ -241,6 +241,20 @@ export class ShallowWater {
     }
   }
 
+  // a makeSponge() forcing runs on the device: its strengths and reference level
+  // go with the step; returns its tag (null when the forcing is none or another)
+  _syncSponge() {
+    const sp = this.forcing && this.forcing[SPONGE_TAG];
+    if (sp) {
+      if (this._spongeSent !== sp.strength) { addon.setField(this._dev, F.SPONGE, sp.strength); this._spongeSent = sp.strength; }
+      addon.setScalar(this._dev, S.SPONGE_REF, typeof sp.etaRef === 'function' ? sp.etaRef(this.t) : sp.etaRef);
+    } else if (this._spongeSent) {
+      addon.setField(this._dev, F.SPONGE, null);
+      this._spongeSent = null;
+    }
+    return sp || null;
+  }
+
   applyBC() {
     const ext = this._boundaries();
     if (!ext) { this._hostApplyBC(); return; }
@@ -300,14 +314,7 @@ export class ShallowWater {
   step(dt = null) {
     const ext = this._boundaries();
     this._syncParams();
-    const sp = this.forcing && this.forcing[SPONGE_TAG];
-    if (sp) {
-      if (this._spongeSent !== sp.strength) { addon.setField(this._dev, F.SPONGE, sp.strength); this._spongeSent = sp.strength; }
-      addon.setScalar(this._dev, S.SPONGE_REF, typeof sp.etaRef === 'function' ? sp.etaRef(this.t) : sp.etaRef);
-    } else if (this._spongeSent) {
-      addon.setField(this._dev, F.SPONGE, null);
-      this._spongeSent = null;
-    }
+    const sp = this._syncSponge();
     addon.setScalar(this._dev, S.T, this.t);
     const dtArg = dt == null ? NaN : dt;
     let taken, mf;
@@ -531,8 +538,10 @@ export class ShallowWater {
  *
  * J = sum over samples and gauges of (eta - obs)^2.  The dt sequence is the
  * forward's (held fixed); branches take the subgradient the forward took.
- * Boundaries reflect/outflow/periodic; no sponge.  The adjoint drives the
- * solver's device state, so `sim.h` etc. are re-uploaded by its next call.
+ * Boundaries reflect/outflow/periodic, and flather described by
+ * harmonicFlather() (the forward runs from t = 0); a makeSponge() forcing is
+ * differentiated too.  The adjoint drives the solver's device state, so
+ * `sim.h` etc. are re-uploaded by its next call.
  */
 export class Adjoint {
   // checkpointEvery: steps per replayed segment (0: chosen by the library);
@@ -549,7 +558,8 @@ export class Adjoint {
 
   _prepare() {
     const sim = this.sim;
-    if (!sim._boundaries()) throw new Error('Adjoint: every boundary must be reflect, outflow or periodic');
+    sim._deviceBoundaries('Adjoint');
+    if (sim.forcing && !sim._syncSponge()) throw new Error('Adjoint: the only forcing it differentiates is makeSponge()');
     sim._syncParams();
     sim._upload(['B']);
   }

@@ -715,8 +715,8 @@ def _bind_adjoint(L):
         "swe_adj_get_dt": (i, [vp, dp]),
         "swe_adj_timing": (i, [vp, dp, dp]),
         "swe_adj_host_residual_rev": (i, [D, dp, dp, dp, dp, d, dp, dp, dp, dp, dp, dp]),
-        "swe_adj_host_bc_rev": (i, [D, ip, dp, dp, dp]),
-        "swe_adj_host_epilogue_rev": (i, [D, i, dp, dp, dp, dp, d, dp, dp, dp, dp, dp, dp, dp]),
+        "swe_adj_host_bc_rev": (i, [D, ip, dp, dp, dp, dp, dp, dp, dp, dp]),
+        "swe_adj_host_epilogue_rev": (i, [D, i, dp, dp, dp, dp, dp, d, dp, dp, dp, dp, dp, dp, dp]),
         "swe_adj_host_hllc_rev": (None, [dp, d, d, d, d, d, d, d, d, d, d, d]),
     }
     for name, (res, args) in sig.items():
@@ -763,21 +763,26 @@ def residual_rev_host(desc, h, hu, hv, b, dt, lrh, lrhu, lrhv):
     return out
 
 
-def bc_rev_host(desc, bc, lh, lhu, lhv):
-    """BC^T of applyBC() (swe.mjs:289-303) for reflect/outflow/periodic sides, on copies."""
+def bc_rev_host(desc, bc, lh, lhu, lhv, ext=None, state=None):
+    """BC^T of applyBC() (swe.mjs:289-303) on copies; a flather side needs the
+    far-field values ext (8) and the state (b, h, hu, hv) applyBC read."""
     L = lib()
     _bind_adjoint(L)
     kinds = (ctypes.c_int32 * 4)(*[BC[k] if isinstance(k, str) else k for k in bc])
     out = [np.array(x, dtype=np.float64, copy=True) for x in (lh, lhu, lhv)]
-    rc = L.swe_adj_host_bc_rev(ctypes.byref(desc), kinds, *[_dp(x) for x in out])
+    e = None if ext is None else _ext(ext)
+    st = [None] * 4 if state is None else [np.ascontiguousarray(x, dtype=np.float64) for x in state]
+    ptr = lambda x: None if x is None else _dp(x)
+    rc = L.swe_adj_host_bc_rev(ctypes.byref(desc), kinds, ptr(e), ptr(st[1]), ptr(st[2]), ptr(st[3]), ptr(st[0]),
+                               *[_dp(x) for x in out])
     if rc != 0:
         raise SweError(f"swe_adj_host_bc_rev: {rc}")
     return out
 
 
-def epilogue_rev_host(desc, stage, pre, nfield, dt, lout):
-    """Adjoint of the pre-dryClean state at a stage's end (stage 2 also friction);
-    returns (lpre_h, lpre_hu, lpre_hv, lN)."""
+def epilogue_rev_host(desc, stage, pre, nfield, dt, lout, sponge=None):
+    """Adjoint of the pre-dryClean state at a stage's end (stage 2 also friction and
+    the sponge with per-cell strengths `sponge`); returns (lpre_h, lpre_hu, lpre_hv, lN)."""
     L = lib()
     _bind_adjoint(L)
     p = [np.ascontiguousarray(x, dtype=np.float64) for x in pre]
@@ -785,8 +790,10 @@ def epilogue_rev_host(desc, stage, pre, nfield, dt, lout):
     out = [np.zeros_like(p[0]) for _ in range(3)]
     lN = np.zeros_like(p[0])
     nf = None if nfield is None else np.ascontiguousarray(nfield, dtype=np.float64)
+    sg = None if sponge is None else np.ascontiguousarray(sponge, dtype=np.float64)
     rc = L.swe_adj_host_epilogue_rev(ctypes.byref(desc), stage, *[_dp(x) for x in p],
-                                     None if nf is None else _dp(nf), dt, *[_dp(x) for x in lo],
+                                     None if nf is None else _dp(nf), None if sg is None else _dp(sg), dt,
+                                     *[_dp(x) for x in lo],
                                      *[_dp(x) for x in out], _dp(lN))
     if rc != 0:
         raise SweError(f"swe_adj_host_epilogue_rev: {rc}")

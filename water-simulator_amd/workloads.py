@@ -179,6 +179,22 @@ def inversion(nx=1024, ny=1024, dx=50.0, seed=4, ngauges=32):
     return b.ravel(), h.ravel(), params, np.array(gauges, dtype=np.int32)
 
 
+def inversion_first_guess(b, h_truth, nx=1024, ny=1024, seed=45, amp=0.05, corr=3.0):
+    """The first guess of the inversion's finite-difference checks: half the
+    hidden truth surface plus `amp` metres of smooth seeded noise (white
+    normals filtered by a Gaussian of `corr` cells, rescaled to unit deviation),
+    over the wet cells; returns the depth h over W*H cells.  The noise keeps
+    the surface differences away from zero, where every limiter sits on its
+    kink, and being smooth it gives neighbouring differences one sign, so few
+    limiter arguments lie near a kink either."""
+    from scipy import ndimage
+    W, H = nx + 4, ny + 4
+    eta_t = np.where(h_truth > 0, b + h_truth, 0.0)
+    n = ndimage.gaussian_filter(SplitMix64(seed).normals(W * H).reshape(H, W), corr).ravel()
+    n /= n.std()
+    return np.maximum(0.0, 0.5 * eta_t + amp * n - b)
+
+
 def inversion_directions(n, nx, ny, dx, wet, seed=46, near=None, reach=2000.0):
     """n seeded random directions (dh, dn) of the inversion's control space, for
     finite-difference checks of its gradient: dh is 1 cm times a sum of 4 Gaussian

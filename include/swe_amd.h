@@ -206,6 +206,23 @@ int swe_last_advance_ms(swe_ctx *ctx, double *ms);
  * the Cartesian volume, one running sum over all cells in the reference, is the
  * exactly rounded sum here.  SWE_S_DIAG_MS gives the kernels' duration. */
 int swe_diagnostics(swe_ctx *ctx, double eta_ref, double out[SWE_D_COUNT]);
+/* swe_diagnostics on fields the caller owns, allocating nothing.  No counterpart
+ * in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * b, h, hu, hv: device pointers to W*H = (nx+4)*(ny+4) doubles in the layout of
+ * the fields above; area: device pointer to the H row areas (SWE_SPHERE only,
+ * else NULL); dx, dy: the Cartesian spacing; min_depth, eps4, g as in swe_desc.
+ * Same kernels and the same results as swe_diagnostics.  Its scratch (row sums,
+ * block partials, maxima) is a static device array of the library sized for
+ * ny <= SWE_DIAG_NOALLOC_MAX_ROWS (about 300 KB per GPU, held until the process
+ * ends); a larger ny is refused with SWE_EINVAL.  Runs on the null stream and
+ * returns when out[] is filled; the caller's fields must be ready (every other
+ * swe_* call returns with its work done).  Calls that overlap on one GPU race on
+ * that array; errors are read with swe_last_error(NULL). */
+#define SWE_DIAG_NOALLOC_MAX_ROWS 16384
+int swe_diagnostics_noalloc(int32_t nx, int32_t ny, int32_t kind, const double *b, const double *h,
+                            const double *hu, const double *hv, const double *area, double dx, double dy,
+                            double min_depth, double eps4, double g, double eta_ref, double out[SWE_D_COUNT]);
 /* Gauge recorder of swe_advance (no reference counterpart: tools/waves.mjs:211-232
  * records sim.eta(i, j) = b + h at chosen cells from JS after each step).  Arms
  * the recorder with ng cells (indices into W*H) and room for `capacity` samples:

@@ -17,6 +17,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "swe_ctx.hpp"
@@ -386,3 +387,87 @@ extern "C" int swe_diagnostics(swe_ctx *c, double eta_ref, double out[SWE_D_COUN
   return SWE_OK;
 }
 
+// ---- swe_diagnostics_noalloc: the same kernels on caller-owned fields, with
+// their scratch in a static device array instead of an allocation -----------
+//
+// Layout of the array (doubles): [0, 3) the three accumulators (bit patterns of
+// the two maxima, the non-finite count; they must start at zero, so every call
+// clears them first), [3, 3 + DG_OUT) the results, then from 16 the per-row sums
+// row_v and row_e (NA_ROWS each) and the blocks' volume partials (2 per block).
+// Each GPU holds its own copy until the process ends, at its own address.
+namespace swe {
+namespace {
+constexpr int NA_ROWS = SWE_DIAG_NOALLOC_MAX_ROWS;
+constexpr int NA_BLOCKS = (NA_ROWS + DG_ROWS - 1) / DG_ROWS;
+constexpr size_t NA_DOUBLES = 16 + 2 * (size_t)NA_ROWS + 2 * (size_t)NA_BLOCKS;
+constexpr int NA_DEVICES = 64;
+
+__global__ void k_clear_acc(unsigned long long *acc) {
+  if (threadIdx.x < 3) acc[threadIdx.x] = 0ull;
+}
+}  // namespace
+}  // namespace swe
+
+static __device__ __align__(256) double scratch[swe::NA_DOUBLES];
+
+namespace {
+std::mutex scratch_mu;
+double *scratch_dev[swe::NA_DEVICES];   // the array's address on each device, once looked up
+
+// The current device's copy of `scratch`: its address is looked up once per
+// device (with that device current) and kept; host code uses only this
+// address, never the array's name, which on the host is a stand-in.
+int scratch_on_device(double **out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return swe::ctx_fail(nullptr, SWE_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= swe::NA_DEVICES) return swe::ctx_fail(nullptr, SWE_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return swe::ctx_fail(nullptr, SWE_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    scratch_dev[dev] = static_cast<double *>(p);
+  }
+  *out = scratch_dev[dev];
+  return SWE_OK;
+}
+}  // namespace
+
+extern "C" int swe_diagnostics_noalloc(int32_t nx, int32_t ny, int32_t kind, const double *b, const double *h,
+                                       const double *hu, const double *hv, const double *area, double dx, double dy,
+                                       double min_depth, double eps4, double g, double eta_ref,
+                                       double out[SWE_D_COUNT]) {
+  if (!b || !h || !hu || !hv || !out) return ctx_fail(nullptr, SWE_EINVAL, "swe_diagnostics_noalloc: null argument");
+  if (nx < 2 || ny < 1) return ctx_fail(nullptr, SWE_EINVAL, "swe_diagnostics_noalloc: nx must be >= 2 and ny >= 1");
+  if (ny > SWE_DIAG_NOALLOC_MAX_ROWS)
+    return ctx_fail(nullptr, SWE_EINVAL, "swe_diagnostics_noalloc: ny = " + std::to_string(ny) + " exceeds its " +
+                                             std::to_string(SWE_DIAG_NOALLOC_MAX_ROWS) + "-row scratch");
+  if (kind != SWE_CARTESIAN && kind != SWE_SPHERE) return ctx_fail(nullptr, SWE_EINVAL, "swe_diagnostics_noalloc: unknown kind");
+  if (kind == SWE_SPHERE && !area) return ctx_fail(nullptr, SWE_EINVAL, "swe_diagnostics_noalloc: the sphere needs its row areas");
+  double *sp = nullptr;
+  int rc = scratch_on_device(&sp);
+  if (rc) return rc;
+  const int ng = 2, nblocks = (ny + DG_ROWS - 1) / DG_ROWS;
+  DiagArgs a{};
+  a.h = h; a.hu = hu; a.hv = hv; a.b = b; a.area = area;
+  a.acc = reinterpret_cast<unsigned long long *>(sp);
+  a.out = sp + 3;
+  a.row_v = sp + 16;
+  a.row_e = sp + 16 + NA_ROWS;
+  a.blk = sp + 16 + 2 * (size_t)NA_ROWS;
+  a.nx = nx; a.ny = ny; a.W = nx + 2 * ng; a.H = ny + 2 * ng; a.ng = ng;
+  a.sphere = kind == SWE_SPHERE; a.nblocks = nblocks;
+  a.min_depth = min_depth; a.eps4 = eps4; a.g = g; a.ref = eta_ref; a.dx = dx; a.dy = dy;
+  hipLaunchKernelGGL(k_clear_acc, dim3(1), dim3(64), 0, nullptr, a.acc);
+  hipLaunchKernelGGL(k_diag_rows<false>, dim3(nblocks), dim3(DG_THREADS), 0, nullptr, a);
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(DG_THREADS), 0, nullptr, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return ctx_fail(nullptr, SWE_EHIP, std::string("swe_diagnostics_noalloc: ") + hipGetErrorString(e));
+  double res[DG_OUT];
+  e = hipMemcpy(res, a.out, sizeof(res), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return ctx_fail(nullptr, SWE_EHIP, std::string("swe_diagnostics_noalloc: ") + hipGetErrorString(e));
+  std::memcpy(out, res, sizeof(double) * SWE_D_COUNT);
+  return SWE_OK;
+}

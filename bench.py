@@ -364,6 +364,92 @@ def sphere_measure(steps, adj_steps, every=10):
     return out
 
 
+def sphere_adjoint_long(hours, fd_dirs, every=10, steps=(1e-1, 1e-2)):
+    """Config 5's adjoint over a horizon where friction acts: member 0 of the
+    2160x1080 global ocean under the M2/S2/K1 equilibrium tide, started from a
+    1 m large-scale surface disturbance, the misfit at 64 open-ocean gauges
+    sampled every 10 steps over `hours` of simulated time.  Observations come
+    from a truth run with n = 0.025 everywhere, the gradient is taken at member
+    0's own n, from the same initial state: a friction-only calibration.  dJ/dn
+    by the adjoint (checkpointing chosen by the library: two levels at this
+    length), checked by central differences along fd_dirs seeded smooth
+    directions of n (1e-3 times 0.5 + 0.5 cos(k lon + a) cos(l lat) on the
+    ocean cells) at the given steps of them; the gradient's cost against one
+    plain forward of the same steps."""
+    import numpy as np
+    import swe_amd
+    import workloads
+    b, h, p, mann = workloads.sphere_tides()
+    nx, ny = p["nx"], p["ny"]
+    W, H = nx + 4, ny + 4
+    rows = swe_amd.geometry_spherical(nx, ny, R=p["R"], lat0=p["lat0"], lat1=p["lat1"], omega=p["omega"])
+    lat, lon = swe_amd.sphere_lonlat(nx, ny, p["lat0"], p["lat1"])
+    z = np.zeros_like(h)
+    s = swe_amd.Solver(nx, ny, rows, sphere=True, R=p["R"], manning=mann[0], cfl=p["cfl"])
+    s.set_tide(p["tides"], lat, lon)
+    LA, LO = np.repeat(lat, W), np.tile(lon, H)
+    h_t = np.where(h > 0, h + 1.0 * np.cos(LA) ** 2 * np.cos(3 * LO), 0.0)
+    s.load_state(b, h_t, z, z)
+    s.autotune()
+    s.advance(20)
+    dt = s.t / 20
+    nsteps = int(math.ceil(hours * 3600.0 / dt))
+    rng = workloads.SplitMix64(51)
+    gauges = []
+    while len(gauges) < 64:
+        k = (2 + int(rng.uniform(0, ny))) * W + 2 + int(rng.uniform(0, nx))
+        if h[k] > 1000.0 and abs(LA[k]) < 1.2 and k not in gauges:
+            gauges.append(k)
+    gauges = np.array(gauges, dtype=np.int32)
+    n_truth = np.full(W * H, 0.025)
+    n0 = np.full(W * H, mann[0])
+    log(f"[bench] sphere adjoint over {hours:g} h: {nsteps} steps of ~{dt:.4f} s")
+    adj = swe_amd.Adjoint(s, nsteps)
+    ns = nsteps // every
+    adj.set_gauges(gauges, every, np.zeros((ns, len(gauges))))
+    adj.forward(h_t, n_truth)
+    log("[bench] sphere adjoint: truth run done")
+    adj.set_gauges(gauges, every, adj.series())
+    J = adj.forward(h_t, n0)
+    log(f"[bench] sphere adjoint: recorded forward done, J = {J:.6e}")
+    gh, gn = adj.backward()
+    f_ms, b_ms = adj.timing()
+    log(f"[bench] sphere adjoint: J = {J:.6e}, |dJ/dn|max = {np.abs(gn).max():.3e}, gradient {b_ms / 1e3:.1f} s")
+    s.load_state(b, h_t, z, z)
+    s.set_manning_field(n0)
+    s.use_graph(True)
+    s.advance(nsteps)
+    pf_ms = s.last_advance_ms()
+    out = {"workload": "sphere_2160x1080_member0_friction_calibration_M2_S2_K1", "hours": hours, "steps": nsteps,
+           "gauges": len(gauges), "sample_every": every, "loss": J, "plain_forward_ms": pf_ms,
+           "recorded_forward_ms": f_ms, "gradient_ms": b_ms, "gradient_over_forward": b_ms / pf_ms,
+           "grad_n_max": float(np.abs(gn).max()), "grad_n_l2": float(np.linalg.norm(gn))}
+    if fd_dirs > 0:
+        ocean = (h > 0).reshape(H, W)
+        ocean[:2] = ocean[-2:] = False
+        ocean[:, :2] = ocean[:, -2:] = False
+        r2 = workloads.SplitMix64(52)
+        rows_out = []
+        for q in range(fd_dirs):
+            k, l, a = int(r2.uniform(1, 7)), int(r2.uniform(1, 5)), r2.uniform(0, 2 * math.pi)
+            dn = (1e-3 * (0.5 + 0.5 * np.cos(k * lon[None, :] + a) * np.cos(l * lat[:, None])) * ocean).ravel()
+            ad = float(np.dot(gn, dn))
+            fd = [(adj.loss(h_t, n0 + e * dn) - adj.loss(h_t, n0 - e * dn)) / (2 * e) for e in steps]
+            rel = [abs(f - ad) / abs(ad) for f in fd]
+            row = {"k": k, "l": l, "adjoint": ad, "fd": fd, "rel_err": rel}
+            if len(fd) > 1:
+                row["step_converged"] = abs(fd[0] - fd[1]) <= 1e-6 * abs(fd[1])
+            rows_out.append(row)
+            log(f"[bench] sphere adjoint FD {q + 1}/{fd_dirs}: adjoint {ad:+.6e}, rel err "
+                + ", ".join(f"{e:g}: {r:.2e}" for e, r in zip(steps, rel)))
+        out["fd_steps"] = list(steps)
+        out["fd"] = rows_out
+        out["fd_within_1e-6"] = int(sum(min(r["rel_err"]) <= 1e-6 for r in rows_out))
+    adj.close()
+    s.close()
+    return out
+
+
 def diagnostics_measure(sim, swe_amd, nx, ny, reps=20):
     """Device diagnostics of the 4096^2 state (kernel time from HIP events, 32 B per
     interior cell read) against a host copy of b, h, hu, hv; and the gauge recorder's
@@ -416,6 +502,12 @@ def main():
     ap.add_argument("--inversion-iters", type=int, default=0, help="L-BFGS iterations of the config 4 inversion")
     ap.add_argument("--sphere-steps", type=int, default=100, help="0 skips the spherical ensemble")
     ap.add_argument("--sphere-adjoint-steps", type=int, default=100)
+    ap.add_argument("--sphere-adjoint-hours", type=float, default=0.0,
+                    help="simulated hours of the friction-calibration adjoint on sphere member 0 (0 skips it)")
+    ap.add_argument("--sphere-adjoint-fd", type=int, default=8, help="its finite-difference directions")
+    ap.add_argument("--sphere-adjoint-fd-steps", default="0.1,0.01", help="their steps, comma-separated")
+    ap.add_argument("--only-sphere-adjoint", action="store_true",
+                    help="run only the --sphere-adjoint-hours leg and print its JSON")
     ap.add_argument("--no-diagnostics", action="store_true", help="skip the device diagnostics / recorder measurement")
     ap.add_argument("--globe-deg", type=float, default=1.0,
                     help="grid spacing of the n = 2 mode-period measurement (globe harness); 0 skips it")
@@ -426,6 +518,12 @@ def main():
     import swe_amd
     import workloads
 
+    if args.only_sphere_adjoint:
+        steps = tuple(float(x) for x in args.sphere_adjoint_fd_steps.split(","))
+        print(json.dumps({"sphere_adjoint_long": sphere_adjoint_long(args.sphere_adjoint_hours or 0.5,
+                                                                     args.sphere_adjoint_fd, steps=steps)}),
+              flush=True)
+        return
     log("[bench] building 4096x4096 workload")
     st, p = workloads.large_grid()
     nx, ny = p["nx"], p["ny"]
@@ -523,6 +621,10 @@ def main():
         out["inversion"] = inversion_measure(args.inversion_iters)
     if args.sphere_steps > 0:
         out["sphere"] = sphere_measure(args.sphere_steps, args.sphere_adjoint_steps)
+    if args.sphere_adjoint_hours > 0:
+        out["sphere_adjoint_long"] = sphere_adjoint_long(
+            args.sphere_adjoint_hours, args.sphere_adjoint_fd,
+            steps=tuple(float(x) for x in args.sphere_adjoint_fd_steps.split(",")))
     if args.globe_deg > 0:
         import globe
         log(f"[bench] globe: n = 2 mode period at {args.globe_deg} deg")

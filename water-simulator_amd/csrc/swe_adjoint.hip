@@ -27,6 +27,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -410,6 +412,20 @@ int step_rev(swe_adj *a, int buf, int64_t q, int64_t n) {
   return SWE_OK;
 }
 
+// Progress lines for long runs: with SWE_ADJ_PROGRESS set in the environment the
+// forward and the backward report on stderr every twentieth of their work,
+// after waiting for the device to get there (so a report means work done).
+int64_t progress_stride(int64_t total) {
+  static const bool on = std::getenv("SWE_ADJ_PROGRESS") != nullptr;
+  return on ? std::max<int64_t>(1, total / 20) : 0;
+}
+
+void progress(hipStream_t st, const char *what, int64_t done, int64_t total) {
+  (void)hipStreamSynchronize(st);
+  std::fprintf(stderr, "[swe_adj] %s: %lld of %lld\n", what, (long long)done, (long long)total);
+  std::fflush(stderr);
+}
+
 int set_initial(swe_adj *a, const double *h0, const double *nfield) {
   swe_ctx *c = a->ctx;
   std::vector<double> z(a->N, 0.0);
@@ -443,7 +459,9 @@ int forward(swe_adj *a, bool record, double *loss) {
     ACHK(a, launch_maxdt(c->P, sp(cur_state(c)), c->dctl, c->stream));
     ACHK(a, launch_finalize(c->dctl, c->mf_part, 0, c->P.cfl, 0, c->stream));
   }
+  const int64_t pstride = progress_stride(a->nsteps);
   for (int64_t n = 0; n < a->nsteps; ++n) {
+    if (pstride && n > 0 && n % pstride == 0) progress(c->stream, record ? "forward steps" : "replayed steps", n, a->nsteps);
     if (record && n % a->KO == 0)
       if ((rc = copy3(a, st3(a->ckpt + (size_t)(n / a->KO) * N3, a->N), sp(cur_state(c))))) return rc;
     if (record) {
@@ -730,7 +748,9 @@ int swe_adj_backward(swe_adj *a, double *grad_h0, double *grad_n) {
   ACHK(a, hipStreamWaitEvent(a->rs, a->reversed[0], 0));
   if ((rc = issue_replay(0))) return rc;
   ACHK(a, hipEventRecord(a->replayed[0], a->rs));
+  const int64_t pstride = progress_stride((int64_t)segs.size());
   for (size_t i = 0; i < segs.size(); ++i) {
+    if (pstride && i > 0 && (int64_t)i % pstride == 0) progress(st, "backward segments", (int64_t)i, (int64_t)segs.size());
     const int buf = (int)(i % 2);
     ACHK(a, hipStreamWaitEvent(st, a->replayed[buf], 0));
     if (i + 1 < segs.size()) {

@@ -15,33 +15,39 @@ namespace swe {
 constexpr double SQRT2 = 1.4142135623730951;   // Math.SQRT2
 
 // ---- JS Math semantics ----------------------------------------------------
+__host__ __device__ __forceinline__ uint64_t dbits(double x) { return __builtin_bit_cast(uint64_t, x); }
+__host__ __device__ __forceinline__ double dfrom(uint64_t u) { return __builtin_bit_cast(double, u); }
 // Math.max(0, x): NaN propagates, -0 becomes +0
 __host__ __device__ __forceinline__ double jmax0(double x) { return (x > 0.0 || x != x) ? x : 0.0; }
+// Math.max(a, b) / Math.min(a, b), branch-free: NaN (the canonical quiet NaN)
+// when either is NaN; of two equal values -- +0 and -0 among them -- the one
+// whose sign bit is the AND (max) or the OR (min) of theirs, so max(+0, -0) =
+// +0 and min(+0, -0) = -0; otherwise the hardware max / min, which then only
+// decides between two distinct ordered values.
 __host__ __device__ __forceinline__ double jmax(double a, double b) {
-  if (a != a || b != b) return __builtin_nan("");
-  if (a > b) return a;
-  if (b > a) return b;
-  if (a == 0.0) return (__builtin_signbit(a) && __builtin_signbit(b)) ? -0.0 : 0.0;
-  return a;
+  const double m = a == b ? dfrom(dbits(a) & dbits(b)) : __builtin_fmax(a, b);
+  return (a != a || b != b) ? __builtin_nan("") : m;
 }
 __host__ __device__ __forceinline__ double jmin(double a, double b) {
-  if (a != a || b != b) return __builtin_nan("");
-  if (a < b) return a;
-  if (b < a) return b;
-  if (a == 0.0) return (__builtin_signbit(a) || __builtin_signbit(b)) ? -0.0 : 0.0;
-  return a;
+  const double m = a == b ? dfrom(dbits(a) | dbits(b)) : __builtin_fmin(a, b);
+  return (a != a || b != b) ? __builtin_nan("") : m;
+}
+// Math.min of two values known to be >= +0 or NaN (no -0): the hardware min,
+// NaN when either is NaN
+__host__ __device__ __forceinline__ double jmin_nn(double a, double b) {
+  return (a != a || b != b) ? __builtin_nan("") : __builtin_fmin(a, b);
 }
 
 // ---- exact division helpers -----------------------------------------------
 // x / d given r = RN(1/d): q = RN(x r); e = x - q d (exact by FMA); RN(q + e r)
 // is RN(x/d) for d, x, x/d normal (Markstein).  A zero quotient keeps the sign
-// of x*r (what x/d gives); an infinite d (a pole face) takes the IEEE divide.
+// of x*r (what x/d gives).  An infinite d (a pole face) has r = 0, where x * r
+// is x / d itself (a signed zero, or NaN for an infinite or NaN x).  Branch-free.
 __host__ __device__ __forceinline__ double div_r(double x, double d, double r) {
-  if (r == 0.0) return x / d;
   const double q = x * r;
   const double e = __builtin_fma(-q, d, x);
   const double q1 = __builtin_fma(e, r, q);
-  return q1 == 0.0 ? q : q1;
+  return (q1 == 0.0 || r == 0.0) ? q : q1;
 }
 
 // ---- vel(): swe.mjs:282-285.  Both components share one denominator. --------
@@ -67,10 +73,10 @@ __host__ __device__ __forceinline__ double vel1(double hq, double h, double eps4
 enum { LIM_MINMOD = 0, LIM_MC = 1, LIM_CENTRAL = 2 };
 __host__ __device__ __forceinline__ double limit(int lim, double a, double b) {
   if (lim == LIM_MC) {
-    if (a * b <= 0) return 0.0;
+    // the three candidates are absolute values: >= +0 or NaN
     const double c = 0.5 * (a + b);
-    const double m = jmin(jmin(__builtin_fabs(c), 2 * __builtin_fabs(a)), 2 * __builtin_fabs(b));
-    return a > 0 ? m : -m;
+    const double m = jmin_nn(jmin_nn(__builtin_fabs(c), 2 * __builtin_fabs(a)), 2 * __builtin_fabs(b));
+    return a * b <= 0 ? 0.0 : (a > 0 ? m : -m);
   }
   if (lim == LIM_MINMOD) {
     if (a > 0 && b > 0) return a < b ? a : b;

@@ -326,7 +326,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
       const double thE = shdn(th);
       const double the = fxPrev.M >= 0 ? th : thE;
       double inv = 0.0;
-      if (q >= 0 && q < H) inv = (the == 1.0) ? rowv(RDX, q) : the / rowv(DX, q);
+      if (q >= 0 && q < H) inv = (the == 1.0) ? rowv(RDX, q) : div_r(the, rowv(DX, q), rowv(RDX, q));
       const double eMc = fxPrev.M * inv, eNL = fxPrev.NL * inv, eNR = fxPrev.NR * inv, eT = fxPrev.T * inv;
       const double wMc = shup(eMc), wNR = shup(eNR), wT = shup(eT);
       const bool rowX = q >= 2 && q <= H - 3;
@@ -340,8 +340,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
     const bool fyPrevOn = fyCol && q - 1 >= 1 && q - 1 <= H - 3;
     if (fyPrevOn) {
       const double thy = fyPrev.M >= 0 ? thPrev : th;
-      const double invL = (thy == 1.0) ? rowv(RDYN, q - 1) : thy / rowv(DYN, q - 1);
-      const double invR = (thy == 1.0) ? rowv(RDYS, q) : thy / rowv(DYS, q);
+      const double invL = (thy == 1.0) ? rowv(RDYN, q - 1) : div_r(thy, rowv(DYN, q - 1), rowv(RDYN, q - 1));
+      const double invR = (thy == 1.0) ? rowv(RDYS, q) : div_r(thy, rowv(DYS, q), rowv(RDYS, q));
       yn.h = fyPrev.M * invL; yn.hu = fyPrev.T * invL; yn.hv = fyPrev.NL * invL;
       ysCur.h = fyPrev.M * invR; ysCur.hu = fyPrev.T * invR; ysCur.hv = fyPrev.NR * invR;
     }

@@ -157,8 +157,9 @@ __device__ __forceinline__ FluxF face_flux(const Face4 &L, const Face4 &R, const
 // One wavefront's share of a stage: strip `strip` (58 output columns) and rows
 // [chunk*rows_per_chunk, +rows_per_chunk) of the basin described by A.  wave_id
 // indexes the massFloored partial this wavefront owns.
-// REC: compile in the optional recording for the adjoint replay (A.recf)
-template <int MODE, bool REC>
+// REC: compile in the optional recording for the adjoint replay (A.recf);
+// LIM: the slope limiter (Params::limiter), fixed at compile time
+template <int MODE, bool REC, int LIM>
 __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int chunk, int wave_id) {
   const Params &P = A.P;
   const int W = P.W, H = P.H;
@@ -242,16 +243,16 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
     double sEx = 0.0, sBx = 0.0, sUx = 0.0, sVx = 0.0, sEy = 0.0, sBy = 0.0, sUy = 0.0, sVy = 0.0;
     if (o2 && slopeCol && s >= 1 && s <= H - 2) {
       if (!(c0.h <= md || hE <= md || hW <= md)) {
-        sEx = limit(P.limiter, c0.eta - eW, eE - c0.eta);
-        sBx = limit(P.limiter, c0.b - bW, bE - c0.b);
-        sUx = limit(P.limiter, c0.U - UW, UE - c0.U);
-        sVx = limit(P.limiter, c0.V - VW, VE - c0.V);
+        sEx = limit(LIM, c0.eta - eW, eE - c0.eta);
+        sBx = limit(LIM, c0.b - bW, bE - c0.b);
+        sUx = limit(LIM, c0.U - UW, UE - c0.U);
+        sVx = limit(LIM, c0.V - VW, VE - c0.V);
       }
       if (!(c0.h <= md || cP.h <= md || cM.h <= md)) {
-        sEy = limit(P.limiter, c0.eta - cM.eta, cP.eta - c0.eta);
-        sBy = limit(P.limiter, c0.b - cM.b, cP.b - c0.b);
-        sUy = limit(P.limiter, c0.U - cM.U, cP.U - c0.U);
-        sVy = limit(P.limiter, c0.V - cM.V, cP.V - c0.V);
+        sEy = limit(LIM, c0.eta - cM.eta, cP.eta - c0.eta);
+        sBy = limit(LIM, c0.b - cM.b, cP.b - c0.b);
+        sUy = limit(LIM, c0.U - cM.U, cP.U - c0.U);
+        sVy = limit(LIM, c0.V - cM.V, cP.V - c0.V);
       }
     }
     if (REC && A.recf && outLane && s >= y0 && s < y1) {
@@ -471,9 +472,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
   }
 }
 
-template <int MODE>
+template <int MODE, int LIM, bool REC>
 __global__ __launch_bounds__(SWEEP_LANES) void k_sweep(SweepArgs A) {
-  sweep_body<MODE, true>(A, blockIdx.x, blockIdx.y, blockIdx.y * gridDim.x + blockIdx.x);
+  sweep_body<MODE, REC, LIM>(A, blockIdx.x, blockIdx.y, blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 // Batched ensembles: a persistent grid of wavefronts pulls (basin, strip, chunk)
@@ -493,7 +494,11 @@ __global__ __launch_bounds__(SWEEP_LANES, 3) void k_sweep_batch(const BasinDev *
     const int strip = __builtin_amdgcn_readfirstlane(tasks[t].strip);
     const int chunk = __builtin_amdgcn_readfirstlane(tasks[t].chunk);
     const int wave = __builtin_amdgcn_readfirstlane(tasks[t].wave);
-    sweep_body<MODE, false>(basins[basin].sw[parity][MODE - 1], strip, chunk, wave);
+    const SweepArgs &A = basins[basin].sw[parity][MODE - 1];
+    const int lim = __builtin_amdgcn_readfirstlane(A.P.limiter);
+    if (lim == LIM_MC) sweep_body<MODE, false, LIM_MC>(A, strip, chunk, wave);
+    else if (lim == LIM_MINMOD) sweep_body<MODE, false, LIM_MINMOD>(A, strip, chunk, wave);
+    else sweep_body<MODE, false, LIM_CENTRAL>(A, strip, chunk, wave);
   }
 }
 
@@ -760,16 +765,33 @@ __global__ void k_finalize_batch(const BasinDev *__restrict__ basins, int nb, un
 }
 
 // ---- launchers (host side, C++ linkage inside the library) ------------------
+// the stage kernel for a limiter; the recording variant only when the launch
+// records (the residual probe never does)
+template <int MODE, int LIM>
+static const void *sweep_fn(bool rec) {
+  if (MODE != MODE_PROBE && rec) return reinterpret_cast<const void *>(&k_sweep<MODE, LIM, MODE != MODE_PROBE>);
+  return reinterpret_cast<const void *>(&k_sweep<MODE, LIM, false>);
+}
+template <int MODE>
+static const void *sweep_fn(int lim, bool rec) {
+  if (lim == LIM_MINMOD) return sweep_fn<MODE, LIM_MINMOD>(rec);
+  if (lim == LIM_CENTRAL) return sweep_fn<MODE, LIM_CENTRAL>(rec);
+  return sweep_fn<MODE, LIM_MC>(rec);
+}
+
 hipError_t launch_sweep(int mode, const SweepArgs &a, hipStream_t st) {
   dim3 grid((a.P.W + SWEEP_OUT - 1) / SWEEP_OUT, (a.P.H + a.rows_per_chunk - 1) / a.rows_per_chunk);
   dim3 block(SWEEP_LANES);
-  if (mode == MODE_STAGE1) hipLaunchKernelGGL(k_sweep<MODE_STAGE1>, grid, block, 0, st, a);
-  else if (mode == MODE_STAGE2) hipLaunchKernelGGL(k_sweep<MODE_STAGE2>, grid, block, 0, st, a);
-  else hipLaunchKernelGGL(k_sweep<MODE_PROBE>, grid, block, 0, st, a);
-  return hipGetLastError();
+  const bool rec = a.recf != nullptr;
+  const void *fn = mode == MODE_STAGE1   ? sweep_fn<MODE_STAGE1>(a.P.limiter, rec)
+                   : mode == MODE_STAGE2 ? sweep_fn<MODE_STAGE2>(a.P.limiter, rec)
+                                         : sweep_fn<MODE_PROBE>(a.P.limiter, false);
+  SweepArgs arg = a;
+  void *args[] = {&arg};
+  return hipLaunchKernel(fn, grid, block, args, 0, st);
 }
 
-const void *k_sweep_stage1_ptr() { return reinterpret_cast<const void *>(&k_sweep<MODE_STAGE1>); }
+const void *k_sweep_stage1_ptr() { return sweep_fn<MODE_STAGE1>(LIM_MC, false); }
 
 int sweep_waves(const Params &P, int rows_per_chunk) {
   return ((P.W + SWEEP_OUT - 1) / SWEEP_OUT) * ((P.H + rows_per_chunk - 1) / rows_per_chunk);

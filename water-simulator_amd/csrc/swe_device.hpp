@@ -73,9 +73,12 @@ __host__ __device__ __forceinline__ double vel1(double hq, double h, double eps4
 enum { LIM_MINMOD = 0, LIM_MC = 1, LIM_CENTRAL = 2 };
 __host__ __device__ __forceinline__ double limit(int lim, double a, double b) {
   if (lim == LIM_MC) {
-    // the three candidates are absolute values: >= +0 or NaN
+    // min(|c|, 2|a|, 2|b|) as Math.min gives it, NaN included: the three are
+    // absolute values (no -0), so the hardware min picks the same double, and
+    // one of them is NaN where a result is used (a * b > 0) only if a or b is
     const double c = 0.5 * (a + b);
-    const double m = jmin_nn(jmin_nn(__builtin_fabs(c), 2 * __builtin_fabs(a)), 2 * __builtin_fabs(b));
+    const double m0 = __builtin_fmin(__builtin_fmin(__builtin_fabs(c), 2 * __builtin_fabs(a)), 2 * __builtin_fabs(b));
+    const double m = (a != a || b != b) ? __builtin_nan("") : m0;
     return a * b <= 0 ? 0.0 : (a > 0 ? m : -m);
   }
   if (lim == LIM_MINMOD) {
@@ -90,6 +93,10 @@ __host__ __device__ __forceinline__ double limit(int lim, double a, double b) {
 struct Flux3 { double m, n, t; };
 struct HllcConst { double g, half_g, rg, dry; };   // rg = RN(1/g)
 
+// H_NONNEG: the caller guarantees hL, hR >= +0 or NaN (the hydrostatic
+// reconstruction's depths are), where Math.max(h, 0) is h itself wherever the
+// wave speed it feeds is used (the side is wet: h > dry)
+template <bool H_NONNEG = false>
 __host__ __device__ __forceinline__ Flux3 hllc(double hL, double huL, double hvL,
                                       double hR, double huR, double hvR, const HllcConst &C) {
   Flux3 out;
@@ -98,7 +105,8 @@ __host__ __device__ __forceinline__ Flux3 hllc(double hL, double huL, double hvL
   double uL = 0.0, vL = 0.0, uR = 0.0, vR = 0.0;
   if (wetL) { const double r = 1.0 / hL; uL = div_r(huL, hL, r); vL = div_r(hvL, hL, r); }
   if (wetR) { const double r = 1.0 / hR; uR = div_r(huR, hR, r); vR = div_r(hvR, hR, r); }
-  const double cL = __builtin_sqrt(C.g * jmax(hL, 0.0)), cR = __builtin_sqrt(C.g * jmax(hR, 0.0));
+  const double cL = __builtin_sqrt(C.g * (H_NONNEG ? hL : jmax(hL, 0.0)));
+  const double cR = __builtin_sqrt(C.g * (H_NONNEG ? hR : jmax(hR, 0.0)));
   double sL, sR;
   if (!wetL) { sL = uR - 2 * cR; sR = uR + cR; }
   else if (!wetR) { sL = uL - cL; sR = uL + 2 * cL; }

@@ -122,13 +122,14 @@ struct Part3 { double h, hu, hv; };
 #if SWE_USE_DPP
 // Whole-wavefront shift by one lane through DPP (wave_shl:1 = 0x130 brings lane
 // i+1 to lane i, wave_shr:1 = 0x138 brings lane i-1): two v_mov_b32_dpp per
-// double, no LDS round trip.  The end lane keeps its own value.
+// double, no LDS round trip.  The end lane, which has no source, reads zero
+// (bound_ctrl): it is a halo lane whose values reach no output lane.
 template <int CTRL>
 __device__ __forceinline__ double dpp_shift(double v) {
   const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
   const int lo = (int)(unsigned)u, hi = (int)(unsigned)(u >> 32);
-  const int rlo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
-  const int rhi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+  const int rlo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xf, 0xf, true);
+  const int rhi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xf, 0xf, true);
   return __builtin_bit_cast(double, ((unsigned long long)(unsigned)rhi << 32) | (unsigned)rlo);
 }
 __device__ __forceinline__ double shdn(double v) { return dpp_shift<0x130>(v); }
@@ -146,7 +147,7 @@ __device__ __forceinline__ FluxF face_flux(const Face4 &L, const Face4 &R, const
   const double bStar = jmax(L.b, R.b);
   const double hsL = jmax0(L.eta - bStar);
   const double hsR = jmax0(R.eta - bStar);
-  const Flux3 f = hllc(hsL, hsL * L.u, hsL * L.v, hsR, hsR * R.u, hsR * R.v, C);
+  const Flux3 f = hllc<true>(hsL, hsL * L.u, hsL * L.v, hsR, hsR * R.u, hsR * R.v, C);
   const double pL = half_g * (hLf * hLf - hsL * hsL);
   const double pR = half_g * (hRf * hRf - hsR * hsR);
   FluxF o;
@@ -194,8 +195,13 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
   const double md = P.min_depth;
   const bool o2 = P.order >= 2;
   const bool bedOn = o2 || P.sphere;
-  // the per-row values, interleaved per row (Params::geoA)
-  const double *RA = P.geoA;
+  // the per-row values, interleaved per row (Params::geoA).  They are
+  // wave-uniform and unchanged while a stage runs, so they are read through the
+  // constant address space: scalar loads into SGPRs, waited on by their own
+  // counter, instead of vector loads whose in-order completion holds up the
+  // wave's other memory traffic.
+  typedef const double __attribute__((address_space(4))) const_double;
+  const const_double *RA = (const const_double *)P.geoA;
   auto rowv = [&](int which, int j) -> double { return RA[j * GEO_AOS + which]; };
   enum { DX, RDX, DYN, RDYN, DYS, RDYS, DYCFL, RDYCFL, BED, RBED, TAN, FROW, GEOC };
 
@@ -273,8 +279,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
     if (bedOn && xin && s >= 2 && s <= H - 3) {
       const double hPx = jmax0(fE.eta - fE.b), hMx = jmax0(fWst.eta - fWst.b);
       const double hPy = jmax0(fN.eta - fN.b), hMy = jmax0(fS.eta - fS.b);
-      bedCur.hu = div_r(P.neg_half_g * (hPx + hMx) * sBx, rowv(DX, s), rowv(RDX, s));
-      bedCur.h = div_r(P.neg_half_g * (hPy + hMy) * sBy, rowv(BED, s), rowv(RBED, s));   // rhv bed term
+      bedCur.hu = div_r(-P.half_g * (hPx + hMx) * sBx, rowv(DX, s), rowv(RDX, s));
+      bedCur.h = div_r(-P.half_g * (hPy + hMy) * sBy, rowv(BED, s), rowv(RBED, s));   // rhv bed term
       bedCur.hv = P.sphere ? rowv(GEOC, s) * P.g * (hPy * hPy + hMy * hMy) : 0.0;        // rhv geometric term
     }
     // ---- y-face (s-1|s): v normal, u tangential (swe.mjs:448-475) ----------

@@ -13,7 +13,8 @@ longer runs:
   floored mass counted (swe.mjs:734-745);
 * the 4096x4096 benchmark workload itself (Flather tide, friction, Coriolis):
   bit-identical after 20 steps whatever the row-chunk height, and finite;
-* the 512-basin ensemble at its full size (65.6 M cells): sampled basins of the
+* the 512-basin ensemble at its full size (65.6 M cells) and the 16-member
+  2160x1080 sphere ensemble under the equilibrium tide: sampled basins of the
   batched run bit-identical to their plain runs.
 
 Left out of the still-lake grid: the check's surface-piercing island.  Tiled in
@@ -250,3 +251,36 @@ def test_sphere_2160x1080_mass_under_tide_and_resting_ocean():
     spd = np.hypot(hu2[wet] / h2[wet], hv2[wet] / h2[wet]).max()
     print(f"  sphere resting ocean, 100 steps: surface {dev:.2e} m, speed {spd:.2e} m/s")
     assert dev < 16 * np.spacing(6000.0) and spd < 1e-11
+
+
+def test_sphere_ensemble_16_members_equal_plain_runs():
+    # config 5 as timed: the 16-member ensemble on the 2160x1080 sphere under the
+    # M2/S2/K1 equilibrium tide, one batched launch per phase (with the batch
+    # autotuner's task shape); sampled members bit-identical to their plain runs
+    # after 20 steps (their own dt, t and Manning n)
+    b, h, p, manning = workloads.sphere_tides()
+    nx, ny = p["nx"], p["ny"]
+    rows = swe_amd.geometry_spherical(nx, ny, R=p["R"], lat0=p["lat0"], lat1=p["lat1"], omega=p["omega"])
+    lat, lon = swe_amd.sphere_lonlat(nx, ny, p["lat0"], p["lat1"])
+    z = np.zeros_like(h)
+    assert len(manning) == 16
+    B = swe_amd.Batch([dict(nx=nx, ny=ny, rows=rows, sphere=True, R=p["R"], manning=n, cfl=p["cfl"])
+                       for n in manning])
+    for q in range(16):
+        B.load_state(q, b, h, z, z)
+        B.set_tide(q, p["tides"], lat, lon)
+    B.autotune(steps=2)                      # restores every basin's state and clock
+    B.use_graph(True)
+    assert B.advance(20) == 20
+    for q in (0, 7, 15):
+        s = swe_amd.Solver(nx, ny, rows, sphere=True, R=p["R"], manning=manning[q], cfl=p["cfl"])
+        s.load_state(b, h, z, z)
+        s.set_tide(p["tides"], lat, lon)
+        assert s.advance(20) == 20
+        _, bh, bhu, bhv = B.state(q)
+        _, gh, ghu, ghv = s.state()
+        assert B.get_scalar(q, swe_amd.S_T) == s.t and s.t > 0
+        assert B.get_scalar(q, swe_amd.S_NONFINITE) == 0
+        assert np.array_equal(bh, gh) and np.array_equal(bhu, ghu) and np.array_equal(bhv, ghv), q
+        s.close()
+    B.close()

@@ -1,4 +1,4 @@
-// The metric rows the GPU step divides by: a restatement of src/geometry.mjs of
+// The metric rows the GPU step divides by: a transcription of src/geometry.mjs of
 // Syntaxswine/water-simulator (cartesianGeometry :163-176, sphericalGeometry
 // :190-299, cosLat :154-157, cellLonLat :312-315, interiorArea :325-329).
 //

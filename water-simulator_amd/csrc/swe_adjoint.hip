@@ -849,7 +849,7 @@ int swe_adj_host_epilogue_rev(const swe_desc *d, int stage, const double *pre_h,
 
 void swe_adj_host_hllc_rev(double out[6], double hL, double huL, double hvL, double hR, double huR, double hvR,
                            double dry, double g, double lm, double ln, double lt) {
-  const HllcConst C = {g, 0.5 * g, 1.0 / g, dry};
+  const HllcConst C = {g, 0.5 * g, 1.0 / g, dry, g * dry >= SQRT_N_MIN};
   const HllcIn r = hllc_rev(hL, huL, hvL, hR, huR, hvR, C, lm, ln, lt);
   out[0] = r.hL; out[1] = r.huL; out[2] = r.hvL; out[3] = r.hR; out[4] = r.huR; out[5] = r.hvR;
 }

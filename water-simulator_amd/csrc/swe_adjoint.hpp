@@ -248,7 +248,7 @@ __host__ __device__ inline FaceSide face_side(const Work &w, const double *b, in
 // the slots hold M, N + pL, N + pR, T
 __host__ __device__ inline void adj_faces_at(const Params &P, const double *b, const Work &w, int i, int j) {
   const int k = j * P.W + i;
-  const HllcConst HC = {P.g, P.half_g, P.rg, P.min_depth};
+  const HllcConst HC = {P.g, P.half_g, P.rg, P.min_depth, P.g * P.min_depth >= SQRT_N_MIN};
   double f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int d = 0; d < 2; ++d) {
     // both sides loaded before the branch (cell k stands in for a face off the grid)
@@ -369,7 +369,7 @@ __host__ __device__ inline void adj_accum_at(const Params &P, const StateP &S, d
 // east x-face and north y-face of cell (i, j)
 __host__ __device__ inline void adj_faces_rev_at(const Params &P, const double *b, const Work &w, int i, int j) {
   const int k = j * P.W + i;
-  const HllcConst HC = {P.g, P.half_g, P.rg, P.min_depth};
+  const HllcConst HC = {P.g, P.half_g, P.rg, P.min_depth, P.g * P.min_depth >= SQRT_N_MIN};
   const double *lout = w.s[S_LOUT];
   double r[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int d = 0; d < 2; ++d) {

@@ -287,6 +287,7 @@ int swe_batch_create(const swe_desc *descs, int nb, swe_batch **out) {
         dstA[(size_t)j * GEO_AOS] = dst[j];
       }
     }
+    set_uniform_rows(b->basins[i].P, host.data() + B.off_rowsA, B.P.H);
   }
   if ((e = hipMemcpyAsync(b->rows, host.data(), sizeof(double) * rpool, hipMemcpyHostToDevice, b->stream)) != hipSuccess)
     return bail("rows upload", e);

@@ -87,6 +87,7 @@ static int upload_row(swe_ctx *c, int row, const double *host) {
     if (src.empty()) continue;
     for (int j = 0; j < H; ++j) a[(size_t)j * GEO_AOS + k] = recip[k] ? 1.0 / src[j] : src[j];
   }
+  set_uniform_rows(c->P, a.data(), H);
   HIPCHK(c, hipMemcpyAsync(c->rowsA, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return SWE_OK;
@@ -122,7 +123,7 @@ double swe_js_pow(double x, double y) {
 
 void swe_hllc(double out[3], double hL, double huL, double hvL, double hR, double huR, double hvR,
               double dry, double g) {
-  const HllcConst C = {g, 0.5 * g, 1.0 / g, dry};
+  const HllcConst C = {g, 0.5 * g, 1.0 / g, dry, g * dry >= SQRT_N_MIN};
   const Flux3 f = swe::hllc(hL, huL, hvL, hR, huR, hvR, C);
   out[0] = f.m; out[1] = f.n; out[2] = f.t;
 }

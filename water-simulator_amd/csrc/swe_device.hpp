@@ -40,21 +40,61 @@ __host__ __device__ __forceinline__ double jmin_nn(double a, double b) {
 
 // ---- exact division helpers -----------------------------------------------
 // x / d given r = RN(1/d): q = RN(x r); e = x - q d (exact by FMA); RN(q + e r)
-// is RN(x/d) for d, x, x/d normal (Markstein).  A zero quotient keeps the sign
-// of x*r (what x/d gives).  An infinite d (a pole face) has r = 0, where x * r
-// is x / d itself (a signed zero, or NaN for an infinite or NaN x).  Branch-free.
+// is RN(x/d) for d, x, x/d normal (Markstein).  Where the corrected quotient is
+// zero or NaN, q = x * r is x / d itself: a zero quotient keeps the sign of x*r
+// (the correction would give +0 for x = -0); an infinite d (a pole face) has
+// r = 0 (q a signed zero, or NaN for an infinite or NaN x); an infinite x over
+// a finite d gives q = +-inf, and a NaN x or d gives NaN.  One compare on |q1|
+// (false for zero and NaN) and a select.  Branch-free.
 __host__ __device__ __forceinline__ double div_r(double x, double d, double r) {
   const double q = x * r;
   const double e = __builtin_fma(-q, d, x);
   const double q1 = __builtin_fma(e, r, q);
-  return (q1 == 0.0 || r == 0.0) ? q : q1;
+  return __builtin_fabs(q1) > 0.0 ? q1 : q;
 }
+
+// ---- square roots ----------------------------------------------------------
+// Math.sqrt (correctly rounded) of an x that is NaN, negative, +-0, +inf or at
+// least 2^-767.  Goldschmidt's iteration from the hardware reciprocal square
+// root estimate y ~ 1/sqrt(x): g = x y ~ sqrt(x) and h = y/2 ~ 1/(2 sqrt(x)),
+// one coupled step r = 1/2 - h g, g += g r, h += h r, then two Newton-Raphson
+// corrections g += (x - g^2) h with the residual exact by FMA.  These are the
+// steps the compiler's own sqrt takes once it has rescaled an x below 2^-767
+// (and rescaled back), so the two agree bit for bit wherever x needs no
+// rescaling; +-0 and +inf are returned as they are (the estimate is inf or 0
+// there), NaN and negative x give NaN through the estimate.
+__host__ __device__ __forceinline__ double sqrt_n(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y;
+  double h = y * 0.5;
+  const double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  return __builtin_amdgcn_class(x, 0x260) ? x : g;   // 0x260: -0, +0, +inf
+#else
+  return __builtin_sqrt(x);
+#endif
+}
+// sqrt(g h) for a depth h above minDepth: sqrt_n when g minDepth >= 2^-767
+// (`normal`, uniform over a launch), the full sqrt otherwise
+__host__ __device__ __forceinline__ double sqrt_gh(double x, bool normal) {
+  return normal ? sqrt_n(x) : __builtin_sqrt(x);
+}
+constexpr double SQRT_N_MIN = 0x1p-767;
 
 // ---- vel(): swe.mjs:282-285.  Both components share one denominator. --------
 struct Vel2 { double u, v; };
+// Math.max(h4, eps4) with eps4 > 0 is the hardware max: the two cannot be equal
+// zeros, and where h4 is NaN the sum is NaN whichever operand the max returns.
 __host__ __device__ __forceinline__ double vel_den(double h, double eps4) {
   const double h4 = h * h * h * h;
-  return __builtin_sqrt(h4 + jmax(h4, eps4));
+  // the argument is at least eps4 = max(minDepth, 1e-12)^4 >= 1e-48 (or NaN, +inf)
+  return sqrt_n(h4 + __builtin_fmax(h4, eps4));
 }
 __host__ __device__ __forceinline__ Vel2 vel2(double hu, double hv, double h, double eps4) {
   const double den = vel_den(h, eps4);
@@ -91,7 +131,9 @@ __host__ __device__ __forceinline__ double limit(int lim, double a, double b) {
 
 // ---- HLLC with Einfeldt speeds and dry states: swe.mjs:974-1018 -----------
 struct Flux3 { double m, n, t; };
-struct HllcConst { double g, half_g, rg, dry; };   // rg = RN(1/g)
+// rg = RN(1/g); normal: g * dry >= 2^-767, so sqrt(g h) of a wet depth h > dry
+// needs no rescaling (sqrt_gh)
+struct HllcConst { double g, half_g, rg, dry; bool normal; };
 
 // H_NONNEG: the caller guarantees hL, hR >= +0 or NaN (the hydrostatic
 // reconstruction's depths are), where Math.max(h, 0) is h itself wherever the
@@ -105,16 +147,18 @@ __host__ __device__ __forceinline__ Flux3 hllc(double hL, double huL, double hvL
   double uL = 0.0, vL = 0.0, uR = 0.0, vR = 0.0;
   if (wetL) { const double r = 1.0 / hL; uL = div_r(huL, hL, r); vL = div_r(hvL, hL, r); }
   if (wetR) { const double r = 1.0 / hR; uR = div_r(huR, hR, r); vR = div_r(hvR, hR, r); }
-  const double cL = __builtin_sqrt(C.g * (H_NONNEG ? hL : jmax(hL, 0.0)));
-  const double cR = __builtin_sqrt(C.g * (H_NONNEG ? hR : jmax(hR, 0.0)));
+  // cL is used only where the left side is wet (hL > dry), cR likewise
+  const double cL = sqrt_gh(C.g * (H_NONNEG ? hL : jmax(hL, 0.0)), C.normal);
+  const double cR = sqrt_gh(C.g * (H_NONNEG ? hR : jmax(hR, 0.0)), C.normal);
   double sL, sR;
   if (!wetL) { sL = uR - 2 * cR; sR = uR + cR; }
   else if (!wetR) { sL = uL - cL; sR = uL + 2 * cL; }
   else {
     const double a = (cL + cR) * 0.5 + (uL - uR) * 0.25;   // (cL+cR)/2 + (uL-uR)/4, exact halving
     const double hStar = div_r(a * a, C.g, C.rg);
-    const double qL = hStar > hL ? __builtin_sqrt(0.5 * (hStar + hL) * hStar / (hL * hL)) : 1.0;
-    const double qR = hStar > hR ? __builtin_sqrt(0.5 * (hStar + hR) * hStar / (hR * hR)) : 1.0;
+    // the arguments exceed 1 where the root is taken (hStar > h), or are NaN / +inf
+    const double qL = hStar > hL ? sqrt_n(0.5 * (hStar + hL) * hStar / (hL * hL)) : 1.0;
+    const double qR = hStar > hR ? sqrt_n(0.5 * (hStar + hR) * hStar / (hR * hR)) : 1.0;
     sL = uL - cL * qL;
     sR = uR + cR * qR;
   }
@@ -153,7 +197,7 @@ __host__ __device__ __forceinline__ double js_hypot(double a, double b) {
   summand = n1 * n1 - comp;
   pre = sum + summand;
   sum = pre;
-  return __builtin_sqrt(sum) * m;
+  return sqrt_n(sum) * m;   // sum is in [1, 2]
 }
 
 // ---- x**y for finite x > 0: fdlibm e_pow general path ----------------------

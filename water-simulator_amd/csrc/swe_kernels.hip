@@ -12,7 +12,9 @@
 //   cell needs stays in that lane's registers from one row to the next; along x
 //   the lanes trade neighbour values with cross-lane shuffles.  Nothing but the
 //   state is read from or written to HBM: slopes, face states, HLLC fluxes,
-//   positivity factors and residuals never leave registers.
+//   positivity factors and residuals never leave registers.  Each row's
+//   (b, h, hu, hv) is read from memory once: a four-row ring in LDS hands it
+//   to the cell update three rows later.
 //
 // Row pipeline of iteration r (row r is loaded):
 //   slopes + face states at s=r-1, y-face (s-1|s), x-faces at s,
@@ -144,7 +146,10 @@ __device__ __forceinline__ double shup(double v) { return __shfl_up(v, 1, SWEEP_
 __device__ __forceinline__ FluxF face_flux(const Face4 &L, const Face4 &R, const HllcConst &C, double half_g) {
   const double hLf = jmax0(L.eta - L.b);
   const double hRf = jmax0(R.eta - R.b);
-  const double bStar = jmax(L.b, R.b);
+  // Math.max(L.b, R.b): where the beds are equal zeros of either sign any of
+  // them serves, since bStar is only subtracted from a surface and the
+  // difference taken through Math.max(0, .), which maps -0 to +0
+  const double bStar = (L.b != L.b || R.b != R.b) ? __builtin_nan("") : __builtin_fmax(L.b, R.b);
   const double hsL = jmax0(L.eta - bStar);
   const double hsR = jmax0(R.eta - bStar);
   const Flux3 f = hllc<true>(hsL, hsL * L.u, hsL * L.v, hsR, hsR * R.u, hsR * R.v, C);
@@ -159,8 +164,9 @@ __device__ __forceinline__ FluxF face_flux(const Face4 &L, const Face4 &R, const
 // [chunk*rows_per_chunk, +rows_per_chunk) of the basin described by A.  wave_id
 // indexes the massFloored partial this wavefront owns.
 // REC: compile in the optional recording for the adjoint replay (A.recf);
-// LIM: the slope limiter (Params::limiter), fixed at compile time
-template <int MODE, bool REC, int LIM>
+// LIM: the slope limiter (Params::limiter), fixed at compile time;
+// UNI: every row's metric values are the same (Params::geo_uni), read once
+template <int MODE, bool REC, int LIM, bool UNI = false>
 __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int chunk, int wave_id) {
   const Params &P = A.P;
   const int W = P.W, H = P.H;
@@ -191,7 +197,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
   const bool posOn = dt > 0;
   const double *tcs = &ctl->tide_cs[(MODE == MODE_STAGE2 || (MODE == MODE_PROBE && P.probe_stage2)) ? 1 : 0][0][0];
   const double *b = A.b;
-  const HllcConst HC = {P.g, P.half_g, P.rg, P.min_depth};
+  const HllcConst HC = {P.g, P.half_g, P.rg, P.min_depth, P.g * P.min_depth >= SQRT_N_MIN};
   const double md = P.min_depth;
   const bool o2 = P.order >= 2;
   const bool bedOn = o2 || P.sphere;
@@ -202,16 +208,22 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
   // wave's other memory traffic.
   typedef const double __attribute__((address_space(4))) const_double;
   const const_double *RA = (const const_double *)P.geoA;
-  auto rowv = [&](int which, int j) -> double { return RA[j * GEO_AOS + which]; };
+  // (with UNI every row equals row 2, whose values are loaded once before the row loop)
+  auto rowv = [&](int which, int j) -> double { return RA[(UNI ? 2 : j) * GEO_AOS + which]; };
   enum { DX, RDX, DYN, RDYN, DYS, RDYS, DYCFL, RDYCFL, BED, RBED, TAN, FROW, GEOC };
 
+  // the last four rows' (b, h, hu, hv) of this wavefront's columns: the cell
+  // update of row r-3 reads its own cell from here instead of from memory
+  __shared__ double ring[4][4][SWEEP_LANES];
   auto load_row = [&](int r) -> Row {
     Row c;
     if (r >= 0 && r < H && colIn) {
       const int k = r * W + gi;
-      const double xh = inp[k], xb = b[k];
+      const double xh = inp[k], xb = b[k], xhu = inp[k + fs], xhv = inp[k + 2 * fs];
+      ring[r & 3][0][lane] = xb; ring[r & 3][1][lane] = xh;
+      ring[r & 3][2][lane] = xhu; ring[r & 3][3][lane] = xhv;
       c.h = xh; c.b = xb; c.eta = xb + xh;
-      const Vel2 uv = vel2(inp[k + fs], inp[k + 2 * fs], xh, P.eps4);
+      const Vel2 uv = vel2(xhu, xhv, xh, P.eps4);
       c.U = uv.u; c.V = uv.v;
     } else {
       c.h = 0.0; c.b = 0.0; c.eta = 0.0; c.U = 0.0; c.V = 0.0;
@@ -362,7 +374,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
       if (fySouth) { rh += ysPrev.h; rhu += ysPrev.hu; rhv += ysPrev.hv; }
       if (fyPrevOn) { rh -= yn.h; rhu -= yn.hu; rhv -= yn.hv; }
       const bool interior = xin && o >= 2 && o <= H - 3;
-      const Cell cur = {b[k], inp[k], inp[k + fs], inp[k + 2 * fs]};
+      const Cell cur = {ring[o & 3][0][lane], ring[o & 3][1][lane], ring[o & 3][2][lane], ring[o & 3][3][lane]};
       if (interior) {
         if (bedOn) {
           rhu += bedPrev2.hu;
@@ -437,7 +449,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
           }
           // next step's maxDt (swe.mjs:632-647)
           if (nh > md) {
-            const double c = __builtin_sqrt(P.g * nh);
+            const double c = sqrt_gh(P.g * nh, HC.normal);   // nh > minDepth
             const Vel2 uv = vel2(nhu, nhv, nh, P.eps4);
             const double sp = div_r(__builtin_fabs(uv.u) + c, rowv(DX, o), rowv(RDX, o)) +
                               div_r(__builtin_fabs(uv.v) + c, rowv(DYCFL, o), rowv(RDYCFL, o));
@@ -478,9 +490,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
   }
 }
 
-template <int MODE, int LIM, bool REC>
+template <int MODE, int LIM, bool REC, bool UNI>
 __global__ __launch_bounds__(SWEEP_LANES) void k_sweep(SweepArgs A) {
-  sweep_body<MODE, REC, LIM>(A, blockIdx.x, blockIdx.y, blockIdx.y * gridDim.x + blockIdx.x);
+  sweep_body<MODE, REC, LIM, UNI>(A, blockIdx.x, blockIdx.y, blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 // Batched ensembles: a persistent grid of wavefronts pulls (basin, strip, chunk)
@@ -772,32 +784,34 @@ __global__ void k_finalize_batch(const BasinDev *__restrict__ basins, int nb, un
 
 // ---- launchers (host side, C++ linkage inside the library) ------------------
 // the stage kernel for a limiter; the recording variant only when the launch
-// records (the residual probe never does)
+// records (the residual probe never does), the uniform-row variant for grids
+// whose rows are all alike when it does not
 template <int MODE, int LIM>
-static const void *sweep_fn(bool rec) {
-  if (MODE != MODE_PROBE && rec) return reinterpret_cast<const void *>(&k_sweep<MODE, LIM, MODE != MODE_PROBE>);
-  return reinterpret_cast<const void *>(&k_sweep<MODE, LIM, false>);
+static const void *sweep_fn(bool rec, bool uni) {
+  if (MODE != MODE_PROBE && rec) return reinterpret_cast<const void *>(&k_sweep<MODE, LIM, MODE != MODE_PROBE, false>);
+  if (uni) return reinterpret_cast<const void *>(&k_sweep<MODE, LIM, false, true>);
+  return reinterpret_cast<const void *>(&k_sweep<MODE, LIM, false, false>);
 }
 template <int MODE>
-static const void *sweep_fn(int lim, bool rec) {
-  if (lim == LIM_MINMOD) return sweep_fn<MODE, LIM_MINMOD>(rec);
-  if (lim == LIM_CENTRAL) return sweep_fn<MODE, LIM_CENTRAL>(rec);
-  return sweep_fn<MODE, LIM_MC>(rec);
+static const void *sweep_fn(int lim, bool rec, bool uni) {
+  if (lim == LIM_MINMOD) return sweep_fn<MODE, LIM_MINMOD>(rec, uni);
+  if (lim == LIM_CENTRAL) return sweep_fn<MODE, LIM_CENTRAL>(rec, uni);
+  return sweep_fn<MODE, LIM_MC>(rec, uni);
 }
 
 hipError_t launch_sweep(int mode, const SweepArgs &a, hipStream_t st) {
   dim3 grid((a.P.W + SWEEP_OUT - 1) / SWEEP_OUT, (a.P.H + a.rows_per_chunk - 1) / a.rows_per_chunk);
   dim3 block(SWEEP_LANES);
-  const bool rec = a.recf != nullptr;
-  const void *fn = mode == MODE_STAGE1   ? sweep_fn<MODE_STAGE1>(a.P.limiter, rec)
-                   : mode == MODE_STAGE2 ? sweep_fn<MODE_STAGE2>(a.P.limiter, rec)
-                                         : sweep_fn<MODE_PROBE>(a.P.limiter, false);
+  const bool rec = a.recf != nullptr, uni = a.P.geo_uni != 0;
+  const void *fn = mode == MODE_STAGE1   ? sweep_fn<MODE_STAGE1>(a.P.limiter, rec, uni)
+                   : mode == MODE_STAGE2 ? sweep_fn<MODE_STAGE2>(a.P.limiter, rec, uni)
+                                         : sweep_fn<MODE_PROBE>(a.P.limiter, false, uni);
   SweepArgs arg = a;
   void *args[] = {&arg};
   return hipLaunchKernel(fn, grid, block, args, 0, st);
 }
 
-const void *k_sweep_stage1_ptr() { return sweep_fn<MODE_STAGE1>(LIM_MC, false); }
+const void *k_sweep_stage1_ptr() { return sweep_fn<MODE_STAGE1>(LIM_MC, false, false); }
 
 int sweep_waves(const Params &P, int rows_per_chunk) {
   return ((P.W + SWEEP_OUT - 1) / SWEEP_OUT) * ((P.H + rows_per_chunk - 1) / rows_per_chunk);

@@ -103,8 +103,19 @@ struct Params {
   // values are one scalar base plus immediate offsets (no per-array offsets
   // held in scalar registers across the row loop).
   const double *geoA;
+  // geo_uni: every row of geoA holds the same values (a Cartesian grid): the
+  // stage kernels then read them once per launch instead of once per row
+  int geo_uni, geo_pad;
 };
 constexpr int GEO_AOS = 16;
+
+// Params::geo_uni from the interleaved rows a[j * GEO_AOS + k], j < H
+inline void set_uniform_rows(Params &P, const double *a, int H) {
+  bool uni = true;
+  for (int j = 1; j < H && uni; ++j)
+    if (__builtin_memcmp(&a[(size_t)j * GEO_AOS], a, sizeof(double) * GEO_AOS) != 0) uni = false;
+  P.geo_uni = uni ? 1 : 0;
+}
 
 struct StateP { const double *h, *hu, *hv; };
 struct StateW { double *h, *hu, *hv; };

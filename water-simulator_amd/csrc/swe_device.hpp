@@ -177,6 +177,11 @@ __host__ __device__ __forceinline__ Flux3 hllc(double hL, double huL, double hvL
 }
 
 // ---- Math.hypot (2 args), V8's normalise-by-max Kahan form -----------------
+// V8 sums (x/m)^2 and (y/m)^2, m the larger magnitude, with Kahan's
+// compensation.  For two values the compensation after the first term is
+// exactly 0, one quotient is m/m = 1 exactly, and IEEE addition commutes, so
+// the sum is 1 + q^2 with q the smaller magnitude over the larger: one
+// division where the loop takes two, and the same double.
 __host__ __device__ __forceinline__ double js_hypot(double a, double b) {
   if (a != a || b != b) {
     if (__builtin_isinf(a) || __builtin_isinf(b)) return __builtin_inf();
@@ -188,16 +193,8 @@ __host__ __device__ __forceinline__ double js_hypot(double a, double b) {
   if (y > m) m = y;
   if (m == __builtin_inf()) return __builtin_inf();
   if (m == 0.0) return 0.0;
-  const double n0 = x / m, n1 = y / m;
-  double sum = 0.0, comp = 0.0;
-  double summand = n0 * n0 - comp;
-  double pre = sum + summand;
-  comp = (pre - sum) - summand;
-  sum = pre;
-  summand = n1 * n1 - comp;
-  pre = sum + summand;
-  sum = pre;
-  return sqrt_n(sum) * m;   // sum is in [1, 2]
+  const double q = (x > y ? y : x) / m;
+  return sqrt_n(1.0 + q * q) * m;   // the sum is in [1, 2]
 }
 
 // ---- x**y for finite x > 0: fdlibm e_pow general path ----------------------
@@ -208,9 +205,12 @@ __host__ __device__ __forceinline__ double mkd(uint32_t hi, uint32_t lo) {
 }
 __host__ __device__ __forceinline__ double zlo(double x) { return mkd(hiw(x), 0u); }
 
-// Also compiled for the host: the C ABI exports it (swe_js_pow) so host code can
-// form eps4 = max(minDepth,1e-12)**4 the way the JS engine does.
-static __host__ __device__ __noinline__ double js_pow_pos(double x, double y) {
+// js_pow_core is inlined where y is a constant (the friction's 4/3, whose
+// split into high and low parts then folds at compile time); js_pow_pos is the
+// out-of-line form.  Also compiled for the host: the C ABI exports it
+// (swe_js_pow) so host code can form eps4 = max(minDepth,1e-12)**4 the way the
+// JS engine does.
+__host__ __device__ __forceinline__ double js_pow_core(double x, double y) {
   const double two53 = 9007199254740992.0,
       L1 = 5.99999999999994648725e-01, L2 = 4.28571428578550184252e-01,
       L3 = 3.33333329818377432918e-01, L4 = 2.72728123808534006489e-01,
@@ -290,5 +290,9 @@ static __host__ __device__ __noinline__ double js_pow_pos(double x, double y) {
   else z = mkd((uint32_t)j, low(z));
   return z;
 }
+static __host__ __device__ __noinline__ double js_pow_pos(double x, double y) { return js_pow_core(x, y); }
+// x**(4/3) (Manning's depth power), out of line: inlined, its registers would
+// add to the stage kernel's peak
+static __device__ __noinline__ double js_pow43(double x) { return js_pow_core(x, 4.0 / 3.0); }
 
 }  // namespace swe

@@ -429,7 +429,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
                 const Vel2 uv = vel2(nhu, nhv, nh, P.eps4);
                 const double speed = js_hypot(uv.u, uv.v);
                 if (!(speed < 1e-14)) {
-                  const double denom = 1 + dt * P.g * n * n * speed / js_pow_pos(nh, 4.0 / 3.0);
+                  const double denom = 1 + dt * P.g * n * n * speed / js_pow43(nh);
                   nhu = nhu / denom;
                   nhv = nhv / denom;
                 }

@@ -165,9 +165,14 @@ __device__ __forceinline__ FluxF face_flux(const Face4 &L, const Face4 &R, const
 // indexes the massFloored partial this wavefront owns.
 // REC: compile in the optional recording for the adjoint replay (A.recf);
 // LIM: the slope limiter (Params::limiter), fixed at compile time;
-// UNI: every row's metric values are the same (Params::geo_uni), read once
+// UNI: every row's metric values are the same (Params::geo_uni), read once.
+// sweep_ring: the wavefront's LDS (one per workgroup of one wavefront, however
+// many bodies a kernel instantiates): the last four rows' (b, h, hu, hv) of its
+// columns, which the cell update of row r-3 reads instead of memory
+__shared__ double sweep_ring[4][4][SWEEP_LANES];
 template <int MODE, bool REC, int LIM, bool UNI = false>
 __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int chunk, int wave_id) {
+  double (&ring)[4][4][SWEEP_LANES] = sweep_ring;
   const Params &P = A.P;
   const int W = P.W, H = P.H;
   const int lane = threadIdx.x & (SWEEP_LANES - 1);
@@ -212,9 +217,6 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
   auto rowv = [&](int which, int j) -> double { return RA[(UNI ? 2 : j) * GEO_AOS + which]; };
   enum { DX, RDX, DYN, RDYN, DYS, RDYS, DYCFL, RDYCFL, BED, RBED, TAN, FROW, GEOC };
 
-  // the last four rows' (b, h, hu, hv) of this wavefront's columns: the cell
-  // update of row r-3 reads its own cell from here instead of from memory
-  __shared__ double ring[4][4][SWEEP_LANES];
   auto load_row = [&](int r) -> Row {
     Row c;
     if (r >= 0 && r < H && colIn) {

@@ -88,15 +88,19 @@ __global__ void k_fill(size_t n, double *a, double v) {
 // factors follow, and are also kept in tcs_out
 __global__ void k_ctl_dt(Ctl *ctl, const double *dts, const double *ts, const double *exts, int64_t n,
                          double *tcs_out) {
-  if (threadIdx.x != 0) return;
-  ctl->dt = dts[n];
-  if (ts) ctl->t = ts[n];
-  for (int q = 0; q < 8; ++q) ctl->ext[q] = exts[8 * n + q];
-  ctl->dt_mode = DT_HOST;
-  ctl->cfl_bits = 0ull;
-  tide_times(ctl);
-  if (tcs_out)
-    for (int q = 0; q < 2 * MAX_TIDE * 2; ++q) tcs_out[q] = (&ctl->tide_cs[0][0][0])[q];
+  const int lane = threadIdx.x;
+  const double dt = dts[n], t = ts ? ts[n] : ctl->t;
+  if (lane == 0) {
+    ctl->dt = dt;
+    if (ts) ctl->t = t;
+    for (int q = 0; q < 8; ++q) ctl->ext[q] = exts[8 * n + q];
+    ctl->dt_mode = DT_HOST;
+    ctl->cfl_bits = 0ull;
+  }
+  // the tide factors, one per lane; lane q's entry is tide_cs[s][c][f] with
+  // c = q / 4, s = (q / 2) & 1, f = q & 1
+  const double v = tide_times_lane(ctl, t, dt, lane);
+  if (tcs_out && lane < 4 * MAX_TIDE) tcs_out[((lane >> 1) & 1) * 2 * MAX_TIDE + (lane >> 2) * 2 + (lane & 1)] = v;
 }
 // self-timed steps from here on, from t = 0, the Flather harmonics evaluated
 // on the device when the context has them; the max-reduction starts from zero

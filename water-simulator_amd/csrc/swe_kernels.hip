@@ -630,7 +630,7 @@ hipError_t launch_dry_clean(const Params &P, StateW S, double *part, int blocks,
 }
 
 __global__ void k_tide_times(Ctl *ctl) {
-  if (threadIdx.x == 0) tide_times(ctl);
+  (void)tide_times_lane(ctl, ctl->t, ctl->dt, threadIdx.x);
 }
 
 hipError_t launch_tide_times(Ctl *ctl, hipStream_t st) {
@@ -655,9 +655,10 @@ __device__ __forceinline__ double harm_eval(int n, double c0, const double *amp,
 // Turn the max-reduction into a step size and set up the next step (one
 // wavefront per basin).
 //   phase 0: after k_maxdt -- dt for the step about to run
-//   phase 1: after a step -- massFloored, t, steps, dt_next, next dt and forcing
+//   phase 1: after a step -- massFloored, t, steps, dt_next and next dt
 // s1, s2: the two stages' massFloored sums (phase 1); the body runs on one lane.
-// rec: the step's gauge recorder is armed (device-resident runs only).
+// rec: the step's gauge recorder is armed (device-resident runs only).  The
+// step's forcing follows on the wavefront's lanes (forcing_lanes).
 __device__ __forceinline__ void finalize_ctl(Ctl *ctl, double s1, double s2, double cfl, int phase, bool rec = false) {
   const unsigned long long bits = ctl->cfl_bits;
   const double inv = __longlong_as_double((long long)bits);
@@ -696,15 +697,21 @@ __device__ __forceinline__ void finalize_ctl(Ctl *ctl, double s1, double s2, dou
     // at t >= T the step is refused (dt 0), which leaves the state as it is
     ctl->dt = ctl->t < ctl->t_end ? jmin(dtn, jmax(1e-6, jmin(ctl->t_end, ctl->rec_next) - ctl->t)) : 0.0;
   ctl->idle = (ctl->dt_mode == DT_SAMPLED || ctl->dt_mode == DT_UNTIL) && !(ctl->t < ctl->t_end);
-  tide_times(ctl);
-  if (ctl->harm_on) {
-    for (int side = 0; side < 4; ++side) {
-      const Harm &h = ctl->harm[side];
-      const double e = harm_eval(h.n_eta, h.msl, h.amp, h.omega, h.phase, ctl->t, h.ramp);
-      ctl->ext[2 * side] = e;
-      ctl->ext[2 * side + 1] = h.u_from_eta ? h.u_scale * (e - h.msl)
-                                            : harm_eval(h.n_u, h.u0, h.uamp, h.uomega, h.uphase, ctl->t, h.ramp);
-    }
+}
+
+// The forcing of the step about to run, after finalize_ctl has set t and dt
+// (passed in, the same on every lane): the tide factors on lanes 0-15, and the
+// harmonic Flather values on lanes 32-39, two per side (eta on the even lane, u
+// on the odd one, which evaluates eta as well when u follows it).
+__device__ __forceinline__ void forcing_lanes(Ctl *ctl, double t, double dt, int lane) {
+  (void)tide_times_lane(ctl, t, dt, lane);
+  if (ctl->harm_on && lane >= 32 && lane < 40) {
+    const int side = (lane - 32) >> 1;
+    const Harm &h = ctl->harm[side];
+    const double e = harm_eval(h.n_eta, h.msl, h.amp, h.omega, h.phase, t, h.ramp);
+    if ((lane & 1) == 0) ctl->ext[2 * side] = e;
+    else ctl->ext[2 * side + 1] = h.u_from_eta ? h.u_scale * (e - h.msl)
+                                               : harm_eval(h.n_u, h.u0, h.uamp, h.uomega, h.uphase, t, h.ramp);
   }
 }
 
@@ -718,6 +725,8 @@ __device__ __forceinline__ void finalize_body(Ctl *ctl, const double *mf_part, i
     for (int off = 32; off >= 1; off >>= 1) { s1 += __shfl_down(s1, off, 64); s2 += __shfl_down(s2, off, 64); }
   }
   if (lane == 0) finalize_ctl(ctl, s1, s2, cfl, phase);
+  __syncthreads();
+  forcing_lanes(ctl, ctl->t, ctl->dt, lane);
 }
 
 // One grid's partials (thousands per stage) by FIN_THREADS threads: strided
@@ -730,7 +739,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize(Ctl *ctl, const double
                                                           int phase, RecArgs r) {
   __shared__ double w1[FIN_THREADS / 64], w2[FIN_THREADS / 64];
   __shared__ long long rrow;
-  __shared__ double rt;
+  __shared__ double rt, rdt;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const bool rec = r.cells != nullptr && phase == 1;
   double s1 = 0.0, s2 = 0.0;
@@ -748,9 +757,11 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize(Ctl *ctl, const double
     finalize_ctl(ctl, s1, s2, cfl, phase, rec);
     rrow = (rec && ctl->rec_flag) ? ctl->rec_row : -1;
     rt = ctl->t;
+    rdt = ctl->dt;
   }
-  if (!rec) return;
   __syncthreads();
+  if (t < 64) forcing_lanes(ctl, rt, rdt, t);
+  if (!rec) return;
   const long long row = rrow;
   if (row < 0 || row >= r.cap) return;
   // tools/waves.mjs:225: sim.eta(i, j) = b + h

@@ -157,14 +157,18 @@ struct BasinDev {
 struct Task { int basin, strip, chunk, wave; };
 
 // (cos, sin)(omega t_s + phase) of every tide constituent at the stage times
-// t_s = t, t + dt of the step about to run
-__device__ __forceinline__ void tide_times(Ctl *ctl) {
-  for (int c = 0; c < ctl->tide_n; ++c) {
-    const double a0 = ctl->tide_om[c] * ctl->t + ctl->tide_ph[c];
-    const double a1 = ctl->tide_om[c] * (ctl->t + ctl->dt) + ctl->tide_ph[c];
-    ctl->tide_cs[0][c][0] = cos(a0); ctl->tide_cs[0][c][1] = sin(a0);
-    ctl->tide_cs[1][c][0] = cos(a1); ctl->tide_cs[1][c][1] = sin(a1);
+// t_s = t, t + dt of the step about to run (t, dt: the control block's, passed
+// in by the caller), one value per lane: lane q < 4 MAX_TIDE takes constituent
+// q / 4, stage time (q / 2) & 1, cos or sin by q & 1, and returns that entry
+// (whether or not the constituent is in use) for callers that keep a copy
+__device__ __forceinline__ double tide_times_lane(Ctl *ctl, double t, double dt, int lane) {
+  if (lane >= 4 * MAX_TIDE) return 0.0;
+  const int c = lane >> 2, s = (lane >> 1) & 1, f = lane & 1;
+  if (c < ctl->tide_n) {
+    const double a = s ? ctl->tide_om[c] * (t + dt) + ctl->tide_ph[c] : ctl->tide_om[c] * t + ctl->tide_ph[c];
+    ctl->tide_cs[s][c][f] = f ? sin(a) : cos(a);
   }
+  return ctl->tide_cs[s][c][f];
 }
 
 constexpr int REC_FIELDS = 21;      // fields of SweepArgs::recf

@@ -64,10 +64,57 @@ __global__ void k_adj_faces_rev(Params P, const double *b, Work w) {
   int i, j;
   if (cell_of(P, i, j)) adj_faces_rev_at(P, b, w, i, j);
 }
-__global__ void k_adj_cells_slopes_rev(Params P, StateP S, const double *b, StateP lR, Work w, const double *tcs,
-                                       StateW lS) {
-  int i, j;
-  if (cell_of(P, i, j)) adj_cells_slopes_rev_at(P, S, b, lR, w, tcs, lS, i, j);
+// Gather^T + limiter^T over 64 x 4 tiles of cells, one thread per cell: each
+// cell's gather (cells_rev) runs once and leaves its slope adjoints in LDS, the
+// tile's one-cell halo along each axis gets that axis's slope adjoints from
+// slope_adj (the same operations, so the same doubles), and limiter^T + vel^T
+// then read a neighbour's slope adjoints from LDS instead of recomputing them
+// from memory.  Tiles are dealt to the XCDs in bands of tile rows as cell_of
+// deals blocks.
+constexpr int CT_X = 64, CT_Y = 4;
+__global__ __launch_bounds__(CT_X * CT_Y) void k_adj_cells_slopes_tiled(Params P, StateP S, const double *b,
+                                                                       StateP lR, Work w, const double *tcs,
+                                                                       StateW lS) {
+  __shared__ double sx[CT_Y][CT_X + 2][3];   // x-slope adjoints (eta, U, V), columns i0 - 1 .. i0 + CT_X
+  __shared__ double sy[CT_Y + 2][CT_X][3];   // y-slope adjoints, rows j0 - 1 .. j0 + CT_Y
+  const int tiles_x = (P.W + CT_X - 1) / CT_X, tiles_y = (P.H + CT_Y - 1) / CT_Y;
+  const int per = gridDim.x / NXCD;
+  const int blk = (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
+  const int tx = blk % tiles_x, ty = blk / tiles_x;
+  const int t = threadIdx.x, lx = t % CT_X, ly = t / CT_X;
+  const int i0 = tx * CT_X, j0 = ty * CT_Y, i = i0 + lx, j = j0 + ly;
+  const bool tile = ty < tiles_y;
+  const bool own = tile && i < P.W && j < P.H;
+  CellRev c = {};
+  if (own) {
+    c = cells_rev(P, S, b, lR, w, tcs, i, j);
+    sx[ly][lx + 1][0] = c.lsE[0]; sx[ly][lx + 1][1] = c.lsU[0]; sx[ly][lx + 1][2] = c.lsV[0];
+    sy[ly + 1][lx][0] = c.lsE[1]; sy[ly + 1][lx][1] = c.lsU[1]; sy[ly + 1][lx][2] = c.lsV[1];
+  }
+  if (tile && t < 2 * CT_Y) {                           // x halo: row t / 2, west or east
+    const int r = t >> 1, east = t & 1, hi = east ? i0 + CT_X : i0 - 1, hj = j0 + r;
+    if (hi >= 0 && hi < P.W && hj < P.H) {
+      const SlopeAdj a = slope_adj(P, b, lR, w, hi, hj, 0);
+      double *q = sx[r][east ? CT_X + 1 : 0];
+      q[0] = a.e; q[1] = a.u; q[2] = a.v;
+    }
+  } else if (tile && t < 2 * CT_Y + 2 * CT_X) {         // y halo: column (t - 2 CT_Y) / 2, south or north
+    const int col = (t - 2 * CT_Y) >> 1, north = t & 1, hi = i0 + col, hj = north ? j0 + CT_Y : j0 - 1;
+    if (hi < P.W && hj >= 0 && hj < P.H) {
+      const SlopeAdj a = slope_adj(P, b, lR, w, hi, hj, 1);
+      double *q = sy[north ? CT_Y + 1 : 0][col];
+      q[0] = a.e; q[1] = a.u; q[2] = a.v;
+    }
+  }
+  __syncthreads();
+  if (!own) return;
+  const int k = j * P.W + i;
+  // limiter^T asks only for the slopes of grid cells that have one
+  slopes_rev_core(P, S, w, lS, i, j, c.leta, c.lU, c.lV, w.s[S_LHTH][k], c.lhuC, c.lhvC, [&](int d, int o) {
+    const double *q = d == 0 ? sx[ly][lx + 1 + o] : sy[ly + 1 + o][lx];
+    SlopeAdj r = {q[0], q[1], q[2]};
+    return r;
+  });
 }
 __global__ void k_adj_bc(Params P, StateW l, int pass, StateP S, const double *b, const double *ext) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -287,7 +334,9 @@ int residual_rev(swe_adj *a, const StateP &S, double dt, const StateP &lR, const
   for (int q = 0; q < REC_FIELDS; ++q) w.s[S_U + q] = rec + (size_t)q * a->N;
   hipLaunchKernelGGL(k_adj_accum, g, blk, 0, st, P, S, dt, lR, w);
   hipLaunchKernelGGL(k_adj_faces_rev, g, blk, 0, st, P, a->bed, w);
-  hipLaunchKernelGGL(k_adj_cells_slopes_rev, g, blk, 0, st, P, S, a->bed, lR, w, tcs, lS);
+  const size_t tiles = (size_t)((P.W + CT_X - 1) / CT_X) * ((P.H + CT_Y - 1) / CT_Y);
+  hipLaunchKernelGGL(k_adj_cells_slopes_tiled, dim3((unsigned)((tiles + NXCD - 1) / NXCD * NXCD)),
+                     dim3(CT_X * CT_Y), 0, st, P, S, a->bed, lR, w, tcs, lS);
   ACHK(a, hipGetLastError());
   return SWE_OK;
 }

@@ -477,8 +477,8 @@ __host__ __device__ inline void adj_cells_rev_at(const Params &P, const StateP &
 
 // The slope adjoints (eta, U, V) along axis d of cell (i, j) alone, in the same
 // operations and order as cells_rev, so bit-identical to its lsE[d], lsU[d],
-// lsV[d]: the fused device pass below evaluates them for a cell's neighbours
-// instead of reading them back from memory.
+// lsV[d]: the device's tiled gather + limiter^T pass evaluates them for the
+// cells on its tiles' edges instead of reading them back from memory.
 struct SlopeAdj { double e, u, v; };
 __host__ __device__ inline SlopeAdj slope_adj(const Params &P, const double *b, const StateP &lR, const Work &w,
                                               int i, int j, int d) {
@@ -568,22 +568,6 @@ __host__ __device__ inline void adj_slopes_rev_at(const Params &P, const StateP 
                     SlopeAdj r = {w.s[S_LSEX + d][c], w.s[S_LSUX + d][c], w.s[S_LSVX + d][c]};
                     return r;
                   });
-}
-
-// passes 6 and 7 in one (device): the cell's own gather, its neighbours' slope
-// adjoints recomputed (slope_adj), no slots written between them
-__host__ __device__ inline void adj_cells_slopes_rev_at(const Params &P, const StateP &S, const double *b,
-                                                        const StateP &lR, const Work &w, const double *tcs,
-                                                        const StateW &lS, int i, int j) {
-  const int k = j * P.W + i;
-  const CellRev c = cells_rev(P, S, b, lR, w, tcs, i, j);
-  slopes_rev_core(P, S, w, lS, i, j, c.leta, c.lU, c.lV, w.s[S_LHTH][k], c.lhuC, c.lhvC, [&](int d, int o) {
-    if (o == 0) {
-      SlopeAdj r = {c.lsE[d], c.lsU[d], c.lsV[d]};
-      return r;
-    }
-    return slope_adj(P, b, lR, w, d == 0 ? i + o : i, d == 0 ? j : j + o, d);
-  });
 }
 
 // ---- boundary conditions^T (swe.mjs:289-303, :1026-1045, :1119-1124) ---------

@@ -120,6 +120,15 @@ __global__ void k_adj_bc(Params P, StateW l, int pass, StateP S, const double *b
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < (pass == 0 ? 4 * P.W : 4 * P.ny)) adj_bc_ghost(P, l, pass, t, S, b, ext);
 }
+// Both passes in one launch, for south and north sides that are not periodic:
+// then every source cell of either pass is an interior cell (the S/N pass's
+// corner ghosts clamp into the interior), no ghost of one pass is a source of
+// the other, and the folds are atomic adds, so the passes commute.
+__global__ void k_adj_bc_both(Params P, StateW l, StateP S, const double *b, const double *ext) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < 4 * P.W) adj_bc_ghost(P, l, 0, t, S, b, ext);
+  else if (t < 4 * P.W + 4 * P.ny) adj_bc_ghost(P, l, 1, t - 4 * P.W, S, b, ext);
+}
 __global__ void k_adj_epilogue(Params P, StateP pre, StateP lOut, const double *nfield, const double *sponge,
                                double dt, double *lN, int stage, StateW lA, StateW lB, StateW lR) {
   int i, j;
@@ -346,8 +355,12 @@ int bc_rev(swe_adj *a, const StateW &l, const StateP &S, int64_t n) {
   swe_ctx *c = a->ctx;
   const Params &P = c->P;
   const double *ext = a->ext_dev + 8 * n;
-  hipLaunchKernelGGL(k_adj_bc, grid_for(4 * P.W), dim3(256), 0, c->stream, P, l, 0, S, a->bed, ext);
-  hipLaunchKernelGGL(k_adj_bc, grid_for(4 * P.ny), dim3(256), 0, c->stream, P, l, 1, S, a->bed, ext);
+  if (P.bc[2] != BC_PERIODIC && P.bc[3] != BC_PERIODIC) {
+    hipLaunchKernelGGL(k_adj_bc_both, grid_for(4 * P.W + 4 * P.ny), dim3(256), 0, c->stream, P, l, S, a->bed, ext);
+  } else {
+    hipLaunchKernelGGL(k_adj_bc, grid_for(4 * P.W), dim3(256), 0, c->stream, P, l, 0, S, a->bed, ext);
+    hipLaunchKernelGGL(k_adj_bc, grid_for(4 * P.ny), dim3(256), 0, c->stream, P, l, 1, S, a->bed, ext);
+  }
   ACHK(a, hipGetLastError());
   return SWE_OK;
 }

@@ -55,7 +55,22 @@ __host__ __device__ __forceinline__ LimD limit_deriv(int lim, double a, double b
   return r;
 }
 
-// Reverse of hllc(hL, huL, hvL, hR, huR, hvR) (swe.mjs:974-1011).
+// 1/x to about an ulp, for factors that only scale derivatives (on the device
+// the hardware estimate and one Newton-Raphson step; on the host 1/x): never for
+// a value a branch of the forward's recomputation compares
+__host__ __device__ __forceinline__ double rcp_d(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double y = __builtin_amdgcn_rcp(x);
+  return __builtin_fma(__builtin_fma(-x, y, 1.0), y, y);
+#else
+  return 1.0 / x;
+#endif
+}
+
+// Reverse of hllc(hL, huL, hvL, hR, huR, hvR) (swe.mjs:974-1011).  The forward's
+// wave pattern (velocities, celerities, hStar, qL, qR, sL, sR, sM) is recomputed
+// with the forward's exact operations, so every branch goes the way the forward
+// went; the reciprocals that only scale derivatives are rcp_d.
 struct HllcIn { double hL, huL, hvL, hR, huR, hvR; };
 __host__ __device__ inline HllcIn hllc_rev(double hL, double huL, double hvL, double hR, double huR, double hvR,
                                            const HllcConst &C, double lm, double ln, double lt) {
@@ -66,15 +81,15 @@ __host__ __device__ inline HllcIn hllc_rev(double hL, double huL, double hvL, do
   double uL = 0.0, vL = 0.0, uR = 0.0, vR = 0.0;
   if (wetL) { uL = huL / hL; vL = hvL / hL; }
   if (wetR) { uR = huR / hR; vR = hvR / hR; }
-  const double cL = __builtin_sqrt(G * jmax(hL, 0.0)), cR = __builtin_sqrt(G * jmax(hR, 0.0));
+  const double cL = sqrt_gh(G * jmax(hL, 0.0), C.normal), cR = sqrt_gh(G * jmax(hR, 0.0), C.normal);
   double sL, sR, a = 0, hStar = 0, qL = 1, qR = 1;
   if (!wetL) { sL = uR - 2 * cR; sR = uR + cR; }
   else if (!wetR) { sL = uL - cL; sR = uL + 2 * cL; }
   else {
     a = (cL + cR) * 0.5 + (uL - uR) * 0.25;
-    hStar = a * a / G;
-    qL = hStar > hL ? __builtin_sqrt(0.5 * (hStar + hL) * hStar / (hL * hL)) : 1.0;
-    qR = hStar > hR ? __builtin_sqrt(0.5 * (hStar + hR) * hStar / (hR * hR)) : 1.0;
+    hStar = div_r(a * a, G, C.rg);
+    qL = hStar > hL ? sqrt_n(0.5 * (hStar + hL) * hStar / (hL * hL)) : 1.0;
+    qR = hStar > hR ? sqrt_n(0.5 * (hStar + hR) * hStar / (hR * hR)) : 1.0;
     sL = uL - cL * qL;
     sR = uR + cR * qR;
   }
@@ -124,16 +139,16 @@ __host__ __device__ inline HllcIn hllc_rev(double hL, double huL, double hvL, do
       // X = (hStar + h) hStar / (2 h^2) and qL = sqrt(X) (qR likewise)
       double lhStar = 0.0;
       if (hStar > hL) {
-        const double r = 1.0 / hL, r2 = r * r;
+        const double r = rcp_d(hL), r2 = r * r;
         const double X = 0.5 * (hStar + hL) * hStar * r2;
-        const double lX = lqL * (0.5 / qL);
+        const double lX = lqL * (0.5 * rcp_d(qL));
         lhStar += lX * 0.5 * (2.0 * hStar + hL) * r2;
         g.hL += lX * (0.5 * hStar * r2 - 2.0 * X * r);
       }
       if (hStar > hR) {
-        const double r = 1.0 / hR, r2 = r * r;
+        const double r = rcp_d(hR), r2 = r * r;
         const double X = 0.5 * (hStar + hR) * hStar * r2;
-        const double lX = lqR * (0.5 / qR);
+        const double lX = lqR * (0.5 * rcp_d(qR));
         lhStar += lX * 0.5 * (2.0 * hStar + hR) * r2;
         g.hR += lX * (0.5 * hStar * r2 - 2.0 * X * r);
       }
@@ -143,14 +158,14 @@ __host__ __device__ inline HllcIn hllc_rev(double hL, double huL, double hvL, do
   }
   // u = hu / h, c = sqrt(G h)
   if (wetL) {
-    const double r = 1.0 / hL;
-    g.hL += lcL * G * (0.5 / cL);
+    const double r = rcp_d(hL);
+    g.hL += lcL * G * (0.5 * rcp_d(cL));
     g.huL += luL * r; g.hvL += lvL * r;
     g.hL -= (luL * uL + lvL * vL) * r;
   }
   if (wetR) {
-    const double r = 1.0 / hR;
-    g.hR += lcR * G * (0.5 / cR);
+    const double r = rcp_d(hR);
+    g.hR += lcR * G * (0.5 * rcp_d(cR));
     g.huR += luR * r; g.hvR += lvR * r;
     g.hR -= (luR * uR + lvR * vR) * r;
   }

@@ -24,7 +24,7 @@
 
 namespace swe {
 hipError_t launch_sweep_batch(int mode, const BasinDev *basins, const Task *tasks, int ntasks, unsigned *qctr,
-                              int parity, int nwaves, hipStream_t st);
+                              int parity, int nwaves, int lim_all, bool uni_all, hipStream_t st);
 const void *k_sweep_batch_ptr();
 hipError_t launch_apply_bc_batch(const BasinDev *basins, int nb, int max_ghost, int parity, int stage, hipStream_t st);
 hipError_t launch_maxdt_batch(const BasinDev *basins, int nb, int max_cells, int parity, hipStream_t st);
@@ -70,6 +70,8 @@ struct swe_batch {
   int parity = 0;             // 0: current state in buffer 0; 1: in buffer 2
   int chunk = 32;
   int nwaves = 0;             // persistent grid size
+  int lim_all = -1;           // the slope limiter every basin uses, or -1
+  bool uni_all = false;       // every basin's rows are alike (Params::geo_uni)
   int max_ghost = 0, max_cells = 0;
   bool dirty = true;
   bool use_graph = true;
@@ -199,9 +201,11 @@ static int enqueue_step(swe_batch *b) {
   const int nb = (int)b->basins.size();
   const int par = b->parity;
   BCHK(b, launch_apply_bc_batch(b->dbasins, nb, b->max_ghost, par, 0, b->stream));
-  BCHK(b, launch_sweep_batch(MODE_STAGE1, b->dbasins, b->dtasks, b->ntasks, b->qctr, par, b->nwaves, b->stream));
+  BCHK(b, launch_sweep_batch(MODE_STAGE1, b->dbasins, b->dtasks, b->ntasks, b->qctr, par, b->nwaves, b->lim_all,
+                             b->uni_all, b->stream));
   BCHK(b, launch_apply_bc_batch(b->dbasins, nb, b->max_ghost, par, 1, b->stream));
-  BCHK(b, launch_sweep_batch(MODE_STAGE2, b->dbasins, b->dtasks, b->ntasks, b->qctr, par, b->nwaves, b->stream));
+  BCHK(b, launch_sweep_batch(MODE_STAGE2, b->dbasins, b->dtasks, b->ntasks, b->qctr, par, b->nwaves, b->lim_all,
+                             b->uni_all, b->stream));
   BCHK(b, launch_finalize_batch(b->dbasins, nb, b->qctr, 1, b->stream));
   b->parity ^= 1;
   return SWE_OK;
@@ -288,6 +292,12 @@ int swe_batch_create(const swe_desc *descs, int nb, swe_batch **out) {
       }
     }
     set_uniform_rows(b->basins[i].P, host.data() + B.off_rowsA, B.P.H);
+  }
+  b->lim_all = b->basins[0].P.limiter;
+  b->uni_all = true;
+  for (const Basin &B : b->basins) {
+    if (B.P.limiter != b->lim_all) b->lim_all = -1;
+    if (!B.P.geo_uni) b->uni_all = false;
   }
   if ((e = hipMemcpyAsync(b->rows, host.data(), sizeof(double) * rpool, hipMemcpyHostToDevice, b->stream)) != hipSuccess)
     return bail("rows upload", e);

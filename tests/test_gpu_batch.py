@@ -73,3 +73,67 @@ def test_batch_autotune_keeps_state_and_results():
         g.advance(6)
         for a, b in zip(B.state(i), g.state()):
             assert np.array_equal(bits(a), bits(b)), i
+
+
+def sphere_basin(limiter):
+    """A 48x24 band of a small sphere (60S-60N), 2 km deep with a 5 m hump, walls
+    at the band's edges, periodic in longitude."""
+    nx, ny, R = 48, 24, 6.371e6 / 20
+    rows = swe_amd.geometry_spherical(nx, ny, R=R, lat0=-60.0, lat1=60.0, omega=7.2921e-5)
+    W, H = nx + 4, ny + 4
+    X, Y = np.meshgrid(np.arange(W), np.arange(H))
+    b = np.full(W * H, -2000.0)
+    h = 2000.0 + 5.0 * np.exp(-(((X - W / 3) / 5.0) ** 2 + ((Y - H / 2) / 4.0) ** 2)).ravel()
+    z = np.zeros(W * H)
+    return dict(nx=nx, ny=ny, rows=rows, R=R, limiter=limiter, state=(b, h, z, z.copy()),
+                bc=("periodic", "periodic", "reflect", "reflect"))
+
+
+@pytest.mark.parametrize("limiters,with_sphere", [(("mc", "minmod", "central"), False),
+                                                   (("minmod",), True), (("central",), False)])
+def test_batch_limiters_and_row_kinds_equal_plain_runs(limiters, with_sphere):
+    """Basins with different slope limiters in one batch (the kernel that switches
+    on each task's limiter), and batches whose basins share one (the kernels
+    specialised on it: with uniform rows on Cartesian grids, with per-row metrics
+    once a spherical basin joins)."""
+    specs = workloads.ensemble(nb=6, lo=6, hi=120, seed=29)
+    for q, s in enumerate(specs):
+        s["limiter"] = limiters[q % len(limiters)]
+    say(f"[gpu] batch limiters {limiters}, sphere basin {with_sphere}")
+    basins = [dict(nx=s["nx"], ny=s["ny"], manning=s["manning"], cfl=s["cfl"], limiter=s["limiter"],
+                   rows=swe_amd.geometry_cartesian(s["nx"], s["ny"], s["dx"])) for s in specs]
+    sph = sphere_basin(limiters[0]) if with_sphere else None
+    if sph:
+        basins.append(dict(nx=sph["nx"], ny=sph["ny"], rows=sph["rows"], sphere=True, R=sph["R"],
+                           limiter=sph["limiter"], manning=0.0))
+    B = swe_amd.Batch(basins)
+    plain = []
+    for i, s in enumerate(specs):
+        B.load_state(i, *s["state"])
+        B.set_bc(i, s["bc"])
+        B.set_harmonic(i, 0, amp=[s["amp"]], omega=[s["omega"]], phase=[0.0],
+                       uamp=[s["uamp"]], uomega=[s["omega"]], uphase=[0.0])
+        g = swe_amd.Solver(s["nx"], s["ny"], swe_amd.geometry_cartesian(s["nx"], s["ny"], s["dx"]),
+                           manning=s["manning"], cfl=s["cfl"], limiter=s["limiter"])
+        g.load_state(*s["state"])
+        g.set_bc(s["bc"])
+        g.set_harmonic(0, amp=[s["amp"]], omega=[s["omega"]], phase=[0.0],
+                       uamp=[s["uamp"]], uomega=[s["omega"]], uphase=[0.0])
+        plain.append(g)
+    if sph:
+        q = len(specs)
+        B.load_state(q, *sph["state"])
+        B.set_bc(q, sph["bc"])
+        g = swe_amd.Solver(sph["nx"], sph["ny"], sph["rows"], sphere=True, R=sph["R"], manning=0.0,
+                           limiter=sph["limiter"])
+        g.load_state(*sph["state"])
+        g.set_bc(sph["bc"])
+        plain.append(g)
+    B.set_chunk_rows(16)
+    assert B.advance(15) == 15
+    for g in plain:
+        assert g.advance(15) == 15
+    for i, g in enumerate(plain):
+        for a, b in zip(B.state(i), g.state()):
+            assert np.array_equal(bits(a), bits(b)), i
+        assert B.get_scalar(i, swe_amd.S_T) == g.t

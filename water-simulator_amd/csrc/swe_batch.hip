@@ -291,7 +291,9 @@ int swe_batch_create(const swe_desc *descs, int nb, swe_batch **out) {
         dstA[(size_t)j * GEO_AOS] = dst[j];
       }
     }
-    set_uniform_rows(b->basins[i].P, host.data() + B.off_rowsA, B.P.H);
+    double *A = host.data() + B.off_rowsA;
+    for (int j = 0; j < B.P.H; ++j) A[(size_t)j * GEO_AOS + GEO_TANR] = A[(size_t)j * GEO_AOS + R_TAN] / B.P.R;
+    set_uniform_rows(b->basins[i].P, A, B.P.H);
   }
   b->lim_all = b->basins[0].P.limiter;
   b->uni_all = true;

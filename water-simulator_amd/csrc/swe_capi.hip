@@ -87,6 +87,7 @@ static int upload_row(swe_ctx *c, int row, const double *host) {
     if (src.empty()) continue;
     for (int j = 0; j < H; ++j) a[(size_t)j * GEO_AOS + k] = recip[k] ? 1.0 / src[j] : src[j];
   }
+  for (int j = 0; j < H; ++j) a[(size_t)j * GEO_AOS + GEO_TANR] = a[(size_t)j * GEO_AOS + R_TAN] / c->P.R;
   set_uniform_rows(c->P, a.data(), H);
   HIPCHK(c, hipMemcpyAsync(c->rowsA, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -384,7 +384,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
           if (P.sphere) rhv += bedPrev2.hv;
         }
         if (P.sphere) {                                   // swe.mjs:600-619
-          const double fe = rowv(FROW, o) + Uo * (rowv(TAN, o) / P.R);
+          const double fe = rowv(FROW, o) + Uo * rowv(GEO_TANR, o);   // tanPhi / R of the row
           rhu += fe * cur.hv;
           rhv -= fe * cur.hu;
         } else if (P.coriolis_on) {                       // swe.mjs:620-628

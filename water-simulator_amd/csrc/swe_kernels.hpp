@@ -108,6 +108,9 @@ struct Params {
   int geo_uni, geo_pad;
 };
 constexpr int GEO_AOS = 16;
+// slot of geoA beside the 13 arrays above (and the host's area row): tanPhi[j] / R,
+// which the reference forms once per row (swe.mjs:610) and the stage kernels read
+constexpr int GEO_TANR = 14;
 
 // Params::geo_uni from the interleaved rows a[j * GEO_AOS + k], j < H
 inline void set_uniform_rows(Params &P, const double *a, int H) {

@@ -137,3 +137,35 @@ def test_batch_limiters_and_row_kinds_equal_plain_runs(limiters, with_sphere):
         for a, b in zip(B.state(i), g.state()):
             assert np.array_equal(bits(a), bits(b)), i
         assert B.get_scalar(i, swe_amd.S_T) == g.t
+
+
+def test_rows_made_uneven_after_a_captured_graph():
+    """A Cartesian grid's rows are all alike, so its captured step graph runs the
+    uniform-row stage kernels; once a row changes, the step must read every row
+    again: the run equals a solver built with the uneven rows from the start."""
+    s = workloads.ensemble(nb=1, lo=60, hi=90, seed=31)[0]
+    nx, ny = s["nx"], s["ny"]
+    rows = swe_amd.geometry_cartesian(nx, ny, s["dx"])
+    uneven = np.array(rows["dxRow"], dtype=np.float64) * (1.0 + 0.01 * np.sin(np.arange(ny + 4)))
+    say(f"[gpu] rows made uneven after capture, {nx}x{ny}")
+
+    def solver(rw):
+        g = swe_amd.Solver(nx, ny, rw, manning=s["manning"], cfl=s["cfl"])
+        g.load_state(*s["state"])
+        g.set_bc(s["bc"])
+        g.set_harmonic(0, amp=[s["amp"]], omega=[s["omega"]], phase=[0.0],
+                       uamp=[s["uamp"]], uomega=[s["omega"]], uphase=[0.0])
+        g.use_graph(True)
+        return g
+
+    a = solver(rows)
+    assert a.advance(5) == 5                 # graph captured on alike rows
+    a.set_row("dxRow", uneven)
+    ref_rows = dict(rows)
+    ref_rows["dxRow"] = uneven
+    b = solver(ref_rows)
+    b.load_state(*a.state())
+    b.set_time(a.t, 5)
+    assert a.advance(7) == 7 and b.advance(7) == 7
+    for x, y in zip(a.state(), b.state()):
+        assert np.array_equal(bits(x), bits(y))

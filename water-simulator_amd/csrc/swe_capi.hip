@@ -88,7 +88,10 @@ static int upload_row(swe_ctx *c, int row, const double *host) {
     for (int j = 0; j < H; ++j) a[(size_t)j * GEO_AOS + k] = recip[k] ? 1.0 / src[j] : src[j];
   }
   for (int j = 0; j < H; ++j) a[(size_t)j * GEO_AOS + GEO_TANR] = a[(size_t)j * GEO_AOS + R_TAN] / c->P.R;
+  // a captured step graph holds the stage-kernel instance chosen for the old rows
+  const int was_uni = c->P.geo_uni;
   set_uniform_rows(c->P, a.data(), H);
+  if (c->P.geo_uni != was_uni) drop_graphs(c);
   HIPCHK(c, hipMemcpyAsync(c->rowsA, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return SWE_OK;

@@ -118,10 +118,6 @@ struct Face4 { double eta, b, u, v; };           // reconstructed state on one s
 struct FluxF { double M, NL, NR, T; };           // face flux: mass, normal+pL, normal+pR, transverse
 struct Part3 { double h, hu, hv; };
 
-#ifndef SWE_USE_DPP
-#define SWE_USE_DPP 1
-#endif
-#if SWE_USE_DPP
 // Whole-wavefront shift by one lane through DPP (wave_shl:1 = 0x130 brings lane
 // i+1 to lane i, wave_shr:1 = 0x138 brings lane i-1): two v_mov_b32_dpp per
 // double, no LDS round trip.  The end lane, which has no source, reads zero
@@ -136,10 +132,6 @@ __device__ __forceinline__ double dpp_shift(double v) {
 }
 __device__ __forceinline__ double shdn(double v) { return dpp_shift<0x130>(v); }
 __device__ __forceinline__ double shup(double v) { return dpp_shift<0x138>(v); }
-#else
-__device__ __forceinline__ double shdn(double v) { return __shfl_down(v, 1, SWEEP_LANES); }
-__device__ __forceinline__ double shup(double v) { return __shfl_up(v, 1, SWEEP_LANES); }
-#endif
 
 // Interface flux + hydrostatic pressure corrections (swe.mjs:413-445 body, and
 // :448-475 with normal/tangential swapped by the caller).

@@ -334,6 +334,14 @@ int swe_adj_host_residual_rev(const swe_desc *d, const double *h, const double *
                               const double *lrhv, double *lh, double *lhu, double *lhv);
 int swe_adj_host_bc_rev(const swe_desc *d, const int32_t bc[4], const double ext[8], const double *h,
                         const double *hu, const double *hv, const double *b, double *lh, double *lhu, double *lhv);
+/* The same transpose in the gather form the device runs where no side is Flather,
+ * the south and north sides are not periodic and nx, ny >= 4 (SWE_EINVAL
+ * otherwise): one work item per source column or row, adding its ghosts' folds in
+ * the loop's order.  No counterpart in the reference: a host check of the device
+ * pass. */
+int swe_adj_host_bc_rev_gather(const swe_desc *d, const int32_t bc[4], const double ext[8], const double *h,
+                               const double *hu, const double *hv, const double *b, double *lh, double *lhu,
+                               double *lhv);
 int swe_adj_host_epilogue_rev(const swe_desc *d, int stage, const double *pre_h, const double *pre_hu,
                               const double *pre_hv, const double *nfield, const double *sponge, double dt,
                               const double *lout_h,

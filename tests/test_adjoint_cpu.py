@@ -236,6 +236,44 @@ def test_flather_bc_rev_matches_fd(kinds, shape):
         assert not g.any()
 
 
+GATHER_SETS = [
+    ("reflect",) * 4,
+    ("outflow",) * 4,
+    ("periodic", "periodic", "reflect", "reflect"),
+    ("reflect", "outflow", "outflow", "reflect"),
+    ("periodic", "periodic", "outflow", "reflect"),
+]
+
+
+@pytest.mark.parametrize("kinds", GATHER_SETS, ids=["-".join(k) for k in GATHER_SETS])
+@pytest.mark.parametrize("shape", [(9, 7), (4, 4), (17, 5)])
+def test_bc_rev_gather_form_equals_the_loop(kinds, shape):
+    """The device's gather form of BC^T (one work item per source column or row,
+    no atomics) gives the ghost-by-ghost loop's result to the bit; it is refused
+    for a periodic south or north side and for Flather sides (those run the
+    ghost-by-ghost form)."""
+    nx, ny = shape
+    rows = swe_amd.geometry_cartesian(nx, ny, 2.0)
+    desc, keep = swe_amd.make_desc(nx, ny, rows, manning=0.0)
+    rng = np.random.default_rng(7 + 3 * nx + ny)
+    W, H = nx + 4, ny + 4
+    N = W * H
+    b = -1.5 + 0.2 * rng.normal(size=N)
+    h = np.maximum(0.0, 0.1 * rng.normal(size=N) - b)
+    h[rng.random(N) < 0.15] = 0.0                       # some dry source cells
+    hu = 0.3 * rng.normal(size=N) * h
+    hv = 0.3 * rng.normal(size=N) * h
+    ext = np.array([0.12, 0.03, -0.05, 0.02, 0.07, -0.04, 0.02, 0.05])
+    lam = [rng.normal(size=N) for _ in range(3)]
+    loop = swe_amd.bc_rev_host(desc, kinds, *lam, ext=ext, state=(b, h, hu, hv))
+    gath = swe_amd.bc_rev_host(desc, kinds, *lam, ext=ext, state=(b, h, hu, hv), gather=True)
+    for x, y in zip(loop, gath):
+        assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
+    for refused in (("reflect", "reflect", "periodic", "periodic"), ("flather", "reflect", "reflect", "reflect")):
+        with pytest.raises(swe_amd.SweError):
+            swe_amd.bc_rev_host(desc, refused, *lam, ext=ext, state=(b, h, hu, hv), gather=True)
+
+
 # ---- dryClean^T / friction^T / sponge^T -----------------------------------------------
 @pytest.mark.parametrize("stage,sponge", [(1, False), (2, False), (2, True)])
 @pytest.mark.parametrize("field", [False, True])

@@ -120,14 +120,11 @@ __global__ void k_adj_bc(Params P, StateW l, int pass, StateP S, const double *b
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < (pass == 0 ? 4 * P.W : 4 * P.ny)) adj_bc_ghost(P, l, pass, t, S, b, ext);
 }
-// Both passes in one launch, for south and north sides that are not periodic:
-// then every source cell of either pass is an interior cell (the S/N pass's
-// corner ghosts clamp into the interior), no ghost of one pass is a source of
-// the other, and the folds are atomic adds, so the passes commute.
-__global__ void k_adj_bc_both(Params P, StateW l, StateP S, const double *b, const double *ext) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < 4 * P.W) adj_bc_ghost(P, l, 0, t, S, b, ext);
-  else if (t < 4 * P.W + 4 * P.ny) adj_bc_ghost(P, l, 1, t - 4 * P.W, S, b, ext);
+// one pass of BC^T in gather form (adj_bc_gather_ok): one work item per source
+// column (pass 0) or row (pass 1), no atomics, the host loop's result
+__global__ void k_adj_bc_gather(Params P, StateW l, int pass) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < (pass == 0 ? 2 * (P.W - 4) : 2 * P.ny)) adj_bc_gather(P, l, pass, q);
 }
 __global__ void k_adj_epilogue(Params P, StateP pre, StateP lOut, const double *nfield, const double *sponge,
                                double dt, double *lN, int stage, StateW lA, StateW lB, StateW lR) {
@@ -355,8 +352,9 @@ int bc_rev(swe_adj *a, const StateW &l, const StateP &S, int64_t n) {
   swe_ctx *c = a->ctx;
   const Params &P = c->P;
   const double *ext = a->ext_dev + 8 * n;
-  if (P.bc[2] != BC_PERIODIC && P.bc[3] != BC_PERIODIC) {
-    hipLaunchKernelGGL(k_adj_bc_both, grid_for(4 * P.W + 4 * P.ny), dim3(256), 0, c->stream, P, l, S, a->bed, ext);
+  if (adj_bc_gather_ok(P)) {
+    hipLaunchKernelGGL(k_adj_bc_gather, grid_for(2 * (P.W - 4)), dim3(256), 0, c->stream, P, l, 0);
+    hipLaunchKernelGGL(k_adj_bc_gather, grid_for(2 * P.ny), dim3(256), 0, c->stream, P, l, 1);
   } else {
     hipLaunchKernelGGL(k_adj_bc, grid_for(4 * P.W), dim3(256), 0, c->stream, P, l, 0, S, a->bed, ext);
     hipLaunchKernelGGL(k_adj_bc, grid_for(4 * P.ny), dim3(256), 0, c->stream, P, l, 1, S, a->bed, ext);
@@ -873,6 +871,24 @@ int swe_adj_host_bc_rev(const swe_desc *d, const int32_t bc[4], const double ext
   const StateP S = {h, hu, hv};
   for (int t = 0; t < 4 * P.W; ++t) adj_bc_ghost(P, l, 0, t, S, b, ext);
   for (int t = 0; t < 4 * P.ny; ++t) adj_bc_ghost(P, l, 1, t, S, b, ext);
+  return SWE_OK;
+}
+
+// The gather form of the same transpose (adj_bc_gather_ok configurations only),
+// for checking it against the loop above on a machine without a GPU.
+int swe_adj_host_bc_rev_gather(const swe_desc *d, const int32_t bc[4], const double ext[8], const double *h,
+                               const double *hu, const double *hv, const double *b, double *lh, double *lhu,
+                               double *lhv) {
+  Params P;
+  if (!bc || host_params(d, bc, P) || !lh || !lhu || !lhv) return SWE_EINVAL;
+  if (!adj_bc_gather_ok(P)) return SWE_EINVAL;
+  bool flather = false;
+  for (int s = 0; s < 4; ++s) flather = flather || bc[s] == SWE_BC_FLATHER;
+  if (flather && (!ext || !h || !hu || !hv || !b)) return SWE_EINVAL;
+  const StateW l = {lh, lhu, lhv};
+  const StateP S = {h, hu, hv};
+  for (int q = 0; q < 2 * (P.W - 4); ++q) adj_bc_gather(P, l, 0, q);
+  for (int q = 0; q < 2 * P.ny; ++q) adj_bc_gather(P, l, 1, q);
   return SWE_OK;
 }
 

@@ -645,6 +645,58 @@ __host__ __device__ inline FlatherD flather_deriv(const Params &P, int side, dou
   return d;
 }
 
+// Ghost t of one pass: its cell g, its source cell s and the side it is on.
+__host__ __device__ inline void adj_bc_cells(const Params &P, int pass, int t, int &g, int &s, int &side) {
+  int gi, gj;
+  if (pass == 0) {
+    const int rr = t / P.W;
+    gi = t % P.W; gj = rr < 2 ? rr : P.H - 4 + rr; side = gj < 2 ? 2 : 3;
+  } else {
+    const int cc = t / P.ny;
+    gj = 2 + t % P.ny; gi = cc < 2 ? cc : P.W - 4 + cc; side = gi < 2 ? 0 : 1;
+  }
+  int si, sj;
+  adj_bc_src(P.bc[side], side, gi - 2, gj - 2, P.nx, P.ny, si, sj);
+  g = gj * P.W + gi;
+  s = (sj + 2) * P.W + (si + 2);
+}
+
+// What a ghost holding adjoints (lh, lhu, lhv) on `side` folds into its source
+// cell: dh into l.h, dhu into l.hu, dhv into l.hv (any = false: nothing, a
+// Flather ghost with a dry source).  For a Flather side the source's state
+// (h, hu, hv and bed) is passed in.
+struct GhostFold { bool any; double dh, dhu, dhv; };
+__host__ __device__ inline GhostFold adj_bc_fold(const Params &P, int side, double lh, double lhu, double lhv,
+                                                 double hInt, double qU, double qV, double bed, const double *ext) {
+  const int kind = P.bc[side];
+  GhostFold f = {true, 0.0, 0.0, 0.0};
+  if (kind == BC_FLATHER) {
+    // a dry source: the ghost is max(0, eta_ext - b), zero momentum, no state in it
+    if (hInt <= P.min_depth) { f.any = false; return f; }
+    const bool we = side < 2;
+    const FlatherD d = flather_deriv(P, side, hInt, we ? qU : qV, bed, ext[2 * side], ext[2 * side + 1]);
+    const double lqn = we ? lhu : lhv, lqt = we ? lhv : lhu;
+    f.dh = lh * d.dh_dh + lqn * d.dq_dh;
+    const double ln = lh * d.dh_dq + lqn * d.dq_dq;
+    f.dhu = we ? ln : lqt;
+    f.dhv = we ? lqt : ln;
+    return f;
+  }
+  const bool flipU = kind == BC_REFLECT && side < 2, flipV = kind == BC_REFLECT && side >= 2;
+  f.dh = lh;
+  f.dhu = flipU ? -lhu : lhu;
+  f.dhv = flipV ? -lhv : lhv;
+  return f;
+}
+// the linear kinds' fold (reflect, outflow, periodic): the ghost's adjoints, the
+// normal component negated at a wall
+__host__ __device__ inline GhostFold adj_bc_fold_linear(const Params &P, int side, double lh, double lhu, double lhv) {
+  const int kind = P.bc[side];
+  const bool flipU = kind == BC_REFLECT && side < 2, flipV = kind == BC_REFLECT && side >= 2;
+  const GhostFold f = {true, lh, flipU ? -lhu : lhu, flipV ? -lhv : lhv};
+  return f;
+}
+
 // Fold ghost t of one pass into its source and clear it.  applyBC writes the
 // W/E columns (every row) and then the S/N rows (every column, corners
 // included, a periodic source possibly a W/E ghost); its transpose runs the S/N
@@ -655,36 +707,84 @@ __host__ __device__ inline FlatherD flather_deriv(const Params &P, int side, dou
 // step's far-field values (eta_ext, u_ext per side); the linear kinds need none.
 __host__ __device__ inline void adj_bc_ghost(const Params &P, const StateW &l, int pass, int t, const StateP &S,
                                              const double *b, const double *ext) {
-  int gi, gj, side;
-  if (pass == 0) {
-    const int rr = t / P.W;
-    gi = t % P.W; gj = rr < 2 ? rr : P.H - 4 + rr; side = gj < 2 ? 2 : 3;
-  } else {
-    const int cc = t / P.ny;
-    gj = 2 + t % P.ny; gi = cc < 2 ? cc : P.W - 4 + cc; side = gi < 2 ? 0 : 1;
-  }
-  const int kind = P.bc[side];
-  int si, sj;
-  adj_bc_src(kind, side, gi - 2, gj - 2, P.nx, P.ny, si, sj);
-  const int g = gj * P.W + gi, s = (sj + 2) * P.W + (si + 2);
+  int g, s, side;
+  adj_bc_cells(P, pass, t, g, s, side);
   const double lh = l.h[g], lhu = l.hu[g], lhv = l.hv[g];
   l.h[g] = 0.0; l.hu[g] = 0.0; l.hv[g] = 0.0;
-  if (kind == BC_FLATHER) {
-    const double hInt = S.h[s];
-    // a dry source: the ghost is max(0, eta_ext - b), zero momentum, no state in it
-    if (hInt <= P.min_depth) return;
-    const bool we = side < 2;
-    const FlatherD d = flather_deriv(P, side, hInt, we ? S.hu[s] : S.hv[s], b[s], ext[2 * side], ext[2 * side + 1]);
-    const double lqn = we ? lhu : lhv, lqt = we ? lhv : lhu;
-    adj_add(&l.h[s], lh * d.dh_dh + lqn * d.dq_dh);
-    adj_add(&(we ? l.hu : l.hv)[s], lh * d.dh_dq + lqn * d.dq_dq);
-    adj_add(&(we ? l.hv : l.hu)[s], lqt);
-    return;
+  const bool fl = P.bc[side] == BC_FLATHER;
+  const GhostFold f = adj_bc_fold(P, side, lh, lhu, lhv, fl ? S.h[s] : 0.0, fl ? S.hu[s] : 0.0, fl ? S.hv[s] : 0.0,
+                                  fl ? b[s] : 0.0, ext);
+  if (!f.any) return;
+  adj_add(&l.h[s], f.dh);
+  adj_add(&l.hu[s], f.dhu);
+  adj_add(&l.hv[s], f.dhv);
+}
+
+// BC^T in gather form, for the linear kinds (no Flather side) with south and
+// north sides that are not periodic and nx, ny >= 4: every source is then an
+// interior cell, the sources of one pass's
+// work items are disjoint, and each work item adds its ghosts' folds into its
+// sources in the order of the loop over t above, so the result is that loop's
+// to the bit, with no atomics.  Pass 0, item q < 2 (W - 4): source column
+// sc = 2 + q % (W - 4) on the south (q < W - 4) or north side, whose ghosts are
+// that column's two ghost cells and, for the first and last interior columns,
+// the corner columns that clamp into them.  Pass 1, item q < 2 ny: interior row
+// 2 + q % ny on the west or east side, whose ghosts are that row's two.  An item
+// reads all its ghosts and sources before it writes any of them.
+template <int N>
+__host__ __device__ inline void adj_bc_gather_item(const Params &P, const StateW &l, int pass, const int (&ts)[N]) {
+  int g[N], s[N], side[N];
+  double v[N][3];
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    adj_bc_cells(P, pass, ts[m], g[m], s[m], side[m]);
+    v[m][0] = l.h[g[m]]; v[m][1] = l.hu[g[m]]; v[m][2] = l.hv[g[m]];
   }
-  const bool flipU = kind == BC_REFLECT && side < 2, flipV = kind == BC_REFLECT && side >= 2;
-  adj_add(&l.h[s], lh);
-  adj_add(&l.hu[s], flipU ? -lhu : lhu);
-  adj_add(&l.hv[s], flipV ? -lhv : lhv);
+  // the item's (at most two) distinct sources, in order of first appearance
+  const int s0 = s[0];
+  int s1 = -1;
+#pragma unroll
+  for (int m = 1; m < N; ++m)
+    if (s1 < 0 && s[m] != s0) s1 = s[m];
+  double a0h = l.h[s0], a0u = l.hu[s0], a0v = l.hv[s0];
+  double a1h = 0.0, a1u = 0.0, a1v = 0.0;
+  if (s1 >= 0) { a1h = l.h[s1]; a1u = l.hu[s1]; a1v = l.hv[s1]; }
+#pragma unroll
+  for (int m = 0; m < N; ++m) { l.h[g[m]] = 0.0; l.hu[g[m]] = 0.0; l.hv[g[m]] = 0.0; }
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    const GhostFold f = adj_bc_fold_linear(P, side[m], v[m][0], v[m][1], v[m][2]);
+    if (s[m] == s0) { a0h += f.dh; a0u += f.dhu; a0v += f.dhv; }
+    else { a1h += f.dh; a1u += f.dhu; a1v += f.dhv; }
+  }
+  l.h[s0] = a0h; l.hu[s0] = a0u; l.hv[s0] = a0v;
+  if (s1 >= 0) { l.h[s1] = a1h; l.hu[s1] = a1u; l.hv[s1] = a1v; }
+}
+__host__ __device__ inline void adj_bc_gather(const Params &P, const StateW &l, int pass, int q) {
+  if (pass == 0) {
+    const int nsc = P.W - 4, r0 = 2 * (q / nsc), sc = 2 + q % nsc;
+    if (sc == 2) {                   // the first interior column: corner columns 0 and 1 clamp into it
+      const int ts[6] = {r0 * P.W, r0 * P.W + 1, r0 * P.W + 2, (r0 + 1) * P.W, (r0 + 1) * P.W + 1, (r0 + 1) * P.W + 2};
+      adj_bc_gather_item<6>(P, l, 0, ts);
+    } else if (sc == P.W - 3) {      // the last: corner columns W - 2 and W - 1
+      const int c = P.W - 3;
+      const int ts[6] = {r0 * P.W + c, r0 * P.W + c + 1, r0 * P.W + c + 2,
+                         (r0 + 1) * P.W + c, (r0 + 1) * P.W + c + 1, (r0 + 1) * P.W + c + 2};
+      adj_bc_gather_item<6>(P, l, 0, ts);
+    } else {
+      const int ts[2] = {r0 * P.W + sc, (r0 + 1) * P.W + sc};
+      adj_bc_gather_item<2>(P, l, 0, ts);
+    }
+  } else {
+    const int c0 = 2 * (q / P.ny), r = q % P.ny;
+    const int ts[2] = {c0 * P.ny + r, (c0 + 1) * P.ny + r};
+    adj_bc_gather_item<2>(P, l, 1, ts);
+  }
+}
+__host__ __device__ inline bool adj_bc_gather_ok(const Params &P) {
+  for (int k = 0; k < 4; ++k)
+    if (P.bc[k] == BC_FLATHER) return false;
+  return P.bc[2] != BC_PERIODIC && P.bc[3] != BC_PERIODIC && P.nx >= 4 && P.ny >= 4;
 }
 
 // ---- dryClean^T, friction^T and sponge^T (swe.mjs:659-704, :734-814, :1188-1197) ----

@@ -242,6 +242,9 @@ HOST_CASES = [
     dict(id="minmod-mixed", limiter="minmod", bc=("outflow", "reflect", "periodic", "periodic")),
     dict(id="mc-flather-sponge", limiter="mc", bc=("flather", "flather", "periodic", "periodic"), harmonic=True,
          sponge=True),
+    # several tiles of the tiled reverse passes along both axes, the last ones partial
+    dict(id="mc-wide-wetdry", limiter="mc", bc=("periodic", "periodic", "reflect", "outflow"), wetdry=True,
+         nx=150, ny=45),
 ]
 
 
@@ -266,7 +269,7 @@ def test_adjoint_step_equals_host_composition(case):
     # oracle's stage states.  Equal up to the order of the atomic adds of the
     # boundary transpose.
     from test_adjoint_cpu import step_rev_composed
-    nx, ny, dx = 40, 32, 25.0
+    nx, ny, dx = case.get("nx", 40), case.get("ny", 32), 25.0
     s, adj, c = make(nx, ny, dx, limiter=case["limiter"], bc=case["bc"], nsteps=2, every=2,
                      wetdry=case.get("wetdry", False), harmonic=case.get("harmonic", False),
                      sponge=case.get("sponge", False))

@@ -56,13 +56,47 @@ __device__ __forceinline__ bool cell_of(const Params &P, int &i, int &j) {
   return j < P.H;
 }
 
-__global__ void k_adj_accum(Params P, StateP S, double dt, StateP lR, Work w) {
-  int i, j;
-  if (cell_of(P, i, j)) adj_accum_at(P, S, dt, lR, w, i, j);
-}
-__global__ void k_adj_faces_rev(Params P, const double *b, Work w) {
-  int i, j;
-  if (cell_of(P, i, j)) adj_faces_rev_at(P, b, w, i, j);
+// Accumulation^T + positivity^T and face^T over AF_X x AF_Y tiles, one thread
+// per cell: a cell's face flux-term adjoints stay in registers from the one pass
+// to the other, and only theta^T's lout crosses cells (face^T of a face whose
+// flow leaves the east or north neighbour reads that neighbour's): the tile keeps
+// its cells' lout in LDS and evaluates the accumulation pass again, the same
+// operations and so the same doubles, for the row north and the column east of
+// it.  Writes theta^T's lh and the face-state adjoints; tiles are dealt to the
+// XCDs in bands of tile rows as cell_of deals blocks.
+constexpr int AF_X = 64, AF_Y = 8;
+__global__ __launch_bounds__(AF_X * AF_Y) void k_adj_accum_faces_tiled(Params P, StateP S, const double *b,
+                                                                      double dt, StateP lR, Work w) {
+  __shared__ double lo[AF_Y + 1][AF_X + 1];   // lout, rows j0 .. j0 + AF_Y, columns i0 .. i0 + AF_X
+  const int tiles_x = (P.W + AF_X - 1) / AF_X, tiles_y = (P.H + AF_Y - 1) / AF_Y;
+  const int per = gridDim.x / NXCD;
+  const int blk = (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
+  const int tx = blk % tiles_x, ty = blk / tiles_x;
+  const int t = threadIdx.x, lx = t % AF_X, ly = t / AF_X;
+  const int i0 = tx * AF_X, j0 = ty * AF_Y, i = i0 + lx, j = j0 + ly;
+  const bool tile = ty < tiles_y;
+  const bool own = tile && i < P.W && j < P.H;
+  AccumOut a = {};
+  if (own) {
+    a = adj_accum_eval(P, S, dt, lR, w, i, j);
+    lo[ly][lx] = a.lout;
+    w.s[S_LHTH][j * P.W + i] = a.lh;
+  }
+  if (tile && ly == AF_Y - 1) {                         // the last wavefront: the row north of the tile
+    const int hj = j0 + AF_Y;
+    if (i < P.W && hj < P.H) lo[AF_Y][lx] = adj_accum_eval(P, S, dt, lR, w, i, hj).lout;
+  } else if (tile && ly == AF_Y - 2 && lx < AF_Y) {     // the one before: the column east of it
+    const int hi = i0 + AF_X, hj = j0 + lx;
+    if (hi < P.W && hj < P.H) lo[lx][AF_X] = adj_accum_eval(P, S, dt, lR, w, hi, hj).lout;
+  }
+  __syncthreads();
+  if (!own) return;
+  const double lf[8] = {a.lx[0], a.lx[1], a.lx[2], a.lx[3], a.ly[0], a.ly[1], a.ly[2], a.ly[3]};
+  double r[12];
+  // a neighbour's lout is read only for a face on the grid, whose neighbour is a grid cell
+  adj_faces_rev_eval(P, b, w, i, j, lf, lo[ly][lx], lo[ly][lx + 1], lo[ly + 1][lx], r);
+  const int k = j * P.W + i;
+  for (int q = 0; q < 12; ++q) w.s[S_LXETAL + q][k] = r[q];
 }
 // Gather^T + limiter^T over 64 x 4 tiles of cells, one thread per cell: each
 // cell's gather (cells_rev) runs once and leaves its slope adjoints in LDS, the
@@ -335,11 +369,11 @@ int residual_rev(swe_adj *a, const StateP &S, double dt, const StateP &lR, const
   swe_ctx *c = a->ctx;
   const Params &P = c->P;
   hipStream_t st = c->stream;
-  const dim3 g = grid_for(a->N), blk(256);
   Work w = a->w;
   for (int q = 0; q < REC_FIELDS; ++q) w.s[S_U + q] = rec + (size_t)q * a->N;
-  hipLaunchKernelGGL(k_adj_accum, g, blk, 0, st, P, S, dt, lR, w);
-  hipLaunchKernelGGL(k_adj_faces_rev, g, blk, 0, st, P, a->bed, w);
+  const size_t atiles = (size_t)((P.W + AF_X - 1) / AF_X) * ((P.H + AF_Y - 1) / AF_Y);
+  hipLaunchKernelGGL(k_adj_accum_faces_tiled, dim3((unsigned)((atiles + NXCD - 1) / NXCD * NXCD)),
+                     dim3(AF_X * AF_Y), 0, st, P, S, a->bed, dt, lR, w);
   const size_t tiles = (size_t)((P.W + CT_X - 1) / CT_X) * ((P.H + CT_Y - 1) / CT_Y);
   hipLaunchKernelGGL(k_adj_cells_slopes_tiled, dim3((unsigned)((tiles + NXCD - 1) / NXCD * NXCD)),
                      dim3(CT_X * CT_Y), 0, st, P, S, a->bed, lR, w, tcs, lS);

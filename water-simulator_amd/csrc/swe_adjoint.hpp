@@ -325,8 +325,9 @@ __host__ __device__ inline double face_lth_y(const Params &P, const StateP &lR, 
 // faces; then the positivity factor's adjoint gathered from the four faces it
 // scaled as their upwind cell (the west and south faces' shares recomputed from
 // their owners' data) and taken through theta = h / (out dt).
-__host__ __device__ inline void adj_accum_at(const Params &P, const StateP &S, double dt, const StateP &lR,
-                                             const Work &w, int i, int j) {
+struct AccumOut { double lx[4], ly[4], lh, lout; };   // east face, north face; theta^T
+__host__ __device__ inline AccumOut adj_accum_eval(const Params &P, const StateP &S, double dt, const StateP &lR,
+                                                   const Work &w, int i, int j) {
   const int k = j * P.W + i;
   double lx[4] = {0, 0, 0, 0}, ly[4] = {0, 0, 0, 0}, lthE = 0.0, lthN = 0.0, lthW = 0.0, lthS = 0.0;
   const double *th = w.s[S_TH];
@@ -363,30 +364,44 @@ __host__ __device__ inline void adj_accum_at(const Params &P, const StateP &S, d
   }
   if (won) lthW = face_lth_x(P, lR, w, k - 1, j);
   if (son) lthS = face_lth_y(P, lR, w, k - P.W, j - 1);
-  w.s[S_LXM][k] = lx[0]; w.s[S_LXNL][k] = lx[1]; w.s[S_LXNR][k] = lx[2]; w.s[S_LXT][k] = lx[3];
-  w.s[S_LYM][k] = ly[0]; w.s[S_LYNL][k] = ly[1]; w.s[S_LYNR][k] = ly[2]; w.s[S_LYT][k] = ly[3];
+  AccumOut r;
+  for (int q = 0; q < 4; ++q) { r.lx[q] = lx[q]; r.ly[q] = ly[q]; }
   // a face off the grid has zero flux and zero share
   double lth = 0.0;
   if (xM >= 0) lth += lthE;                    // east face
   if (i >= 1 && fw < 0) lth += lthW;           // west face
   if (yM >= 0) lth += lthN;                    // north face
   if (j >= 1 && fs < 0) lth += lthS;           // south face
-  double lh = 0.0, lout = 0.0;
+  r.lh = 0.0;
+  r.lout = 0.0;
   if (thc < 1.0 && h > 0 && out * dt > h) {
-    lh = lth / (out * dt);
-    lout = -lth * thc / out;
+    r.lh = lth / (out * dt);
+    r.lout = -lth * thc / out;
   }
-  w.s[S_LHTH][k] = lh;
-  w.s[S_LOUT][k] = lout;
+  return r;
+}
+__host__ __device__ inline void adj_accum_at(const Params &P, const StateP &S, double dt, const StateP &lR,
+                                             const Work &w, int i, int j) {
+  const int k = j * P.W + i;
+  const AccumOut a = adj_accum_eval(P, S, dt, lR, w, i, j);
+  w.s[S_LXM][k] = a.lx[0]; w.s[S_LXNL][k] = a.lx[1]; w.s[S_LXNR][k] = a.lx[2]; w.s[S_LXT][k] = a.lx[3];
+  w.s[S_LYM][k] = a.ly[0]; w.s[S_LYNL][k] = a.ly[1]; w.s[S_LYNR][k] = a.ly[2]; w.s[S_LYT][k] = a.ly[3];
+  w.s[S_LHTH][k] = a.lh;
+  w.s[S_LOUT][k] = a.lout;
 }
 
 // pass 5: face^T: the adjoints of both sides' reconstructed states of the
-// east x-face and north y-face of cell (i, j)
-__host__ __device__ inline void adj_faces_rev_at(const Params &P, const double *b, const Work &w, int i, int j) {
+// east x-face and north y-face of cell (i, j), given pass 4's flux-term adjoints
+// of the two faces (lf: M, N + pL, N + pR, T of the x-face, then of the y-face)
+// and theta^T's lout of the cell and of its east and north neighbours (read
+// only for a face on the grid); r: eta, u, v of the x-face's L and R sides, then
+// the y-face's
+__host__ __device__ inline void adj_faces_rev_eval(const Params &P, const double *b, const Work &w, int i, int j,
+                                                   const double (&lf)[8], double loutK, double loutE, double loutN,
+                                                   double (&r)[12]) {
   const int k = j * P.W + i;
   const HllcConst HC = {P.g, P.half_g, P.rg, P.min_depth, P.g * P.min_depth >= SQRT_N_MIN};
-  const double *lout = w.s[S_LOUT];
-  double r[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int q = 0; q < 12; ++q) r[q] = 0.0;
   for (int d = 0; d < 2; ++d) {
     // everything loaded before the branch (cell k stands in for a face off the grid)
     const bool on = d == 0 ? xface_on(P, i, j) : yface_on(P, i, j);
@@ -394,11 +409,10 @@ __host__ __device__ inline void adj_faces_rev_at(const Params &P, const double *
     const double divL = d == 0 ? P.geo.dx[j] : P.geo.dyN[j];
     const double divR = d == 0 ? P.geo.dx[j] : P.geo.dyS[on ? j + 1 : j];
     const double M = w.s[d == 0 ? S_FXM : S_FYM][k];
-    double lM = w.s[d == 0 ? S_LXM : S_LYM][k];
-    const double loutL = lout[k], loutR = lout[kR];
+    double lM = lf[4 * d];
+    const double loutL = loutK, loutR = d == 0 ? loutE : loutN;
     const FaceSide L = face_side(w, b, k, d, 1.0), R = face_side(w, b, kR, d, -1.0);
-    const int o = d == 0 ? S_LXNL : S_LYNL;
-    const double lNL = w.s[o][k], lNR = w.s[o + 1][k], lT = w.s[o + 2][k];
+    const double lNL = lf[4 * d + 1], lNR = lf[4 * d + 2], lT = lf[4 * d + 3];
     if (!on) continue;
     if (M > 0) lM += loutL / divL;
     if (M < 0) lM -= loutR / divR;
@@ -409,6 +423,15 @@ __host__ __device__ inline void adj_faces_rev_at(const Params &P, const double *
     q[0] = f.etaL; q[1] = d == 0 ? f.uL : f.vL; q[2] = d == 0 ? f.vL : f.uL;
     q[3] = f.etaR; q[4] = d == 0 ? f.uR : f.vR; q[5] = d == 0 ? f.vR : f.uR;
   }
+}
+// pass 5 reading pass 4's slots
+__host__ __device__ inline void adj_faces_rev_at(const Params &P, const double *b, const Work &w, int i, int j) {
+  const int k = j * P.W + i;
+  const double *lout = w.s[S_LOUT];
+  double lf[8], r[12];
+  for (int q = 0; q < 8; ++q) lf[q] = w.s[S_LXM + q][k];
+  const int kE = xface_on(P, i, j) ? k + 1 : k, kN = yface_on(P, i, j) ? k + P.W : k;
+  adj_faces_rev_eval(P, b, w, i, j, lf, lout[k], lout[kE], lout[kN], r);
   for (int q = 0; q < 12; ++q) w.s[S_LXETAL + q][k] = r[q];
 }
 

@@ -82,12 +82,16 @@ __global__ __launch_bounds__(AF_X * AF_Y) void k_adj_accum_faces_tiled(Params P,
     lo[ly][lx] = a.lout;
     w.s[S_LHTH][j * P.W + i] = a.lh;
   }
-  if (tile && ly == AF_Y - 1) {                         // the last wavefront: the row north of the tile
-    const int hj = j0 + AF_Y;
-    if (i < P.W && hj < P.H) lo[AF_Y][lx] = adj_accum_eval(P, S, dt, lR, w, i, hj).lout;
-  } else if (tile && ly == AF_Y - 2 && lx < AF_Y) {     // the one before: the column east of it
-    const int hi = i0 + AF_X, hj = j0 + lx;
-    if (hi < P.W && hj < P.H) lo[lx][AF_X] = adj_accum_eval(P, S, dt, lR, w, hi, hj).lout;
+  // the AF_X + AF_Y cells of the row north and the column east of the tile, on
+  // the threads of the last wavefronts (as few as hold them)
+  const int hq = t - (AF_X * AF_Y - 64 * ((AF_X + AF_Y + 63) / 64));
+  if (tile && hq >= 0 && hq < AF_X + AF_Y) {
+    const bool north = hq < AF_X;
+    const int hi = north ? i0 + hq : i0 + AF_X, hj = north ? j0 + AF_Y : j0 + hq - AF_X;
+    if (hi < P.W && hj < P.H) {
+      const double v = adj_accum_eval(P, S, dt, lR, w, hi, hj).lout;
+      if (north) lo[AF_Y][hq] = v; else lo[hq - AF_X][AF_X] = v;
+    }
   }
   __syncthreads();
   if (!own) return;

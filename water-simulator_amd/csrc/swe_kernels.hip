@@ -197,7 +197,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
   const HllcConst HC = {P.g, P.half_g, P.rg, P.min_depth, P.g * P.min_depth >= SQRT_N_MIN};
   const double md = P.min_depth;
   const bool o2 = P.order >= 2;
-  const bool bedOn = o2 || P.sphere;
+  // a sphere's rows differ (set_uniform_rows), so UNI instances are Cartesian: no
+  // geometric or spherical Coriolis terms, no equilibrium tide
+  const bool sph = !UNI && P.sphere;
+  const bool bedOn = o2 || sph;
   // the per-row values, interleaved per row (Params::geoA).  They are
   // wave-uniform and unchanged while a stage runs, so they are read through the
   // constant address space: scalar loads into SGPRs, waited on by their own
@@ -287,7 +290,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
       const double hPy = jmax0(fN.eta - fN.b), hMy = jmax0(fS.eta - fS.b);
       bedCur.hu = div_r(-P.half_g * (hPx + hMx) * sBx, rowv(DX, s), rowv(RDX, s));
       bedCur.h = div_r(-P.half_g * (hPy + hMy) * sBy, rowv(BED, s), rowv(RBED, s));   // rhv bed term
-      bedCur.hv = P.sphere ? rowv(GEOC, s) * P.g * (hPy * hPy + hMy * hMy) : 0.0;        // rhv geometric term
+      bedCur.hv = sph ? rowv(GEOC, s) * P.g * (hPy * hPy + hMy * hMy) : 0.0;        // rhv geometric term
     }
     // ---- y-face (s-1|s): v normal, u tangential (swe.mjs:448-475) ----------
     FluxF fyCur = {0.0, 0.0, 0.0, 0.0};
@@ -373,9 +376,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
         if (bedOn) {
           rhu += bedPrev2.hu;
           rhv += bedPrev2.h;
-          if (P.sphere) rhv += bedPrev2.hv;
+          if (sph) rhv += bedPrev2.hv;
         }
-        if (P.sphere) {                                   // swe.mjs:600-619
+        if (sph) {                                        // swe.mjs:600-619
           const double fe = rowv(FROW, o) + Uo * rowv(GEO_TANR, o);   // tanPhi / R of the row
           rhu += fe * cur.hv;
           rhv -= fe * cur.hu;
@@ -385,7 +388,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
         }
         // equilibrium tide: g h grad(eta_eq), eta_eq = sum A cos^2(phi) cos(w t + 2 lon + ph)
         // (semidiurnal) or A sin(2 phi) cos(w t + lon + ph) (diurnal); no reference counterpart
-        for (int c = 0; c < P.tide_n; ++c) {
+        for (int c = 0; c < (UNI ? 0 : P.tide_n); ++c) {
           const double *T = P.tide + (size_t)c * (2 * H + 2 * W);
           const double cl = T[2 * H + gi], sl = T[2 * H + W + gi];
           const double tc = tcs[2 * c], ts = tcs[2 * c + 1];
@@ -413,7 +416,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
         if (interior) {
           if (nh < 0) { mf -= nh; nh = 0.0; }
           if (nh <= md) { nhu = 0.0; nhv = 0.0; }
-          if (A.rec) { A.rec[k] = nh; A.rec[k + fs] = nhu; A.rec[k + 2 * fs] = nhv; }
+          if (REC && A.rec) { A.rec[k] = nh; A.rec[k + fs] = nhu; A.rec[k + 2 * fs] = nhv; }
           // applyFriction (swe.mjs:659-704)
           if (P.friction) {
             if (nh <= md) { nhu = 0.0; nhv = 0.0; }

@@ -112,9 +112,11 @@ constexpr int GEO_AOS = 16;
 // which the reference forms once per row (swe.mjs:610) and the stage kernels read
 constexpr int GEO_TANR = 14;
 
-// Params::geo_uni from the interleaved rows a[j * GEO_AOS + k], j < H
+// Params::geo_uni from the interleaved rows a[j * GEO_AOS + k], j < H.  Never
+// set on a sphere (whose rows differ by latitude anyway): the uniform-row stage
+// kernels leave out the spherical terms and the equilibrium tide.
 inline void set_uniform_rows(Params &P, const double *a, int H) {
-  bool uni = true;
+  bool uni = !P.sphere;
   for (int j = 1; j < H && uni; ++j)
     if (__builtin_memcmp(&a[(size_t)j * GEO_AOS], a, sizeof(double) * GEO_AOS) != 0) uni = false;
   P.geo_uni = uni ? 1 : 0;

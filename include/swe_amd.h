@@ -342,6 +342,14 @@ int swe_adj_host_bc_rev(const swe_desc *d, const int32_t bc[4], const double ext
 int swe_adj_host_bc_rev_gather(const swe_desc *d, const int32_t bc[4], const double ext[8], const double *h,
                                const double *hu, const double *hv, const double *b, double *lh, double *lhu,
                                double *lhv);
+/* The same transpose folded on read, as the device's reverse epilogues apply it
+ * in the configurations of the gather form (SWE_EINVAL otherwise): each interior
+ * cell adds the folds of the ghosts whose source it is, in the loop's order, from
+ * the ghosts as given; the ghosts are then cleared.  ext, h, hu, hv and b are not
+ * read.  No counterpart in the reference: a host check of the device pass. */
+int swe_adj_host_bc_rev_onread(const swe_desc *d, const int32_t bc[4], const double ext[8], const double *h,
+                               const double *hu, const double *hv, const double *b, double *lh, double *lhu,
+                               double *lhv);
 int swe_adj_host_epilogue_rev(const swe_desc *d, int stage, const double *pre_h, const double *pre_hu,
                               const double *pre_hv, const double *nfield, const double *sponge, double dt,
                               const double *lout_h,

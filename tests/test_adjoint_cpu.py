@@ -249,9 +249,10 @@ GATHER_SETS = [
 @pytest.mark.parametrize("shape", [(9, 7), (4, 4), (17, 5)])
 def test_bc_rev_gather_form_equals_the_loop(kinds, shape):
     """The device's gather form of BC^T (one work item per source column or row,
-    no atomics) gives the ghost-by-ghost loop's result to the bit; it is refused
-    for a periodic south or north side and for Flather sides (those run the
-    ghost-by-ghost form)."""
+    no atomics) and the same transpose folded on read by each interior cell give
+    the ghost-by-ghost loop's result to the bit; both are refused for a periodic
+    south or north side and for Flather sides (those run the ghost-by-ghost
+    form)."""
     nx, ny = shape
     rows = swe_amd.geometry_cartesian(nx, ny, 2.0)
     desc, keep = swe_amd.make_desc(nx, ny, rows, manning=0.0)
@@ -269,9 +270,14 @@ def test_bc_rev_gather_form_equals_the_loop(kinds, shape):
     gath = swe_amd.bc_rev_host(desc, kinds, *lam, ext=ext, state=(b, h, hu, hv), gather=True)
     for x, y in zip(loop, gath):
         assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
+    # folded on read, as the device's reverse epilogues apply it
+    onread = swe_amd.bc_rev_host(desc, kinds, *lam, onread=True)
+    for x, y in zip(loop, onread):
+        assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
     for refused in (("reflect", "reflect", "periodic", "periodic"), ("flather", "reflect", "reflect", "reflect")):
-        with pytest.raises(swe_amd.SweError):
-            swe_amd.bc_rev_host(desc, refused, *lam, ext=ext, state=(b, h, hu, hv), gather=True)
+        for form in ("gather", "onread"):
+            with pytest.raises(swe_amd.SweError):
+                swe_amd.bc_rev_host(desc, refused, *lam, ext=ext, state=(b, h, hu, hv), **{form: True})
 
 
 # ---- dryClean^T / friction^T / sponge^T -----------------------------------------------

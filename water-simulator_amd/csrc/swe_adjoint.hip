@@ -165,9 +165,9 @@ __global__ void k_adj_bc_gather(Params P, StateW l, int pass) {
   if (q < (pass == 0 ? 2 * (P.W - 4) : 2 * P.ny)) adj_bc_gather(P, l, pass, q);
 }
 __global__ void k_adj_epilogue(Params P, StateP pre, StateP lOut, const double *nfield, const double *sponge,
-                               double dt, double *lN, int stage, StateW lA, StateW lB, StateW lR) {
+                               double dt, double *lN, int stage, StateW lA, StateW lB, StateW lR, int fold) {
   int i, j;
-  if (cell_of(P, i, j)) adj_epilogue_rev_at(P, pre, lOut, nfield, sponge, dt, lN, stage, lA, lB, lR, i, j);
+  if (cell_of(P, i, j)) adj_epilogue_rev_at(P, pre, lOut, nfield, sponge, dt, lN, stage, lA, lB, lR, i, j, fold != 0);
 }
 
 __global__ void k_fill(size_t n, double *a, double v) {
@@ -296,6 +296,9 @@ struct swe_adj {
   int ng = 0, every = 1, nsamp = 0;
   int lam = 0, lap = 0;            // scratch slots holding the running adjoint and lA'
   bool recorded = false;
+  // the adjoint slot lam still holds the ghost adjoints of the step after it
+  // (BC^T folded on read by the next stage-2 epilogue, adj_bc_fold_into)
+  bool pending = false;
   std::vector<double> h_dt;
   std::vector<double> h_eta;
   double loss = 0.0;
@@ -498,18 +501,22 @@ int step_rev(swe_adj *a, int buf, int64_t q, int64_t n) {
   const StateW lR = tslot(a, T_R), lBp = tslot(a, T_LBP), lam = tslot(a, a->lam), lAp = tslot(a, a->lap);
   const StateW none = {nullptr, nullptr, nullptr};
   const double *tcs = tcs_of(a, buf, q);
+  // BC^T of a stage runs as its own pass, or (the configurations of the gather
+  // form) is folded on read by the epilogue that next reads that adjoint
+  const bool defer = adj_bc_gather_ok(P);
   int rc;
   // stage 2: friction^T, dryClean^T, C0 = 0.5 (A' + B' + dt R2)^T, residual^T, BC^T
   hipLaunchKernelGGL(k_adj_epilogue, g, blk, 0, st, P, sp(dc), sp(lam), c->mfield, c->sponge, dt, a->lN, 2, lAp,
-                     lBp, lR);
+                     lBp, lR, a->pending ? 1 : 0);
   if ((rc = residual_rev(a, sp(Bp), dt, sp(lR), lBp, tcs + 2 * MAX_TIDE, recslot(a, buf, q, 2)))) return rc;
-  if ((rc = bc_rev(a, lBp, sp(Bp), n))) return rc;
+  if (!defer && (rc = bc_rev(a, lBp, sp(Bp), n))) return rc;
   // stage 1: dryClean^T (B's interior is stage 1's dryCleaned output), B0 = A' + dt R1 ^T,
   // residual^T, BC^T
   hipLaunchKernelGGL(k_adj_epilogue, g, blk, 0, st, P, sp(Bp), sp(lBp), c->mfield, c->sponge, dt, a->lN, 1, lAp,
-                     none, lR);
+                     none, lR, defer ? 1 : 0);
   if ((rc = residual_rev(a, sp(Ap), dt, sp(lR), lAp, tcs, recslot(a, buf, q, 1)))) return rc;
-  if ((rc = bc_rev(a, lAp, sp(Ap), n))) return rc;
+  if (!defer && (rc = bc_rev(a, lAp, sp(Ap), n))) return rc;
+  a->pending = defer;
   std::swap(a->lam, a->lap);
   return SWE_OK;
 }
@@ -821,6 +828,7 @@ int swe_adj_backward(swe_adj *a, double *grad_h0, double *grad_n) {
   int rc;
   hipLaunchKernelGGL(k_fill, grid_for(N3), dim3(256), 0, st, N3, tslot(a, a->lam).h, 0.0);
   hipLaunchKernelGGL(k_fill, grid_for(N), dim3(256), 0, st, N, a->lN, 0.0);
+  a->pending = false;
   if ((rc = inject(a, a->nsteps))) return rc;
   // the reverse passes read a private copy of the bed (ghost ring as the forward
   // leaves it); the replay stream keeps writing the context's own
@@ -865,6 +873,12 @@ int swe_adj_backward(swe_adj *a, double *grad_h0, double *grad_n) {
       if ((rc = inject(a, n))) return rc;
     }
     ACHK(a, hipEventRecord(a->reversed[buf], st));
+  }
+  // the first step's BC^T, when folded on read, has no epilogue after it
+  if (a->pending) {
+    const StateW lam = tslot(a, a->lam);
+    if ((rc = bc_rev(a, lam, sp(lam), 0))) return rc;
+    a->pending = false;
   }
   ACHK(a, hipEventRecord(t.e1, st));
   ACHK(a, hipEventSynchronize(t.e1));
@@ -927,6 +941,31 @@ int swe_adj_host_bc_rev_gather(const swe_desc *d, const int32_t bc[4], const dou
   const StateP S = {h, hu, hv};
   for (int q = 0; q < 2 * (P.W - 4); ++q) adj_bc_gather(P, l, 0, q);
   for (int q = 0; q < 2 * P.ny; ++q) adj_bc_gather(P, l, 1, q);
+  return SWE_OK;
+}
+
+// The same transpose folded on read (adj_bc_fold_into), as the device's
+// epilogues apply it: every interior cell's folds from a snapshot of the ghosts,
+// the ghosts then cleared, for checking it against the loop above.
+int swe_adj_host_bc_rev_onread(const swe_desc *d, const int32_t bc[4], const double ext[8], const double *h,
+                               const double *hu, const double *hv, const double *b, double *lh, double *lhu,
+                               double *lhv) {
+  (void)ext; (void)h; (void)hu; (void)hv; (void)b;
+  Params P;
+  if (!bc || host_params(d, bc, P) || !lh || !lhu || !lhv) return SWE_EINVAL;
+  if (!adj_bc_gather_ok(P)) return SWE_EINVAL;
+  const size_t N = (size_t)P.W * P.H;
+  const std::vector<double> sh(lh, lh + N), shu(lhu, lhu + N), shv(lhv, lhv + N);
+  const StateP l = {sh.data(), shu.data(), shv.data()};
+  for (int j = 0; j < P.H; ++j)
+    for (int i = 0; i < P.W; ++i) {
+      const size_t k = (size_t)j * P.W + i;
+      if (i >= 2 && i <= P.W - 3 && j >= 2 && j <= P.H - 3) {
+        adj_bc_fold_into(P, l, i, j, lh[k], lhu[k], lhv[k]);
+      } else {
+        lh[k] = 0.0; lhu[k] = 0.0; lhv[k] = 0.0;
+      }
+    }
   return SWE_OK;
 }
 

@@ -809,6 +809,39 @@ __host__ __device__ inline bool adj_bc_gather_ok(const Params &P) {
     if (P.bc[k] == BC_FLATHER) return false;
   return P.bc[2] != BC_PERIODIC && P.bc[3] != BC_PERIODIC && P.nx >= 4 && P.ny >= 4;
 }
+// BC^T folded on read, for the configurations of adj_bc_gather: adds to (h, hu,
+// hv), the adjoint of interior cell (i, j) held in l, the folds of the ghosts of
+// l whose source it is, in the order of the ghost-by-ghost loop (the
+// south/north pass's ghosts by row, then column; then the west/east pass's by
+// column), so the sum is adj_bc_gather's to the bit.  The ghosts are read, not
+// cleared.  Only cells within two of an edge of the interior have any.
+__host__ __device__ inline void adj_bc_fold_into(const Params &P, const StateP &l, int i, int j, double &h,
+                                                 double &hu, double &hv) {
+  const int W = P.W, H = P.H, k = j * W + i;
+  if (j <= 3 || j >= H - 4) {                  // south/north pass: ghost rows 0, 1 or H - 2, H - 1
+    const int r0 = j <= 3 ? 0 : 2;
+    // the ghost columns that can have (i, j) as source, in ascending order: its
+    // own and, at the first and last interior column, the corner columns
+    const int c0 = i == 2 ? 0 : i, nc = (i == 2 || i == W - 3) ? 3 : 1;
+    for (int rr = r0; rr < r0 + 2; ++rr)
+      for (int c = 0; c < nc; ++c) {
+        int g, s, side;
+        adj_bc_cells(P, 0, rr * W + c0 + c, g, s, side);
+        if (s != k) continue;
+        const GhostFold f = adj_bc_fold_linear(P, side, l.h[g], l.hu[g], l.hv[g]);
+        h += f.dh; hu += f.dhu; hv += f.dhv;
+      }
+  }
+  if (i <= 3 || i >= W - 4) {                  // west/east pass: ghost columns 0, 1, W - 2, W - 1 of row j
+    for (int cc = 0; cc < 4; ++cc) {
+      int g, s, side;
+      adj_bc_cells(P, 1, cc * P.ny + (j - 2), g, s, side);
+      if (s != k) continue;
+      const GhostFold f = adj_bc_fold_linear(P, side, l.h[g], l.hu[g], l.hv[g]);
+      h += f.dh; hu += f.dhu; hv += f.dhv;
+    }
+  }
+}
 
 // ---- dryClean^T, friction^T and sponge^T (swe.mjs:659-704, :734-814, :1188-1197) ----
 // Reverse of a stage's pointwise part at cell k, given the adjoint lOut of its
@@ -824,7 +857,7 @@ __host__ __device__ inline bool adj_bc_gather_ok(const Params &P) {
 __host__ __device__ inline void adj_epilogue_rev_at(const Params &P, const StateP &dc, const StateP &lOut,
                                                     const double *nfield, const double *sponge, double dt,
                                                     double *lN, int stage, const StateW &lA, const StateW &lB,
-                                                    const StateW &lR, int i, int j) {
+                                                    const StateW &lR, int i, int j, bool fold = false) {
   const int k = j * P.W + i;
   double lh = 0.0, lhu = 0.0, lhv = 0.0;
   if (i >= 2 && i <= P.W - 3 && j >= 2 && j <= P.H - 3) {
@@ -832,6 +865,9 @@ __host__ __device__ inline void adj_epilogue_rev_at(const Params &P, const State
     const bool floored = h == 0.0;
     const bool dry = h <= P.min_depth;
     lh = lOut.h[k]; lhu = lOut.hu[k]; lhv = lOut.hv[k];
+    // fold: lOut still holds its ghosts' adjoints (BC^T of the stage output,
+    // adj_bc_fold_into)
+    if (fold) adj_bc_fold_into(P, lOut, i, j, lh, lhu, lhv);
     if (stage == 2 && P.sponge && sponge) {
       const double str = sponge[k];
       if (str > 0) {

@@ -717,6 +717,7 @@ def _bind_adjoint(L):
         "swe_adj_host_residual_rev": (i, [D, dp, dp, dp, dp, d, dp, dp, dp, dp, dp, dp]),
         "swe_adj_host_bc_rev": (i, [D, ip, dp, dp, dp, dp, dp, dp, dp, dp]),
         "swe_adj_host_bc_rev_gather": (i, [D, ip, dp, dp, dp, dp, dp, dp, dp, dp]),
+        "swe_adj_host_bc_rev_onread": (i, [D, ip, dp, dp, dp, dp, dp, dp, dp, dp]),
         "swe_adj_host_epilogue_rev": (i, [D, i, dp, dp, dp, dp, dp, d, dp, dp, dp, dp, dp, dp, dp]),
         "swe_adj_host_hllc_rev": (None, [dp, d, d, d, d, d, d, d, d, d, d, d]),
     }
@@ -764,10 +765,12 @@ def residual_rev_host(desc, h, hu, hv, b, dt, lrh, lrhu, lrhv):
     return out
 
 
-def bc_rev_host(desc, bc, lh, lhu, lhv, ext=None, state=None, gather=False):
+def bc_rev_host(desc, bc, lh, lhu, lhv, ext=None, state=None, gather=False, onread=False):
     """BC^T of applyBC() (swe.mjs:289-303) on copies; a flather side needs the
     far-field values ext (8) and the state (b, h, hu, hv) applyBC read.  gather:
-    the form the device runs when the south and north sides are not periodic."""
+    the form the device runs when no side is Flather and the south and north
+    sides are not periodic; onread: the same folded on read, as the device's
+    reverse epilogues apply it there."""
     L = lib()
     _bind_adjoint(L)
     kinds = (ctypes.c_int32 * 4)(*[BC[k] if isinstance(k, str) else k for k in bc])
@@ -775,10 +778,11 @@ def bc_rev_host(desc, bc, lh, lhu, lhv, ext=None, state=None, gather=False):
     e = None if ext is None else _ext(ext)
     st = [None] * 4 if state is None else [np.ascontiguousarray(x, dtype=np.float64) for x in state]
     ptr = lambda x: None if x is None else _dp(x)
-    fn = L.swe_adj_host_bc_rev_gather if gather else L.swe_adj_host_bc_rev
+    name = "swe_adj_host_bc_rev" + ("_onread" if onread else "_gather" if gather else "")
+    fn = getattr(L, name)
     rc = fn(ctypes.byref(desc), kinds, ptr(e), ptr(st[1]), ptr(st[2]), ptr(st[3]), ptr(st[0]), *[_dp(x) for x in out])
     if rc != 0:
-        raise SweError(f"swe_adj_host_bc_rev{'_gather' if gather else ''}: {rc}")
+        raise SweError(f"{name}: {rc}")
     return out
 
 

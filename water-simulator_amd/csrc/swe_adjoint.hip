@@ -863,16 +863,20 @@ int swe_adj_backward(swe_adj *a, double *grad_h0, double *grad_n) {
     if (pstride && i > 0 && (int64_t)i % pstride == 0) progress(st, "backward segments", (int64_t)i, (int64_t)segs.size());
     const int buf = (int)(i % 2);
     ACHK(a, hipStreamWaitEvent(st, a->replayed[buf], 0));
-    if (i + 1 < segs.size()) {
-      ACHK(a, hipStreamWaitEvent(a->rs, a->reversed[1 - buf], 0));
-      if ((rc = issue_replay(i + 1))) return rc;
-      ACHK(a, hipEventRecord(a->replayed[1 - buf], a->rs));
-    }
     for (int64_t n = segs[i].n1 - 1; n >= segs[i].n0; --n) {
       if ((rc = step_rev(a, buf, n - segs[i].n0, n))) return rc;
       if ((rc = inject(a, n))) return rc;
     }
     ACHK(a, hipEventRecord(a->reversed[buf], st));
+    // the replay of the next segment into the other store, once the segment
+    // before this one (which read that store) is reversed: it runs beside this
+    // segment's reverse passes.  Issued after them, so that the time the host
+    // takes to issue it (or to wait on the event) never holds them back.
+    if (i + 1 < segs.size()) {
+      ACHK(a, hipStreamWaitEvent(a->rs, a->reversed[1 - buf], 0));
+      if ((rc = issue_replay(i + 1))) return rc;
+      ACHK(a, hipEventRecord(a->replayed[1 - buf], a->rs));
+    }
   }
   // the first step's BC^T, when folded on read, has no epilogue after it
   if (a->pending) {

@@ -9,9 +9,11 @@
 // the RK combination^T, residual^T of stage 2, BC^T, then the same for stage 1.
 // residual^T runs as one-thread-per-cell passes over the grid with their
 // intermediates in HBM (swe_adjoint.hpp): on the host all seven, on the device
-// three (accumulation + positivity^T, face^T, gather + limiter^T), since the
-// replayed stage sweeps record the velocities, surface, slopes, face flux terms
-// and positivity factors that the other passes recompute.
+// two tiled ones (accumulation + positivity^T + face^T, gather + limiter^T),
+// since the replayed stage sweeps record the velocities, surface, slopes, face
+// flux terms and positivity factors that the other passes recompute.  BC^T runs
+// as passes of its own, or folded on read by the next epilogue (adj_bc_fold_into)
+// where no side is Flather and south and north are not periodic.
 //
 // Time: the forward records dt per step on the device and keeps the state every
 // K steps (K ~ sqrt(nsteps)); the reverse sweep re-runs each segment from its

@@ -158,11 +158,13 @@ __device__ __forceinline__ FluxF face_flux(const Face4 &L, const Face4 &R, const
 // REC: compile in the optional recording for the adjoint replay (A.recf);
 // LIM: the slope limiter (Params::limiter), fixed at compile time;
 // UNI: every row's metric values are the same (Params::geo_uni), read once.
+// LEAN (stage 2 of a plain launch): no per-cell Manning field, no sponge, and
+// neither theta nor the residual written out, so none of them is compiled in.
 // sweep_ring: the wavefront's LDS (one per workgroup of one wavefront, however
 // many bodies a kernel instantiates): the last four rows' (b, h, hu, hv) of its
 // columns, which the cell update of row r-3 reads instead of memory
 __shared__ double sweep_ring[4][4][SWEEP_LANES];
-template <int MODE, bool REC, int LIM, bool UNI = false>
+template <int MODE, bool REC, int LIM, bool UNI = false, bool LEAN = false>
 __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int chunk, int wave_id) {
   double (&ring)[4][4][SWEEP_LANES] = sweep_ring;
   const Params &P = A.P;
@@ -421,7 +423,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
           if (P.friction) {
             if (nh <= md) { nhu = 0.0; nhv = 0.0; }
             else {
-              const double n = P.friction == 2 ? A.mfield[k] : P.manning;
+              const double n = (!LEAN && P.friction == 2) ? A.mfield[k] : P.manning;
               if (n != 0.0 && n == n) {
                 const Vel2 uv = vel2(nhu, nhv, nh, P.eps4);
                 const double speed = js_hypot(uv.u, uv.v);
@@ -434,7 +436,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
             }
           }
           // makeSponge forcing body (swe.mjs:1188-1197)
-          if (P.sponge) {
+          if (!LEAN && P.sponge) {
             const double str = A.sponge[k];
             if (str > 0) {
               const double f = jmin(1.0, str * dt * 60);
@@ -458,8 +460,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
         }
         if (!(__builtin_isfinite(nh) && __builtin_isfinite(nhu) && __builtin_isfinite(nhv))) bad = true;
         outp[k] = nh; outp[k + fs] = nhu; outp[k + 2 * fs] = nhv;
-        if (A.P.want_theta) A.theta[k] = thPrev;
-        if (A.P.want_rh) { A.rh[k] = rh; A.rh[k + fs] = rhu; A.rh[k + 2 * fs] = rhv; }
+        if (!LEAN && A.P.want_theta) A.theta[k] = thPrev;
+        if (!LEAN && A.P.want_rh) { A.rh[k] = rh; A.rh[k + fs] = rhu; A.rh[k + 2 * fs] = rhv; }
       }
     }
 
@@ -487,9 +489,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
   }
 }
 
-template <int MODE, int LIM, bool REC, bool UNI>
+template <int MODE, int LIM, bool REC, bool UNI, bool LEAN = false>
 __global__ __launch_bounds__(SWEEP_LANES) void k_sweep(SweepArgs A) {
-  sweep_body<MODE, REC, LIM, UNI>(A, blockIdx.x, blockIdx.y, blockIdx.y * gridDim.x + blockIdx.x);
+  sweep_body<MODE, REC, LIM, UNI, LEAN>(A, blockIdx.x, blockIdx.y, blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 // Batched ensembles: a persistent grid of wavefronts pulls (basin, strip, chunk)
@@ -800,26 +802,31 @@ __global__ void k_finalize_batch(const BasinDev *__restrict__ basins, int nb, un
 // ---- launchers (host side, C++ linkage inside the library) ------------------
 // the stage kernel for a limiter; the recording variant only when the launch
 // records (the residual probe never does), the uniform-row variant for grids
-// whose rows are all alike when it does not
+// whose rows are all alike when it does not, and of that at stage 2 the lean
+// one when the launch has none of the per-cell Manning field, the sponge and the
+// theta / residual outputs.  (A captured step graph holds the kernel chosen at
+// capture: every setter that changes one of these conditions drops the graphs.)
 template <int MODE, int LIM>
-static const void *sweep_fn(bool rec, bool uni) {
+static const void *sweep_fn(bool rec, bool uni, bool lean) {
   if (MODE != MODE_PROBE && rec) return reinterpret_cast<const void *>(&k_sweep<MODE, LIM, MODE != MODE_PROBE, false>);
+  if (MODE == MODE_STAGE2 && uni && lean) return reinterpret_cast<const void *>(&k_sweep<MODE, LIM, false, true, true>);
   if (uni) return reinterpret_cast<const void *>(&k_sweep<MODE, LIM, false, true>);
   return reinterpret_cast<const void *>(&k_sweep<MODE, LIM, false, false>);
 }
 template <int MODE>
-static const void *sweep_fn(int lim, bool rec, bool uni) {
-  if (lim == LIM_MINMOD) return sweep_fn<MODE, LIM_MINMOD>(rec, uni);
-  if (lim == LIM_CENTRAL) return sweep_fn<MODE, LIM_CENTRAL>(rec, uni);
-  return sweep_fn<MODE, LIM_MC>(rec, uni);
+static const void *sweep_fn(int lim, bool rec, bool uni, bool lean = false) {
+  if (lim == LIM_MINMOD) return sweep_fn<MODE, LIM_MINMOD>(rec, uni, lean);
+  if (lim == LIM_CENTRAL) return sweep_fn<MODE, LIM_CENTRAL>(rec, uni, lean);
+  return sweep_fn<MODE, LIM_MC>(rec, uni, lean);
 }
 
 hipError_t launch_sweep(int mode, const SweepArgs &a, hipStream_t st) {
   dim3 grid((a.P.W + SWEEP_OUT - 1) / SWEEP_OUT, (a.P.H + a.rows_per_chunk - 1) / a.rows_per_chunk);
   dim3 block(SWEEP_LANES);
   const bool rec = a.recf != nullptr, uni = a.P.geo_uni != 0;
+  const bool lean = a.P.friction != 2 && !a.P.sponge && !a.P.want_theta && !a.P.want_rh;
   const void *fn = mode == MODE_STAGE1   ? sweep_fn<MODE_STAGE1>(a.P.limiter, rec, uni)
-                   : mode == MODE_STAGE2 ? sweep_fn<MODE_STAGE2>(a.P.limiter, rec, uni)
+                   : mode == MODE_STAGE2 ? sweep_fn<MODE_STAGE2>(a.P.limiter, rec, uni, lean)
                                          : sweep_fn<MODE_PROBE>(a.P.limiter, false, uni);
   SweepArgs arg = a;
   void *args[] = {&arg};

@@ -276,8 +276,9 @@ int swe_batch_use_graph(swe_batch *b, int on);
  * steps any basin has taken in total */
 int swe_batch_advance(swe_batch *b, int64_t nsteps, int64_t *min_steps);
 int swe_batch_last_advance_ms(swe_batch *b, double *ms);
-/* autotuner: times `steps` steps for each (rows, wavefronts) candidate on a copy
- * of the current state, restores the state, keeps the fastest */
+/* autotuner: times `steps` steps (the faster of two runs, after one untimed step)
+ * for each (rows, wavefronts) candidate on a copy of the current state, restores
+ * the state, keeps the fastest */
 int swe_batch_autotune(swe_batch *b, int steps, int *best_rows, int *best_waves);
 
 /* ---- discrete adjoint (no reference counterpart: the reference has no

@@ -498,9 +498,10 @@ __global__ __launch_bounds__(SWEEP_LANES) void k_sweep(SweepArgs A) {
 // tasks from a queue until it is empty, so basins of very different sizes share
 // one launch and the device stays full to the last task.  LIMS >= 0: every
 // basin uses that slope limiter (and with UNI every basin's rows are alike), so
-// the kernel holds that one body; LIMS < 0: each task's limiter is switched on
-// in the kernel (three bodies, per-row metric).
-template <int MODE, int LIMS, bool UNI>
+// the kernel holds that one body (LEAN: at stage 2, no basin has a per-cell
+// Manning field or a sponge); LIMS < 0: each task's limiter is switched on in the
+// kernel (three bodies, per-row metric).
+template <int MODE, int LIMS, bool UNI, bool LEAN = false>
 __global__ __launch_bounds__(SWEEP_LANES, 3) void k_sweep_batch(const BasinDev *__restrict__ basins,
                                                              const Task *__restrict__ tasks, int ntasks,
                                                              unsigned *qctr, int parity) {
@@ -516,7 +517,7 @@ __global__ __launch_bounds__(SWEEP_LANES, 3) void k_sweep_batch(const BasinDev *
     const int wave = __builtin_amdgcn_readfirstlane(tasks[t].wave);
     const SweepArgs &A = basins[basin].sw[parity][MODE - 1];
     if constexpr (LIMS >= 0) {
-      sweep_body<MODE, false, LIMS, UNI>(A, strip, chunk, wave);
+      sweep_body<MODE, false, LIMS, UNI, LEAN>(A, strip, chunk, wave);
     } else {
       const int lim = __builtin_amdgcn_readfirstlane(A.P.limiter);
       if (lim == LIM_MC) sweep_body<MODE, false, LIM_MC>(A, strip, chunk, wave);
@@ -839,31 +840,37 @@ int sweep_waves(const Params &P, int rows_per_chunk) {
   return ((P.W + SWEEP_OUT - 1) / SWEEP_OUT) * ((P.H + rows_per_chunk - 1) / rows_per_chunk);
 }
 
-// the batch stage kernel for the ensemble's common limiter (lim_all < 0: mixed)
-// and whether all its basins have uniform rows
+// the batch stage kernel for the ensemble's common limiter (lim_all < 0: mixed),
+// whether all its basins have uniform rows, and (stage 2) whether none has a
+// per-cell Manning field or a sponge
+template <int MODE, int LIM>
+static const void *sweep_batch_one(bool uni_all, bool lean_all) {
+  if (MODE == MODE_STAGE2 && lean_all)
+    return uni_all ? reinterpret_cast<const void *>(&k_sweep_batch<MODE, LIM, true, true>)
+                   : reinterpret_cast<const void *>(&k_sweep_batch<MODE, LIM, false, true>);
+  return uni_all ? reinterpret_cast<const void *>(&k_sweep_batch<MODE, LIM, true>)
+                 : reinterpret_cast<const void *>(&k_sweep_batch<MODE, LIM, false>);
+}
 template <int MODE>
-static const void *sweep_batch_fn(int lim_all, bool uni_all) {
+static const void *sweep_batch_fn(int lim_all, bool uni_all, bool lean_all) {
   switch (lim_all) {
-    case LIM_MC: return uni_all ? reinterpret_cast<const void *>(&k_sweep_batch<MODE, LIM_MC, true>)
-                                : reinterpret_cast<const void *>(&k_sweep_batch<MODE, LIM_MC, false>);
-    case LIM_MINMOD: return uni_all ? reinterpret_cast<const void *>(&k_sweep_batch<MODE, LIM_MINMOD, true>)
-                                    : reinterpret_cast<const void *>(&k_sweep_batch<MODE, LIM_MINMOD, false>);
-    case LIM_CENTRAL: return uni_all ? reinterpret_cast<const void *>(&k_sweep_batch<MODE, LIM_CENTRAL, true>)
-                                     : reinterpret_cast<const void *>(&k_sweep_batch<MODE, LIM_CENTRAL, false>);
+    case LIM_MC: return sweep_batch_one<MODE, LIM_MC>(uni_all, lean_all);
+    case LIM_MINMOD: return sweep_batch_one<MODE, LIM_MINMOD>(uni_all, lean_all);
+    case LIM_CENTRAL: return sweep_batch_one<MODE, LIM_CENTRAL>(uni_all, lean_all);
     default: return reinterpret_cast<const void *>(&k_sweep_batch<MODE, -1, false>);
   }
 }
 
 hipError_t launch_sweep_batch(int mode, const BasinDev *basins, const Task *tasks, int ntasks, unsigned *qctr,
-                              int parity, int nwaves, int lim_all, bool uni_all, hipStream_t st) {
-  const void *fn = mode == MODE_STAGE1 ? sweep_batch_fn<MODE_STAGE1>(lim_all, uni_all)
-                                       : sweep_batch_fn<MODE_STAGE2>(lim_all, uni_all);
+                              int parity, int nwaves, int lim_all, bool uni_all, bool lean_all, hipStream_t st) {
+  const void *fn = mode == MODE_STAGE1 ? sweep_batch_fn<MODE_STAGE1>(lim_all, uni_all, false)
+                                       : sweep_batch_fn<MODE_STAGE2>(lim_all, uni_all, lean_all);
   unsigned *q = mode == MODE_STAGE1 ? qctr : qctr + 1;
   void *args[] = {&basins, &tasks, &ntasks, &q, &parity};
   return hipLaunchKernel(fn, dim3(nwaves), dim3(SWEEP_LANES), args, 0, st);
 }
 
-const void *k_sweep_batch_ptr() { return sweep_batch_fn<MODE_STAGE1>(-1, false); }
+const void *k_sweep_batch_ptr() { return sweep_batch_fn<MODE_STAGE1>(-1, false, false); }
 
 hipError_t launch_apply_bc_batch(const BasinDev *basins, int nb, int max_ghost, int parity, int stage, hipStream_t st) {
   hipLaunchKernelGGL(k_apply_bc_batch, dim3((max_ghost + 255) / 256, nb), dim3(256), 0, st, basins, parity, stage);

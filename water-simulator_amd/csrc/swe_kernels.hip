@@ -803,14 +803,16 @@ __global__ void k_finalize_batch(const BasinDev *__restrict__ basins, int nb, un
 // ---- launchers (host side, C++ linkage inside the library) ------------------
 // the stage kernel for a limiter; the recording variant only when the launch
 // records (the residual probe never does), the uniform-row variant for grids
-// whose rows are all alike when it does not, and of that at stage 2 the lean
-// one when the launch has none of the per-cell Manning field, the sponge and the
-// theta / residual outputs.  (A captured step graph holds the kernel chosen at
+// whose rows are all alike when it does not, and at stage 2 the lean one of
+// either when the launch has none of the per-cell Manning field, the sponge and
+// the theta / residual outputs.  (A captured step graph holds the kernel chosen at
 // capture: every setter that changes one of these conditions drops the graphs.)
 template <int MODE, int LIM>
 static const void *sweep_fn(bool rec, bool uni, bool lean) {
   if (MODE != MODE_PROBE && rec) return reinterpret_cast<const void *>(&k_sweep<MODE, LIM, MODE != MODE_PROBE, false>);
-  if (MODE == MODE_STAGE2 && uni && lean) return reinterpret_cast<const void *>(&k_sweep<MODE, LIM, false, true, true>);
+  if (MODE == MODE_STAGE2 && lean)
+    return uni ? reinterpret_cast<const void *>(&k_sweep<MODE, LIM, false, true, true>)
+               : reinterpret_cast<const void *>(&k_sweep<MODE, LIM, false, false, true>);
   if (uni) return reinterpret_cast<const void *>(&k_sweep<MODE, LIM, false, true>);
   return reinterpret_cast<const void *>(&k_sweep<MODE, LIM, false, false>);
 }

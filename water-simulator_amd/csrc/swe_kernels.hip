@@ -460,8 +460,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
         }
         if (!(__builtin_isfinite(nh) && __builtin_isfinite(nhu) && __builtin_isfinite(nhv))) bad = true;
         outp[k] = nh; outp[k + fs] = nhu; outp[k + 2 * fs] = nhv;
-        if (!LEAN && A.P.want_theta) A.theta[k] = thPrev;
-        if (!LEAN && A.P.want_rh) { A.rh[k] = rh; A.rh[k + fs] = rhu; A.rh[k + 2 * fs] = rhv; }
+        // (an adjoint replay, REC, never asks for theta or the residual)
+        if (!LEAN && !REC && A.P.want_theta) A.theta[k] = thPrev;
+        if (!LEAN && !REC && A.P.want_rh) { A.rh[k] = rh; A.rh[k + fs] = rhu; A.rh[k + 2 * fs] = rhv; }
       }
     }
 

@@ -164,6 +164,11 @@ __device__ __forceinline__ FluxF face_flux(const Face4 &L, const Face4 &R, const
 // many bodies a kernel instantiates): the last four rows' (b, h, hu, hv) of its
 // columns, which the cell update of row r-3 reads instead of memory
 __shared__ double sweep_ring[4][4][SWEEP_LANES];
+// sweep_tide: the wavefront's columns' equilibrium-tide factors of the stage,
+// per constituent (ts cl + tc sl, tc cl - ts sl): functions of the column and
+// the stage time only, formed once before the rows are marched (per-row-metric
+// bodies only; a grid with uniform rows has no tide)
+__shared__ double sweep_tide[MAX_TIDE][2][SWEEP_LANES];
 template <int MODE, bool REC, int LIM, bool UNI = false, bool LEAN = false>
 __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int chunk, int wave_id) {
   double (&ring)[4][4][SWEEP_LANES] = sweep_ring;
@@ -229,6 +234,21 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
     }
     return c;
   };
+
+  if constexpr (!UNI) {
+    for (int c = 0; c < P.tide_n; ++c) {
+      double X = 0.0, Y = 0.0;
+      if (colIn) {
+        const double *T = P.tide + (size_t)c * (2 * H + 2 * W);
+        const double cl = T[2 * H + gi], sl = T[2 * H + W + gi];
+        const double tc = tcs[2 * c], ts = tcs[2 * c + 1];
+        X = ts * cl + tc * sl;
+        Y = tc * cl - ts * sl;
+      }
+      sweep_tide[c][0][lane] = X;
+      sweep_tide[c][1][lane] = Y;
+    }
+  }
 
   Row cM = load_row(y0 - 3);
   Row c0 = load_row(y0 - 2);
@@ -390,12 +410,13 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, int strip, int ch
         }
         // equilibrium tide: g h grad(eta_eq), eta_eq = sum A cos^2(phi) cos(w t + 2 lon + ph)
         // (semidiurnal) or A sin(2 phi) cos(w t + lon + ph) (diurnal); no reference counterpart
+        // (the column factors from sweep_tide, formed before the row loop; the
+        // row factors, wave-uniform, through the constant address space as the
+        // row metric is read)
         for (int c = 0; c < (UNI ? 0 : P.tide_n); ++c) {
-          const double *T = P.tide + (size_t)c * (2 * H + 2 * W);
-          const double cl = T[2 * H + gi], sl = T[2 * H + W + gi];
-          const double tc = tcs[2 * c], ts = tcs[2 * c + 1];
-          rhu += cur.h * T[o] * (ts * cl + tc * sl);
-          rhv += cur.h * T[H + o] * (tc * cl - ts * sl);
+          const const_double *T = (const const_double *)P.tide + (size_t)c * (2 * H + 2 * W);
+          rhu += cur.h * T[o] * sweep_tide[c][0][lane];
+          rhv += cur.h * T[H + o] * sweep_tide[c][1][lane];
         }
       }
       if (MODE == MODE_PROBE) {

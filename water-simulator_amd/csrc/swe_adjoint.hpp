@@ -435,6 +435,20 @@ __host__ __device__ inline void adj_faces_rev_at(const Params &P, const double *
   for (int q = 0; q < 12; ++q) w.s[S_LXETAL + q][k] = r[q];
 }
 
+// T[idx] for an idx that is the same on every active lane: on the device through
+// the constant address space with the index taken from the first active lane (a
+// scalar load, waited on apart from the vector loads).  The device callers of
+// cells_rev run one grid row per wavefront (the tiled gather pass), so a row's
+// index qualifies.
+__host__ __device__ inline double row_uniform(const double *T, int idx) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef const double __attribute__((address_space(4))) const_double;
+  return ((const const_double *)T)[__builtin_amdgcn_readfirstlane(idx)];
+#else
+  return T[idx];
+#endif
+}
+
 // pass 6: per cell, gather the face-state adjoints into eta, U, V and the
 // slopes; the bed (and spherical geometric) term^T and Coriolis^T
 // (swe.mjs:570-628)
@@ -487,12 +501,13 @@ __host__ __device__ inline CellRev cells_rev(const Params &P, const StateP &S, c
       lhvC = P.coriolis * lrhu; lhuC = -P.coriolis * lrhv;
     }
     // equilibrium tide, linear in h: rhu += h X sin(theta), rhv += h Y cos(theta);
-    // tcs holds (cos, sin)(omega t_s + phase) of the stage
+    // tcs holds (cos, sin)(omega t_s + phase) of the stage; the row factors and
+    // tcs are the same on every lane of a device caller's wavefront (row_uniform)
     for (int c = 0; c < P.tide_n; ++c) {
       const double *T = P.tide + (size_t)c * (2 * P.H + 2 * W);
       const double cl = T[2 * P.H + i], sl = T[2 * P.H + W + i];
-      const double tc = tcs[2 * c], ts = tcs[2 * c + 1];
-      leta += lrhu * T[j] * (ts * cl + tc * sl) + lrhv * T[P.H + j] * (tc * cl - ts * sl);
+      const double tc = row_uniform(tcs, 2 * c), ts = row_uniform(tcs, 2 * c + 1);
+      leta += lrhu * row_uniform(T, j) * (ts * cl + tc * sl) + lrhv * row_uniform(T, P.H + j) * (tc * cl - ts * sl);
     }
   }
   CellRev r;

@@ -215,6 +215,21 @@ enum Slot {
 };
 struct Work { double *s[S_COUNT]; };
 
+// T[idx] for an idx that is the same on every active lane: on the device through
+// the constant address space with the index taken from the first active lane (a
+// scalar load, waited on apart from the vector loads).  The device callers of
+// adj_faces_rev_eval and cells_rev run one grid row per wavefront (the tiled
+// passes' own cells), so a row's index qualifies.
+__host__ __device__ inline double row_uniform(const double *T, int idx) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef const double __attribute__((address_space(4))) const_double;
+  return ((const const_double *)T)[__builtin_amdgcn_readfirstlane(idx)];
+#else
+  return T[idx];
+#endif
+}
+
+
 __host__ __device__ inline bool xface_on(const Params &P, int i, int j) {
   return j >= 2 && j <= P.H - 3 && i >= 1 && i <= P.W - 3;
 }
@@ -406,8 +421,9 @@ __host__ __device__ inline void adj_faces_rev_eval(const Params &P, const double
     // everything loaded before the branch (cell k stands in for a face off the grid)
     const bool on = d == 0 ? xface_on(P, i, j) : yface_on(P, i, j);
     const int kR = on ? (d == 0 ? k + 1 : k + P.W) : k;
-    const double divL = d == 0 ? P.geo.dx[j] : P.geo.dyN[j];
-    const double divR = d == 0 ? P.geo.dx[j] : P.geo.dyS[on ? j + 1 : j];
+    // (dyS of row j + 1 is read only for a face on the grid, j + 1 <= H - 2)
+    const double divL = d == 0 ? row_uniform(P.geo.dx, j) : row_uniform(P.geo.dyN, j);
+    const double divR = d == 0 ? row_uniform(P.geo.dx, j) : row_uniform(P.geo.dyS, j + 1 < P.H ? j + 1 : j);
     const double M = w.s[d == 0 ? S_FXM : S_FYM][k];
     double lM = lf[4 * d];
     const double loutL = loutK, loutR = d == 0 ? loutE : loutN;
@@ -433,20 +449,6 @@ __host__ __device__ inline void adj_faces_rev_at(const Params &P, const double *
   const int kE = xface_on(P, i, j) ? k + 1 : k, kN = yface_on(P, i, j) ? k + P.W : k;
   adj_faces_rev_eval(P, b, w, i, j, lf, lout[k], lout[kE], lout[kN], r);
   for (int q = 0; q < 12; ++q) w.s[S_LXETAL + q][k] = r[q];
-}
-
-// T[idx] for an idx that is the same on every active lane: on the device through
-// the constant address space with the index taken from the first active lane (a
-// scalar load, waited on apart from the vector loads).  The device callers of
-// cells_rev run one grid row per wavefront (the tiled gather pass), so a row's
-// index qualifies.
-__host__ __device__ inline double row_uniform(const double *T, int idx) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  typedef const double __attribute__((address_space(4))) const_double;
-  return ((const const_double *)T)[__builtin_amdgcn_readfirstlane(idx)];
-#else
-  return T[idx];
-#endif
 }
 
 // pass 6: per cell, gather the face-state adjoints into eta, U, V and the
@@ -480,21 +482,21 @@ __host__ __device__ inline CellRev cells_rev(const Params &P, const StateP &S, c
       for (int d = 0; d < 2; ++d) {
         const double sE = w.s[S_SEX + d][k], sB = w.s[S_SBX + d][k];
         const double aP = eta + 0.5 * sE - (bk + 0.5 * sB), aM = eta - 0.5 * sE - (bk - 0.5 * sB);
-        const double div = d == 0 ? P.geo.dx[j] : P.geo.bedPhi[j];
+        const double div = d == 0 ? row_uniform(P.geo.dx, j) : row_uniform(P.geo.bedPhi, j);
         const double lt = d == 0 ? lrhu : lrhv;
         // term = (-G/2) (hP + hM) sB / div; on a sphere rhv also gets gc G (hP^2 + hM^2)
         double lhP = lt * P.neg_half_g * sB / div, lhM = lhP;
         if (d == 1 && P.sphere) {
-          lhP += lrhv * P.geo.geo[j] * P.g * 2.0 * jmax0(aP);
-          lhM += lrhv * P.geo.geo[j] * P.g * 2.0 * jmax0(aM);
+          lhP += lrhv * row_uniform(P.geo.geo, j) * P.g * 2.0 * jmax0(aP);
+          lhM += lrhv * row_uniform(P.geo.geo, j) * P.g * 2.0 * jmax0(aM);
         }
         if (aP > 0) { leta += lhP; lsE[d] += 0.5 * lhP; }
         if (aM > 0) { leta += lhM; lsE[d] -= 0.5 * lhM; }
       }
     }
     if (P.sphere) {
-      const double tR = P.geo.tanPhi[j] / P.R;
-      const double fe = P.geo.fRow[j] + w.s[S_U][k] * tR;
+      const double tR = row_uniform(P.geo.tanPhi, j) / P.R;
+      const double fe = row_uniform(P.geo.fRow, j) + w.s[S_U][k] * tR;
       lhvC = fe * lrhu; lhuC = -fe * lrhv;
       lU += tR * (S.hv[k] * lrhu - S.hu[k] * lrhv);
     } else if (P.coriolis_on) {

@@ -420,34 +420,43 @@ const double *tcs_of(swe_adj *a, int buf, int64_t q) { return a->tcs_seg[buf] + 
 // output (BC applied in place), stage 2's dryCleaned pre-friction state, and its
 // output straight into slot q + 1.  The replay touches only the store, the
 // control block and the context's bed, never what the reverse sweep reads.
-int replay_segment(swe_adj *a, const double *src, int64_t n0, int64_t n1, int buf) {
+// The replay of a segment [n0, n1) into store buf on the replay stream, in two
+// parts so that its steps can be issued between other work: the start state,
+// then step n (in order, n0 first).
+int segment_begin(swe_adj *a, const double *src, int buf) {
+  return copy3_on(a, gslot(a, buf, 0, G_AP), sp3(src, a->N), a->rs);
+}
+int segment_step(swe_adj *a, int64_t n0, int64_t n, int64_t n1, int buf) {
   swe_ctx *c = a->ctx;
   const Params &P = c->P;
   hipStream_t st = a->rs;
   const int np = sweep_waves(P, c->chunk);
-  if (int rc = copy3_on(a, gslot(a, buf, 0, G_AP), sp3(src, a->N), st)) return rc;
-  for (int64_t n = n0; n < n1; ++n) {
-    const int64_t q = n - n0;
-    const StateW Ap = gslot(a, buf, q, G_AP), Bp = gslot(a, buf, q, G_BP), dc = gslot(a, buf, q, G_DC2);
-    const StateW next = n + 1 < n1 ? gslot(a, buf, q + 1, G_AP) : tslot(a, T_CUR);
-    hipLaunchKernelGGL(k_ctl_dt, dim3(1), dim3(64), 0, st, c->dctl, a->dt_dev, a->t_dev, a->ext_dev, n,
-                       a->tcs_seg[buf] + (size_t)q * 2 * MAX_TIDE * 2);
-    ACHK(a, launch_apply_bc(P, Ap, c->b, c->dctl, st));
-    SweepArgs s1 = ctx_base_args(c);
-    s1.in = sp(Ap);
-    s1.out = Bp;
-    s1.recf = recslot(a, buf, q, 1);
-    ACHK(a, launch_sweep(MODE_STAGE1, s1, st));
-    ACHK(a, launch_apply_bc(P, Bp, c->b, c->dctl, st));
-    SweepArgs s2 = ctx_base_args(c);
-    s2.in = sp(Bp);
-    s2.h0 = sp(Ap);
-    s2.out = next;
-    s2.rec = dc.h;
-    s2.recf = recslot(a, buf, q, 2);
-    s2.mf_part = c->mf_part + np;
-    ACHK(a, launch_sweep(MODE_STAGE2, s2, st));
-  }
+  const int64_t q = n - n0;
+  const StateW Ap = gslot(a, buf, q, G_AP), Bp = gslot(a, buf, q, G_BP), dc = gslot(a, buf, q, G_DC2);
+  const StateW next = n + 1 < n1 ? gslot(a, buf, q + 1, G_AP) : tslot(a, T_CUR);
+  hipLaunchKernelGGL(k_ctl_dt, dim3(1), dim3(64), 0, st, c->dctl, a->dt_dev, a->t_dev, a->ext_dev, n,
+                     a->tcs_seg[buf] + (size_t)q * 2 * MAX_TIDE * 2);
+  ACHK(a, launch_apply_bc(P, Ap, c->b, c->dctl, st));
+  SweepArgs s1 = ctx_base_args(c);
+  s1.in = sp(Ap);
+  s1.out = Bp;
+  s1.recf = recslot(a, buf, q, 1);
+  ACHK(a, launch_sweep(MODE_STAGE1, s1, st));
+  ACHK(a, launch_apply_bc(P, Bp, c->b, c->dctl, st));
+  SweepArgs s2 = ctx_base_args(c);
+  s2.in = sp(Bp);
+  s2.h0 = sp(Ap);
+  s2.out = next;
+  s2.rec = dc.h;
+  s2.recf = recslot(a, buf, q, 2);
+  s2.mf_part = c->mf_part + np;
+  ACHK(a, launch_sweep(MODE_STAGE2, s2, st));
+  return SWE_OK;
+}
+int replay_segment(swe_adj *a, const double *src, int64_t n0, int64_t n1, int buf) {
+  if (int rc = segment_begin(a, src, buf)) return rc;
+  for (int64_t n = n0; n < n1; ++n)
+    if (int rc = segment_step(a, n0, n, n1, buf)) return rc;
   return SWE_OK;
 }
 
@@ -849,12 +858,16 @@ int swe_adj_backward(swe_adj *a, double *grad_h0, double *grad_n) {
     for (int64_t m = (o1 - o0 + a->K - 1) / a->K - 1; m >= 0; --m)
       segs.push_back({so, m, o0 + m * a->K, std::min<int64_t>(o1, o0 + (m + 1) * a->K)});
   }
+  // a replay that starts an outer segment first recomputes its inner checkpoints
+  auto new_outer = [&](size_t i) { return two && (i == 0 || segs[i - 1].so != segs[i].so); };
+  auto src_of = [&](size_t i) {
+    return two ? a->ickpt + (size_t)segs[i].m * N3 : a->ckpt + (size_t)segs[i].so * N3;
+  };
   auto issue_replay = [&](size_t i) -> int {
     const Seg &g = segs[i];
-    if (two && (i == 0 || segs[i - 1].so != g.so))
+    if (new_outer(i))
       if (int r = inner_checkpoints(a, g.so)) return r;
-    const double *src = two ? a->ickpt + (size_t)g.m * N3 : a->ckpt + (size_t)g.so * N3;
-    return replay_segment(a, src, g.n0, g.n1, (int)(i % 2));
+    return replay_segment(a, src_of(i), g.n0, g.n1, (int)(i % 2));
   };
   for (int q = 0; q < 2; ++q) ACHK(a, hipEventRecord(a->reversed[q], st));
   ACHK(a, hipStreamWaitEvent(a->rs, a->reversed[0], 0));
@@ -865,16 +878,34 @@ int swe_adj_backward(swe_adj *a, double *grad_h0, double *grad_n) {
     if (pstride && i > 0 && (int64_t)i % pstride == 0) progress(st, "backward segments", (int64_t)i, (int64_t)segs.size());
     const int buf = (int)(i % 2);
     ACHK(a, hipStreamWaitEvent(st, a->replayed[buf], 0));
+    // The replay of the next segment into the other store runs beside this
+    // segment's reverse passes, once the segment before this one (which read
+    // that store) is reversed.  Its steps are issued one after each reverse
+    // step, so that it starts with this segment's first reverse pass and
+    // neither stream waits while the host issues the other's launches.  One
+    // that first recomputes inner checkpoints (many steps) is issued whole,
+    // after this segment's reverse passes.
+    const bool next = i + 1 < segs.size();
+    const bool split = next && !new_outer(i + 1);
+    int64_t rn = next ? segs[i + 1].n0 : 0;   // the next segment's first step not yet issued
+    if (split) {
+      ACHK(a, hipStreamWaitEvent(a->rs, a->reversed[1 - buf], 0));
+      if ((rc = segment_begin(a, src_of(i + 1), 1 - buf))) return rc;
+    }
     for (int64_t n = segs[i].n1 - 1; n >= segs[i].n0; --n) {
       if ((rc = step_rev(a, buf, n - segs[i].n0, n))) return rc;
       if ((rc = inject(a, n))) return rc;
+      if (split && rn < segs[i + 1].n1) {
+        if ((rc = segment_step(a, segs[i + 1].n0, rn, segs[i + 1].n1, 1 - buf))) return rc;
+        ++rn;
+      }
     }
     ACHK(a, hipEventRecord(a->reversed[buf], st));
-    // the replay of the next segment into the other store, once the segment
-    // before this one (which read that store) is reversed: it runs beside this
-    // segment's reverse passes.  Issued after them, so that the time the host
-    // takes to issue it (or to wait on the event) never holds them back.
-    if (i + 1 < segs.size()) {
+    if (split) {
+      for (; rn < segs[i + 1].n1; ++rn)
+        if ((rc = segment_step(a, segs[i + 1].n0, rn, segs[i + 1].n1, 1 - buf))) return rc;
+      ACHK(a, hipEventRecord(a->replayed[1 - buf], a->rs));
+    } else if (next) {
       ACHK(a, hipStreamWaitEvent(a->rs, a->reversed[1 - buf], 0));
       if ((rc = issue_replay(i + 1))) return rc;
       ACHK(a, hipEventRecord(a->replayed[1 - buf], a->rs));

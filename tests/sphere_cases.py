@@ -12,13 +12,13 @@ import numpy as np
 import swe_amd
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-BASELINE = os.path.join(HERE, "golden", "sphere_baseline.json")
+SPHERE_BASELINE_JSON = os.path.join(HERE, "golden", "sphere_baseline.json")
 G_REF, R_REF, H_OCEAN = 9.80665, 6371e3, 4000.0
 D = math.pi / 180
 
 
 def baseline():
-    with open(BASELINE) as f:
+    with open(SPHERE_BASELINE_JSON) as f:
         return json.load(f)
 
 
